@@ -1,0 +1,132 @@
+#!/usr/bin/env python3
+"""Bandwidth benchmark of the fused D2Q9 stream-collide path on one MI355X.
+
+Workload: one periodic 16384 x 16384 block, rho = 1, (u, v) ~ N(0, 0.01^2)
+from a fixed seed, tau = 0.6; one step = one fused pull-stream + BGK launch
+over all nodes, F planes in HBM the whole time. MLUPS = nodes * steps / wall
+seconds. The roofline figure uses the algorithmic 72 B per node update of
+FP32 planes (9 loads + 9 stores of 4 B) against the 8 TB/s HBM peak, with the
+per-launch time taken from HIP events on the library's stream.
+
+The CPU baseline times the FP64 oracle (three passes per step, OpenMP) on a
+bounded sample of the same workload (a smaller periodic block, a few steps).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "multi-block-lbm-gpu_amd"))
+
+METRIC = "MLUPS & %HBM roofline; time-to-converged Cd (±0.5%) vs plain FP32"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = {0: 72, 1: 72, 2: 36, 3: 36}
+STORAGE_NAMES = {0: "f32", 1: "f32s", 2: "f16s", 3: "bf16s"}
+
+
+def traffic_from_profile(storage):
+    """HBM bytes per launch measured with rocprofv3 PMC counters, if recorded."""
+    path = os.path.join(REPO, "profiles", f"traffic_periodic16384_{STORAGE_NAMES[storage]}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("bytes_per_launch")
+
+
+def gpu_run(lib, n, steps, warmup, storage, tuning=None):
+    import mbl
+
+    blk = mbl.PeriodicBlock(lib, n, n, tau=0.6, storage=storage, tuning=tuning)
+    blk.step(warmup)
+    blk.sync()
+    t0 = time.perf_counter()
+    ev_ms = blk.timed_steps(steps)  # events bracket the K launches; ends with a synchronize
+    wall = time.perf_counter() - t0
+    blk.close()
+    nodes = n * n
+    return {"wall_s": wall, "event_ms": ev_ms, "per_launch_ms": ev_ms / steps,
+            "mlups": nodes * steps / wall / 1e6, "nodes": nodes}
+
+
+def cpu_baseline(target_s):
+    """FP64 oracle on a 4096 x 4096 periodic sample, steps sized to ~target_s seconds."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    n = 4096
+    o = oracle.OraclePeriodic(n, n, tau=0.6)
+    t1 = o.run(1)
+    steps = max(1, min(50, int(target_s / max(t1, 1e-3))))
+    secs = o.run(steps)
+    threads = o.lib.orc_threads()
+    o.close()
+    return {"value": round(n * n * steps / secs / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"periodic {n}x{n}, tau=0.6, FP64 oracle (stb1/stb2/stb3 passes, OpenMP), {steps} steps, "
+                      f"{secs:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--storage", type=int, default=0, help="0 f32, 1 f32 shifted, 2 f16 shifted, 3 bf16 shifted")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variants", action="store_true", help="also time the 16-bit storage modes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design, not a run)", file=sys.stderr)
+        sys.exit(2)
+    import mbl
+
+    lib = mbl.load_library(False)
+    print(f"# bench: periodic {args.n}^2 storage={STORAGE_NAMES[args.storage]} steps={args.steps}", file=sys.stderr,
+          flush=True)
+    r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage)
+    bpn = BYTES_PER_NODE[args.storage]
+    achieved = bpn * r["nodes"] / (r["per_launch_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["wall_s"] / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random velocity field)",
+        "config": {"workload": f"periodic {args.n}x{args.n} D2Q9 BGK block, tau=0.6, "
+                               f"{STORAGE_NAMES[args.storage]} planes, {bpn} B/node/step",
+                   "nodes": r["nodes"], "storage": STORAGE_NAMES[args.storage]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.storage),
+                     "kernel": "k_stream_collide (periodic)", "per_launch_ms": round(r["per_launch_ms"], 5),
+                     "bytes_per_node": bpn},
+    }
+    if args.variants:
+        var = {}
+        for st in (1, 2, 3):
+            v = gpu_run(lib, args.n, max(50, args.steps // 4), args.warmup, st)
+            b = BYTES_PER_NODE[st]
+            var[STORAGE_NAMES[st]] = {"mlups": round(v["mlups"], 1), "per_launch_ms": round(v["per_launch_ms"], 5),
+                                      "achieved_gbs": round(b * v["nodes"] / (v["per_launch_ms"] * 1e-3) / 1e9, 1),
+                                      "bytes_per_node": b}
+        out["storage_variants"] = var
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

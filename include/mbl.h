@@ -1,0 +1,364 @@
+/*
+ * mbl.h -- C ABI of libmbl_hip, the MI355X multi-block D2Q9 BGK lattice
+ * Boltzmann time-stepping library.
+ *
+ * Pure C. No HIP or CUDA header is included; every device buffer is a plain
+ * pointer obtained from mbl_malloc() (or any HIP allocation of the same
+ * process). Two builds of the same source exist:
+ *
+ *   libmbl_hip.so    Lbd = float   (FP32 state; FP32/FP16/BF16 plane storage)
+ *   libmbl_hip64.so  Lbd = double  (FP64 state and planes)
+ *
+ * A caller includes this header with MBL_FP64 defined when it links the FP64
+ * build. Both builds export exactly the same symbol names.
+ *
+ * The drop-in entry points keep the names and argument order of the
+ * reference program (zhangya9741/Multi-block-LBM-GPU):
+ *
+ *   init_General   lbm/init_block.h:2         (launched at Multi-block-LBM.cu:117)
+ *   init_Fstore    multi/init_multi.h:3       (Multi-block-LBM.cu:138)
+ *   init_tri_A     multi/init_multi.h:30      (Multi-block-LBM.cu:139)
+ *   evolution      evolution.h:2-3            (Multi-block-LBM.cu:151-152)
+ *   sum_N_drag     lbm/force_block.h:50       (Multi-block-LBM.cu:127)
+ *   fdrag_store    lbm/force_block.h:78       (Multi-block-LBM.cu:131)
+ *   getDrag_block  lbm/force_block.h:180      (Multi-block-LBM.cu:157)
+ *   error_L2       lbm/error_L2.h:57-58       (Multi-block-LBM.cu:155)
+ *
+ * Differences at the boundary (each one deliberate):
+ *   - `dim3` becomes the POD mbl_dim3; launch shapes passed in are accepted
+ *     and ignored (the library picks its own).
+ *   - `cusparseHandle_t` becomes mbl_handle_t, an opaque context carrying the
+ *     stream, the graph cache, the physical constants that were __constant__
+ *     globals (parameters.h:42-52) and the level tables (mi, LEVEL,
+ *     number_line2) that evolution.h reads as globals.  Entry points of the
+ *     reference that take no handle use the "current" handle (the last one
+ *     created or made current), mirroring the reference's global constants.
+ *   - AREA's real members are double in both builds (setup-time values).
+ *   - Drag and residual outputs are double in both builds (FP64 accumulation).
+ *   - State layout: F and f of a D2Q9 block are a ping-pong pair. After every
+ *     evolution() call F holds the post-collision state and f the state one
+ *     (sub)step older: evolution() swaps the two pointer sets in the caller's
+ *     pps_d[] entries whenever a level took an odd number of substeps. The
+ *     post-stream distribution of the reference (f after stb1) is implicit:
+ *     f_post[k][x] = pps_d.f[k][x - e_k].
+ *   - Diagnostics (error_L2.dat, fdrag_t.dat lines) are reduced on the device
+ *     and written in batches (mbl_flush_diagnostics, automatic every
+ *     flush_interval records and at handle destruction); the bytes written are
+ *     the reference formats "%d\t%g\n" and "%d\t%g\t%g\t\n".
+ *
+ * Error convention: functions returning int return 0 on success and a
+ * negative MBL_E* code on failure; void entry points record the code in the
+ * handle (sticky, read with mbl_get_last_error).
+ */
+#ifndef MBL_H
+#define MBL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifdef MBL_FP64
+typedef double Lbd;
+#else
+typedef float Lbd;
+#endif
+typedef int Lbi;
+typedef unsigned int Lbui;
+
+#define MBL_Q 9
+#define MBL_MAX_LEVEL 8
+
+/* Node flags. FLUID, SOLID_SBB, BOUNDARY_INLET, BOUNDARY_OUTLET and
+ * C_BOUNDARY keep the reference's values (lbm/lbm.h:26-32). The reference
+ * declares INLET/OUTLET without using them; here they are the Dirichlet
+ * equilibrium inlet (parabolic profile) and the zero-gradient outlet of the
+ * channel boundary mode. MBL_SOLID_WALL (new) is a full-way bounce-back node
+ * that does not take part in the drag sum. */
+#define FLUID 0
+#define SOLID_SBB 1
+#define MBL_SOLID_WALL 2
+#define BOUNDARY_INLET 5
+#define BOUNDARY_OUTLET 7
+#define C_BOUNDARY 9
+
+/* ---- data model (lbm/data_type_lbm.h:7-51, multi/data_type_multi.h:7-31) ---- */
+
+/* Per-block SoA state. Member order and contiguity are ABI: the 25 real
+ * pointers starting at &x may be walked as an array (Multi-block-LBM.cu:31-37).
+ * Planes are nx*ny elements, row-major, ixy = i + j*nx. With 16-bit plane
+ * storage (mbl_config.storage = MBL_STORE_F16S / MBL_STORE_BF16S) the 18
+ * f/F pointers address 2-byte elements holding f - w_k; with MBL_STORE_F32S
+ * they hold f - w_k in float. */
+typedef struct D2Q9 {
+    Lbi *solid;
+    Lbd *x, *y;
+    Lbd *rho, *ux, *uy;
+    Lbd *forcex, *forcey;
+    Lbd *f[MBL_Q];
+    Lbd *F[MBL_Q];
+} D2Q9;
+
+typedef struct P4 { Lbi xstart, xend, ystart, yend; } P4;
+typedef struct P3i { Lbi nc, nf, direction; } P3i;
+typedef struct P4d { double xstart, xend, ystart, yend; } P4d;
+typedef struct P4d4i { P4d reduced; P4 move; } P4d4i;
+typedef struct Int3 { Lbi i, j, k; } Int3;
+
+typedef struct AREA {
+    Lbi m, nx, ny;
+    double base_x, base_y, tau, rv_tau;
+} AREA;
+
+/* Coarse/fine (or same-level) interface line: a line is vertical when
+ * xstart == xend, else horizontal. */
+typedef struct Inf_Ff_store {
+    Lbi areac_n;
+    Lbi Nc;
+    P4 line_C;
+    Lbi areaf_n;
+    Lbi Nf;
+    P4 line_F;
+} Inf_Ff_store;
+
+typedef struct F_k3 { Lbd *F[MBL_Q]; } F_k3;
+
+/* 3-slot time ring of the fine-edge distributions; the slots are walked as
+ * an array from &f_t1 and selected through marker[] (host state). */
+typedef struct Ff_Store {
+    Lbi N;
+    Lbi marker[3];
+    F_k3 f_t1, f_t, f_t_1;
+} Ff_Store;
+
+/* Spline work arrays of one coarse line (Nc nodes). The library's spline
+ * kernel keeps its whole solve in LDS; l/m/u are filled by init_tri_A for
+ * callers that inspect them, Varibs_M receives the second-derivative columns
+ * (column-major, 12 columns of N) and rho..F[8] the gathered line, so the
+ * arrays hold the same values as after the reference's gtsv2 call. */
+typedef struct FM_spline3 {
+    Lbi N;
+    Lbd *l, *m, *u;
+    Lbd *Varibs_M;
+    Lbd *rho, *ux, *uy;
+    Lbd *F[MBL_Q];
+} FM_spline3;
+
+typedef struct mbl_dim3 { unsigned x, y, z; } mbl_dim3;
+
+typedef struct mbl_context *mbl_handle_t;
+
+/* ---- configuration ---- */
+
+enum {
+    MBL_STORE_NATIVE = 0, /* Lbd planes holding f (FP32 build: float, FP64 build: double) */
+    MBL_STORE_F32S = 1,   /* float planes holding f - w_k             (FP32 build only) */
+    MBL_STORE_F16S = 2,   /* IEEE binary16 planes holding f - w_k      (FP32 build only) */
+    MBL_STORE_BF16S = 3   /* bfloat16 planes holding f - w_k           (FP32 build only) */
+};
+
+enum {
+    MBL_BC_REFERENCE = 0, /* equilibrium (rho0, U0) on every outer domain edge, as the reference */
+    MBL_BC_CHANNEL = 1,   /* walls y=0, y=LY-1; parabolic equilibrium inlet x=0; zero-gradient outlet x=LX-1 */
+    MBL_BC_PERIODIC = 2   /* single block, periodic in x and y */
+};
+
+enum {
+    MBL_INIT_UNIFORM = 0, /* rho0, U0 everywhere (reference init_General) */
+    MBL_INIT_REST = 1,    /* rho0, u = 0 */
+    MBL_INIT_RANDOM = 2   /* rho0, (u, v) ~ N(0, sigma^2) from a counter-based generator (seed) */
+};
+
+typedef struct mbl_config {
+    /* level tables read as globals by evolution.h:5-34 */
+    int LEVEL;
+    int mi[MBL_MAX_LEVEL];
+    int number_line2;
+    /* physical constants (parameters.h:24-52) */
+    double rho0;
+    double U0[2];
+    int LX, LY;
+    double R, CX, CY;
+    /* boundary, initial state and storage modes */
+    int bc_mode;
+    int init_mode;
+    int storage;
+    unsigned long long seed;
+    double sigma;
+    /* execution */
+    int use_graph;      /* capture each refinement-cycle phase as a hipGraph (default 1) */
+    int sync_diag;      /* 1: error_L2/getDrag_block synchronise and write at once (reference timing) */
+    int flush_interval; /* records buffered on the device before a file flush (default 1000) */
+    int write_files;    /* 1: append error_L2.dat / fdrag_t.dat in out_dir (default 1) */
+    char out_dir[256];  /* directory of the .dat files ("" = current directory) */
+} mbl_config;
+
+/* Fills a config with the reference defaults (parameters.h, _GEO_2_5 levels). */
+void mbl_config_default(mbl_config *cfg);
+
+int mbl_handle_create(mbl_handle_t *h, const mbl_config *cfg);
+void mbl_handle_destroy(mbl_handle_t h);
+void mbl_handle_make_current(mbl_handle_t h);
+mbl_handle_t mbl_handle_current(void);
+int mbl_get_last_error(mbl_handle_t h);
+const char *mbl_error_string(int code);
+int mbl_synchronize(mbl_handle_t h);
+int mbl_flush_diagnostics(mbl_handle_t h);
+/* Runs the library's work on an external HIP stream (0 = the handle's own). */
+int mbl_set_stream(mbl_handle_t h, void *hip_stream);
+
+enum {
+    MBL_OK = 0,
+    MBL_E_INVALID = -1,  /* invalid argument or geometry */
+    MBL_E_HIP = -2,      /* HIP runtime error */
+    MBL_E_NOMEM = -3,    /* allocation failed */
+    MBL_E_NOT_FINITE = -4, /* non-finite residual or force */
+    MBL_E_DIVERGED = -5, /* density <= 0 detected */
+    MBL_E_NO_HANDLE = -6,
+    MBL_E_UNSUPPORTED = -7
+};
+
+/* ---- memory helpers (the C host never includes HIP headers) ---- */
+void *mbl_malloc(size_t bytes);
+void mbl_free(void *p);
+int mbl_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int mbl_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int mbl_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int mbl_memset_d(void *dst, int value, size_t bytes);
+/* Bytes of one f/F plane of nx*ny nodes in the handle's storage mode. */
+size_t mbl_plane_bytes(mbl_handle_t h, int nx, int ny);
+/* Allocates every plane of a block (solid, 25 state planes in the handle's
+ * storage) and the two previous-velocity planes (may be NULL). */
+int mbl_block_alloc(mbl_handle_t h, const AREA *area, D2Q9 *pp, Lbd **upx, Lbd **upy);
+void mbl_block_free(D2Q9 *pp, Lbd *upx, Lbd *upy);
+/* Reads plane k of F (which=0) or f (which=1) of a block as f values in double,
+ * decoding the storage mode. */
+int mbl_read_distribution(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, double *out);
+/* Writes f values (double) into plane k of F (which=0) or f (which=1). */
+int mbl_write_distribution(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, const double *in);
+
+/* ---- geometry (geo/geometry.h, geo/init_geo.h) ---- */
+
+/* Runtime form of the compile-time tables of geo/geometry.h. AreaDefine may be
+ * given directly (integer coarse coordinates); otherwise it is derived from
+ * AreaZone as (int)(frac*(L-1)) + move (init_geo.h:91-97). */
+typedef struct mbl_geometry {
+    int LEVEL;
+    int mi[MBL_MAX_LEVEL];
+    int number_area, number_line, number_line2;
+    int ASP[MBL_MAX_LEVEL + 1];
+    int LTCFFCP[MBL_MAX_LEVEL + 1];
+    int LS3CFFCP[MBL_MAX_LEVEL + 1];
+    int LCCP[MBL_MAX_LEVEL + 1];
+    const P4d4i *AreaZone;
+    const P4 *AreaDefine;
+    const P3i *NeborCF;
+    const P3i *NeborCC;
+    int LX, LY;
+    double Re, U0_2, R;
+} mbl_geometry;
+
+/* geo_init (init_geo.h:89-165) + Line_init (:17-86): fills AreaDefine_out,
+ * Areas, Line_Inf_C/F (number_line) and Line_Inf_CC (number_line2). Rejects
+ * layouts with tau == 1 on either side of a coarse/fine line (singular
+ * rescale) or Nf != r*(Nc-1)+1. */
+int mbl_geo_init(const mbl_geometry *g, P4 *AreaDefine_out, AREA *Areas,
+                 Inf_Ff_store *Line_Inf_C, Inf_Ff_store *Line_Inf_F, Inf_Ff_store *Line_Inf_CC);
+
+/* The layouts of geo/geometry.h as runtime data: "GEO_1_1", "GEO_2_2",
+ * "GEO_2_5", "GEO_4_9" with the parameters.h constants (LX=768, LY=320,
+ * R=10, Re=10, U0=0.1). Fills *g (pointing at static tables); 0 on success. */
+int mbl_geometry_preset(const char *name, mbl_geometry *g);
+
+/* Nested-refinement channel around a cylinder (no reference counterpart):
+ * level 0 spans the LX x LY node domain; region[l] (integer coarse
+ * coordinates, l = 1..levels-1) is refined by 2 per level, each region
+ * strictly inside the previous one with a margin of at least 2. Every level
+ * but the finest is split into top/bottom/left/right blocks around the next
+ * region, as _GEO_2_5 splits level 0. The tables are written into the caller's
+ * buffers (AreaDefine: 4*levels entries, NeborCF: 8*levels, NeborCC:
+ * 8*levels); g points at them. */
+typedef struct mbl_channel_spec {
+    int LX, LY;
+    int levels;
+    P4 region[MBL_MAX_LEVEL];
+    double Re, U, R;
+} mbl_channel_spec;
+int mbl_geometry_channel(const mbl_channel_spec *s, mbl_geometry *g, P4 *AreaDefine_buf, P3i *NeborCF_buf,
+                         P3i *NeborCC_buf);
+
+/* ---- drop-in entry points ---- */
+
+/* lbm/init_block.h:2-51. Coordinates, rho/u (init_mode), upx/upy, forces = 0,
+ * node flags (bc_mode), F = f = feq. */
+void init_General(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, Lbd *upx, Lbd *upy);
+
+/* multi/init_multi.h:3-28: the three ring slots <- the fine edge's F. */
+void init_Fstore(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, Inf_Ff_store line_Inf, Ff_Store line_t3);
+
+/* multi/init_multi.h:30-51: tridiag(0.5, 2, 0.5) with decoupled end rows. */
+void init_tri_A(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, Inf_Ff_store line_Inf, FM_spline3 line_S3);
+
+/* evolution.h:2-65: one step of `level` (level 0: one coarse step) with the
+ * whole recursive refinement cycle below it. `pps_h` is not used. */
+void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lccp, Lbi level,
+               mbl_dim3 block, mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d,
+               Inf_Ff_store *line_Inf_c, Inf_Ff_store *line_Inf_f, Inf_Ff_store *line_Inf_cc,
+               Ff_Store *line_t3, FM_spline3 *line_S3, mbl_handle_t handle);
+
+/* lbm/force_block.h:50-75: number of momentum-exchange links of a block. */
+Lbi sum_N_drag(mbl_dim3 block, mbl_dim3 grid, AREA area, D2Q9 pp);
+
+/* lbm/force_block.h:78-127: builds the (i, j, k) link list, i-major order,
+ * into d_pdrag. Only h_pp.solid is written (the reference copies all 26
+ * arrays; the other 25 are not needed). */
+void fdrag_store(AREA area, D2Q9 h_pp, Int3 *d_pdrag, D2Q9 d_pp, Lbi N);
+
+/* lbm/force_block.h:180-216: momentum-exchange force on the body from the
+ * post-stream distribution of the block's last substep; appends
+ * "t\tCd\tCl\t" to fdrag_t.dat. force[0..1] (may be NULL) receives Fx, Fy
+ * immediately when sync_diag is set, else at the next flush. */
+void getDrag_block(Lbi t, Int3 *d_pdrag, AREA area, D2Q9 d_pp, double *force, Lbi N);
+
+/* lbm/error_L2.h:57-91: relative temporal-change residual
+ * sqrt(sum_a sum|u-u_prev|^2/m_a) / sqrt(sum_a sum|u|^2/m_a), u_prev <- u;
+ * appends "t\tresidual" to error_L2.dat. The d_*_b arrays receive per-tile
+ * partial sums (at most ceil(nx/16)*ceil(ny/16) entries per block, as the
+ * reference allocates); h_*_b are not used. */
+void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *grid, AREA *areas, D2Q9 *pps_d,
+              Lbd **upsx, Lbd **upsy, double **h_uvp2_b, double **d_uvp2_b,
+              double **h_uv2_b, double **d_uv2_b);
+
+/* ---- additions (no counterpart in the reference) ---- */
+
+/* Latest values recorded by error_L2 / getDrag_block, after a flush:
+ * residual, Fx, Fy, Cd, Cl. */
+int mbl_last_diagnostics(mbl_handle_t h, double out[5]);
+
+/* Copies the recorded diagnostics series (all records since the handle was
+ * created or cleared) into caller arrays of capacity `cap`; returns the count.
+ * kind 0: (t, residual); kind 1: (t, Cd, Cl). Arrays not needed may be NULL. */
+int mbl_diagnostics_series(mbl_handle_t h, int kind, int cap, int *t, double *a, double *b);
+void mbl_diagnostics_clear(mbl_handle_t h);
+
+/* Advances one block by `nsteps` fused stream-collide steps on its own
+ * (no interface work), swapping pp->F / pp->f after every step. This is the
+ * path of the single-block periodic bandwidth benchmark. */
+int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
+
+/* Per-launch time of the fused stream-collide kernel: wraps the next
+ * mbl_step_block() launches in HIP events on the handle's stream. */
+int mbl_timer_begin(mbl_handle_t h);
+int mbl_timer_end(mbl_handle_t h, double *elapsed_ms);
+
+/* Launch tuning of the fused stream-collide kernel: nodes per thread along x
+ * (1, 2 or 4) and threads per workgroup (64..1024). 0 keeps the default. */
+int mbl_set_tuning(mbl_handle_t h, int nodes_per_thread, int threads_per_group);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBL_H */

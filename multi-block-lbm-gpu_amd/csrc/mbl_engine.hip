@@ -1,0 +1,1536 @@
+// mbl_engine.hip -- host side of libmbl_hip: the handle, the device tables
+// built from the caller's reference structures, the refinement-cycle driver
+// with per-phase hipGraph capture, device-side diagnostics and every
+// extern "C" entry point of include/mbl.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "mbl_internal.h"
+
+using namespace mbl;
+
+namespace {
+
+constexpr int EXh[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int EYh[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+
+struct Sub {
+    int off = 0, count = 0, max_n = 0, max_nc = 0;
+    bool spline = false;
+};
+typedef std::vector<Sub> Batch;
+
+// Device tables for one multi-block layout (areas, planes, lines) as passed
+// to evolution(). Rebuilt when the caller's pointers or geometry change.
+struct Layout {
+    uint64_t sig = 0;
+    int LEVEL = 0, nblocks = 0, number_line = 0, number_line2 = 0;
+    int mi[MBL_MAX_LEVEL] = {0};
+    int ASP[MBL_MAX_LEVEL + 1] = {0}, LTC[MBL_MAX_LEVEL + 1] = {0}, LS3[MBL_MAX_LEVEL + 1] = {0},
+        LCC[MBL_MAX_LEVEL + 1] = {0};
+    std::vector<BlockDesc> hb;
+    BlockDesc *db = nullptr;
+    int tiles[MBL_MAX_LEVEL] = {0};
+    int vx[MBL_MAX_LEVEL] = {0};
+    bool periodic = false;
+    std::vector<LineOp> hops;
+    LineOp *dops = nullptr;
+    int *didx = nullptr;
+    uint8_t *dmask = nullptr;
+    double *dM64 = nullptr;
+    Batch bT[MBL_MAX_LEVEL], bCC[MBL_MAX_LEVEL], bS[MBL_MAX_LEVEL], bCF[MBL_MAX_LEVEL];
+    std::vector<std::vector<int>> batch_lines_T, batch_lines_S, batch_lines_CF; // line ids per level
+    std::map<uint64_t, hipGraphExec_t> graphs;
+    void release()
+    {
+        for (auto &g : graphs) hipGraphExecDestroy(g.second);
+        graphs.clear();
+        if (db) hipFree(db);
+        if (dops) hipFree(dops);
+        if (didx) hipFree(didx);
+        if (dmask) hipFree(dmask);
+        if (dM64) hipFree(dM64);
+        db = nullptr;
+        dops = nullptr;
+        didx = nullptr;
+        dmask = nullptr;
+        dM64 = nullptr;
+    }
+};
+
+struct Single {
+    uint64_t sig = 0;
+    BlockDesc hd{};
+    BlockDesc *dd = nullptr;
+    int tiles = 0, vx = 1, rows = 4, tpb = 256;
+    bool flags = true, periodic = false;
+};
+
+struct Residual {
+    uint64_t sig = 0;
+    std::vector<BlockDesc> hd;
+    BlockDesc *dd = nullptr;
+    int total = 0;
+    double *partials = nullptr;
+};
+
+struct PendingDrag {
+    int t;
+    int m;
+};
+
+} // namespace
+
+struct mbl_context {
+    mbl_config cfg;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int device = 0;
+    int last_error = 0;
+    int vx = 4, tpb = 256;
+    Layout *layout = nullptr;
+    Single single;
+    Residual resid;
+    // diagnostics: device record slots and pinned host mirrors
+    int cap = 0;
+    double *d_l2 = nullptr, *d_drag = nullptr;
+    double *h_l2 = nullptr, *h_drag = nullptr;
+    std::vector<int> pend_l2;
+    std::vector<PendingDrag> pend_drag;
+    std::vector<int> ser_l2_t, ser_drag_t;
+    std::vector<double> ser_l2, ser_cd, ser_cl;
+    double last[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing = false;
+};
+
+static mbl_context *g_current = nullptr;
+
+static int set_err(mbl_context *h, int code)
+{
+    if (h && code && !h->last_error) h->last_error = code;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                                                \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                        \
+            fprintf(stderr, "mbl: %s failed: %s\n", #call, hipGetErrorString(e_));                                     \
+            return set_err(h, MBL_E_HIP);                                                                              \
+        }                                                                                                              \
+    } while (0)
+
+static inline uint64_t mix64(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+    return h;
+}
+
+static inline Consts make_consts(const mbl_config &c)
+{
+    Consts k;
+    k.rho0 = c.rho0;
+    k.U0x = c.U0[0];
+    k.U0y = c.U0[1];
+    k.LY = (double)c.LY;
+    k.umax = c.U0[0];
+    return k;
+}
+
+static int storage_mode(const mbl_config &c)
+{
+#ifdef MBL_FP64
+    return c.storage == MBL_STORE_NATIVE ? ST_NATIVE : -1;
+#else
+    return (c.storage >= 0 && c.storage <= 3) ? c.storage : -1;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// configuration and handle
+// ---------------------------------------------------------------------------
+extern "C" void mbl_config_default(mbl_config *c)
+{
+    memset(c, 0, sizeof(*c));
+    c->LEVEL = 2; // _GEO_2_5 (geo/geometry.h:57-58)
+    c->mi[0] = 1;
+    c->mi[1] = 2;
+    c->number_line2 = 8;
+    c->rho0 = 1.0; // parameters.h:27-43
+    c->U0[0] = 0.1;
+    c->U0[1] = 0.0;
+    c->LX = 768;
+    c->LY = 320;
+    c->R = 10;
+    c->CX = 160;
+    c->CY = 160;
+    c->bc_mode = MBL_BC_REFERENCE;
+    c->init_mode = MBL_INIT_UNIFORM;
+    c->storage = MBL_STORE_NATIVE;
+    c->seed = 0x5EED2024ULL;
+    c->sigma = 0.01;
+    c->use_graph = 1;
+    c->sync_diag = 0;
+    c->flush_interval = 1000;
+    c->write_files = 1;
+    c->out_dir[0] = 0;
+}
+
+extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
+{
+    if (!out || !cfg) return MBL_E_INVALID;
+    *out = nullptr;
+    if (storage_mode(*cfg) < 0) return MBL_E_UNSUPPORTED;
+    if (cfg->LEVEL < 1 || cfg->LEVEL > MBL_MAX_LEVEL) return MBL_E_INVALID;
+    mbl_context *h = new mbl_context();
+    h->cfg = *cfg;
+    if (h->cfg.flush_interval <= 0) h->cfg.flush_interval = 1000;
+    h->cap = h->cfg.flush_interval;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return MBL_E_HIP;
+    }
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return MBL_E_HIP;
+    }
+    h->stream = h->own_stream;
+    if (hipMalloc(&h->d_l2, sizeof(double) * 3 * h->cap) != hipSuccess ||
+        hipMalloc(&h->d_drag, sizeof(double) * 2 * h->cap) != hipSuccess ||
+        hipHostMalloc(&h->h_l2, sizeof(double) * 3 * h->cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&h->h_drag, sizeof(double) * 2 * h->cap, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        mbl_handle_destroy(h);
+        return MBL_E_NOMEM;
+    }
+    g_current = h;
+    *out = h;
+    return 0;
+}
+
+static void release_single(mbl_context *h)
+{
+    if (h->single.dd) hipFree(h->single.dd);
+    h->single.dd = nullptr;
+    h->single.sig = 0;
+}
+
+static void release_resid(mbl_context *h)
+{
+    if (h->resid.dd) hipFree(h->resid.dd);
+    if (h->resid.partials) hipFree(h->resid.partials);
+    h->resid.dd = nullptr;
+    h->resid.partials = nullptr;
+    h->resid.sig = 0;
+}
+
+extern "C" void mbl_handle_destroy(mbl_handle_t h)
+{
+    if (!h) return;
+    if (h->stream) {
+        mbl_flush_diagnostics(h);
+        hipStreamSynchronize(h->stream);
+    }
+    if (h->layout) {
+        h->layout->release();
+        delete h->layout;
+    }
+    release_single(h);
+    release_resid(h);
+    if (h->d_l2) hipFree(h->d_l2);
+    if (h->d_drag) hipFree(h->d_drag);
+    if (h->h_l2) hipHostFree(h->h_l2);
+    if (h->h_drag) hipHostFree(h->h_drag);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->own_stream) hipStreamDestroy(h->own_stream);
+    if (g_current == h) g_current = nullptr;
+    delete h;
+}
+
+extern "C" void mbl_handle_make_current(mbl_handle_t h) { g_current = h; }
+extern "C" mbl_handle_t mbl_handle_current(void) { return g_current; }
+extern "C" int mbl_get_last_error(mbl_handle_t h) { return h ? h->last_error : MBL_E_NO_HANDLE; }
+
+extern "C" const char *mbl_error_string(int code)
+{
+    switch (code) {
+    case MBL_OK: return "ok";
+    case MBL_E_INVALID: return "invalid argument or geometry";
+    case MBL_E_HIP: return "HIP runtime error";
+    case MBL_E_NOMEM: return "allocation failed";
+    case MBL_E_NOT_FINITE: return "non-finite residual or force";
+    case MBL_E_DIVERGED: return "density <= 0";
+    case MBL_E_NO_HANDLE: return "no current handle";
+    case MBL_E_UNSUPPORTED: return "unsupported mode or size";
+    default: return "unknown error";
+    }
+}
+
+extern "C" int mbl_synchronize(mbl_handle_t h)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return h->last_error;
+}
+
+extern "C" int mbl_set_stream(mbl_handle_t h, void *s)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->stream = s ? (hipStream_t)s : h->own_stream;
+    if (h->layout)
+        for (auto &g : h->layout->graphs) hipGraphExecDestroy(g.second), (void)0;
+    if (h->layout) h->layout->graphs.clear();
+    return 0;
+}
+
+extern "C" int mbl_set_tuning(mbl_handle_t h, int vx, int tpb)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    if (vx && vx != 1 && vx != 2 && vx != 4) return MBL_E_INVALID;
+    if (tpb && (tpb % 64 || tpb < 64 || tpb > 256)) return MBL_E_INVALID;
+    if (vx) h->vx = vx;
+    if (tpb) h->tpb = tpb;
+    release_single(h);
+    if (h->layout) {
+        h->layout->release();
+        delete h->layout;
+        h->layout = nullptr;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// memory helpers
+// ---------------------------------------------------------------------------
+extern "C" void *mbl_malloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+extern "C" void mbl_free(void *p)
+{
+    if (p) hipFree(p);
+}
+extern "C" int mbl_memcpy_h2d(void *d, const void *s, size_t n)
+{
+    return hipMemcpy(d, s, n, hipMemcpyHostToDevice) == hipSuccess ? 0 : MBL_E_HIP;
+}
+extern "C" int mbl_memcpy_d2h(void *d, const void *s, size_t n)
+{
+    return hipMemcpy(d, s, n, hipMemcpyDeviceToHost) == hipSuccess ? 0 : MBL_E_HIP;
+}
+extern "C" int mbl_memcpy_d2d(void *d, const void *s, size_t n)
+{
+    return hipMemcpy(d, s, n, hipMemcpyDeviceToDevice) == hipSuccess ? 0 : MBL_E_HIP;
+}
+extern "C" int mbl_memset_d(void *d, int v, size_t n) { return hipMemset(d, v, n) == hipSuccess ? 0 : MBL_E_HIP; }
+
+extern "C" size_t mbl_plane_bytes(mbl_handle_t h, int nx, int ny)
+{
+    int mode = h ? storage_mode(h->cfg) : ST_NATIVE;
+    if (mode < 0) return 0;
+    return store_elem_bytes(mode) * (size_t)nx * (size_t)ny;
+}
+
+extern "C" int mbl_block_alloc(mbl_handle_t h, const AREA *a, D2Q9 *pp, Lbd **upx, Lbd **upy)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!a || !pp || a->nx <= 0 || a->ny <= 0) return set_err(h, MBL_E_INVALID);
+    memset(pp, 0, sizeof(*pp));
+    const size_t n = (size_t)a->nx * a->ny;
+    const size_t pb = mbl_plane_bytes(h, a->nx, a->ny);
+    bool ok = (hipMalloc(&pp->solid, n * sizeof(int)) == hipSuccess);
+    Lbd **v = &pp->x;
+    for (int b = 0; b < 7 && ok; b++) ok = (hipMalloc(&v[b], n * sizeof(Lbd)) == hipSuccess);
+    for (int k = 0; k < 9 && ok; k++)
+        ok = (hipMalloc(&pp->f[k], pb) == hipSuccess) && (hipMalloc(&pp->F[k], pb) == hipSuccess);
+    if (ok && upx) ok = (hipMalloc(upx, n * sizeof(Lbd)) == hipSuccess);
+    if (ok && upy) ok = (hipMalloc(upy, n * sizeof(Lbd)) == hipSuccess);
+    if (!ok) {
+        mbl_block_free(pp, upx ? *upx : nullptr, upy ? *upy : nullptr);
+        return set_err(h, MBL_E_NOMEM);
+    }
+    return 0;
+}
+
+extern "C" void mbl_block_free(D2Q9 *pp, Lbd *upx, Lbd *upy)
+{
+    if (pp) {
+        if (pp->solid) hipFree(pp->solid);
+        Lbd **v = &pp->x;
+        for (int b = 0; b < 25; b++)
+            if (v[b]) hipFree(v[b]);
+        memset(pp, 0, sizeof(*pp));
+    }
+    if (upx) hipFree(upx);
+    if (upy) hipFree(upy);
+}
+
+static int plane_io(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, double *host, bool read)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!pp || k < 0 || k > 8 || !host) return set_err(h, MBL_E_INVALID);
+    void *plane = which ? (void *)pp->f[k] : (void *)pp->F[k];
+    double *tmp = nullptr;
+    HIPCHK(h, hipMalloc(&tmp, n * sizeof(double) + 8));
+    int rc = 0;
+    int mode = storage_mode(h->cfg);
+    if (read) {
+        rc = launch_decode(h->stream, mode, plane, k, n, tmp);
+        if (!rc && hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+            rc = MBL_E_HIP;
+    } else {
+        if (hipMemcpyAsync(tmp, host, n * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+            rc = MBL_E_HIP;
+        if (!rc) rc = launch_encode(h->stream, mode, plane, k, n, tmp);
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) rc = MBL_E_HIP;
+    hipFree(tmp);
+    return set_err(h, rc);
+}
+
+extern "C" int mbl_read_distribution(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, double *out)
+{
+    return plane_io(h, pp, which, k, n, out, true);
+}
+
+extern "C" int mbl_write_distribution(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, const double *in)
+{
+    return plane_io(h, pp, which, k, n, (double *)in, false);
+}
+
+// ---------------------------------------------------------------------------
+// block descriptors
+// ---------------------------------------------------------------------------
+static BlockDesc make_desc(const AREA &a, const D2Q9 &pp, int level, Lbd *upx, Lbd *upy)
+{
+    BlockDesc d;
+    memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 9; k++) {
+        d.P[0][k] = (void *)pp.F[k];
+        d.P[1][k] = (void *)pp.f[k];
+    }
+    d.flag = pp.solid;
+    d.rho = pp.rho;
+    d.ux = pp.ux;
+    d.uy = pp.uy;
+    d.upx = upx;
+    d.upy = upy;
+    d.nx = a.nx;
+    d.ny = a.ny;
+    d.level = level;
+    d.m = a.m;
+    d.base_y = a.base_y;
+    d.dx = 1.0 / a.m;
+    d.omega = a.rv_tau;
+    d.inv_m = 1.0 / a.m;
+    return d;
+}
+
+// Reads the node flags of a block and checks the rules the fused kernel
+// relies on: bounce-back solids off the block edge, outlets with a fluid or
+// ghost x-1 neighbour.
+static int validate_flags(mbl_context *h, const AREA &a, const int *d_solid, bool periodic, bool *any_solid)
+{
+    const size_t n = (size_t)a.nx * a.ny;
+    std::vector<int> f(n);
+    HIPCHK(h, hipMemcpy(f.data(), d_solid, n * sizeof(int), hipMemcpyDeviceToHost));
+    bool solid = false;
+    for (int j = 0; j < a.ny; j++) {
+        for (int i = 0; i < a.nx; i++) {
+            int s = f[(size_t)j * a.nx + i];
+            if (s == SOLID_SBB) {
+                solid = true;
+                if (!periodic && (i < 1 || j < 1 || i > a.nx - 2 || j > a.ny - 2)) {
+                    fprintf(stderr, "mbl: SOLID_SBB node (%d,%d) on a block edge\n", i, j);
+                    return set_err(h, MBL_E_INVALID);
+                }
+            } else if (s == MBL_SOLID_WALL) {
+                solid = true;
+            } else if (s == BOUNDARY_OUTLET || s == BOUNDARY_INLET) {
+                // the extrapolation neighbour (x-1 for an outlet, x+1 for an inlet)
+                int in = (s == BOUNDARY_OUTLET) ? i - 1 : i + 1;
+                int nb = (in >= 0 && in < a.nx) ? f[(size_t)j * a.nx + in] : -1;
+                if (nb != FLUID && nb != C_BOUNDARY) {
+                    fprintf(stderr, "mbl: inlet/outlet node (%d,%d) without a fluid or ghost neighbour\n", i, j);
+                    return set_err(h, MBL_E_INVALID);
+                }
+            } else if (s != FLUID && s != C_BOUNDARY) {
+                fprintf(stderr, "mbl: unknown node flag %d at (%d,%d)\n", s, i, j);
+                return set_err(h, MBL_E_INVALID);
+            }
+        }
+    }
+    if (any_solid) *any_solid = solid;
+    return 0;
+}
+
+static void line_node(const P4 &l, int ip, int &i, int &j)
+{
+    if (l.xstart == l.xend) {
+        i = l.xstart;
+        j = l.ystart + ip;
+    } else {
+        i = l.xstart + ip;
+        j = l.ystart;
+    }
+}
+
+// c_c_boundary_transfer index rules (assit_multi.h:21-37)
+static void cc_nodes(const Inf_Ff_store &L, int ip, int &i1, int &j1, int &i2, int &j2)
+{
+    i1 = j1 = i2 = j2 = 0;
+    if (L.line_C.xstart == L.line_C.xend) {
+        i1 = L.line_C.xstart;
+        j1 = L.line_C.ystart + ip;
+        i2 = L.line_F.xstart;
+        j2 = L.line_F.ystart + ip;
+    }
+    if (L.line_C.ystart == L.line_C.yend) {
+        i1 = L.line_C.xstart + ip;
+        j1 = L.line_C.ystart;
+        i2 = L.line_F.xstart + ip;
+        j2 = L.line_F.ystart;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hazard scheduler for one batch of interface ops issued in reference order.
+// Units are (block, node, field) with field 0 = rho/u, 1 = F. A write that is
+// overwritten later in the batch before any read is dead and its entry bit is
+// cleared; the remaining ops are levelled so that every read-after-write and
+// write-after-read pair lands in successive sub-batches, which then run as
+// one launch each.
+// ---------------------------------------------------------------------------
+struct OpBuild {
+    LineOp op;
+    std::vector<int> src, dst;
+    std::vector<uint8_t> mask;
+    std::vector<uint64_t> rd;                          // read units
+    std::vector<std::pair<uint64_t, int>> wr;          // write units, (unit, entry*2+field)
+};
+
+static inline uint64_t unit(int blk, size_t node, int field)
+{
+    return ((uint64_t)blk << 40) | ((uint64_t)node << 1) | (uint64_t)field;
+}
+
+static std::vector<int> schedule(std::vector<OpBuild> &ops)
+{
+    struct St {
+        int wop = -1, went = 0;
+        bool read_since = false;
+        std::vector<int> readers;
+    };
+    std::unordered_map<uint64_t, St> st;
+    std::vector<int> lvl(ops.size(), 0);
+    for (size_t j = 0; j < ops.size(); j++) {
+        int L = 0;
+        for (uint64_t u : ops[j].rd) {
+            St &s = st[u];
+            if (s.wop >= 0) L = std::max(L, lvl[s.wop] + 1);
+            s.read_since = true;
+            if (s.readers.empty() || s.readers.back() != (int)j) s.readers.push_back((int)j);
+        }
+        for (auto &w : ops[j].wr) {
+            St &s = st[w.first];
+            if (s.wop >= 0 && !s.read_since) {
+                int ent = s.went >> 1, fld = s.went & 1;
+                ops[s.wop].mask[ent] &= (uint8_t)~(1u << fld);
+            }
+            for (int r : s.readers)
+                if (r != (int)j) L = std::max(L, lvl[r] + 1);
+            s.wop = (int)j;
+            s.went = w.second;
+            s.read_since = false;
+            s.readers.clear();
+        }
+        lvl[j] = L;
+    }
+    return lvl;
+}
+
+static bool in_block(const AREA &a, int i, int j) { return i >= 0 && j >= 0 && i < a.nx && j < a.ny; }
+
+// ---------------------------------------------------------------------------
+// layout build
+// ---------------------------------------------------------------------------
+struct EvoArgs {
+    Lbi *asp, *ls3, *ltc, *lcc;
+    AREA *areas;
+    D2Q9 *pps;
+    Inf_Ff_store *lc, *lf, *lccs;
+    Ff_Store *t3;
+    FM_spline3 *s3;
+};
+
+static uint64_t layout_sig(mbl_context *h, const EvoArgs &A, int nblocks, int nline, int nline2)
+{
+    uint64_t s = 0x1234;
+    s = mix64(s, (uint64_t)nblocks);
+    s = mix64(s, (uint64_t)nline);
+    s = mix64(s, (uint64_t)nline2);
+    for (int l = 0; l <= h->cfg.LEVEL; l++) {
+        s = mix64(s, (uint64_t)A.asp[l]);
+        s = mix64(s, (uint64_t)A.ltc[l]);
+        s = mix64(s, (uint64_t)A.ls3[l]);
+        s = mix64(s, (uint64_t)A.lcc[l]);
+    }
+    for (int a = 0; a < nblocks; a++) {
+        const AREA &ar = A.areas[a];
+        s = mix64(s, (uint64_t)ar.m | ((uint64_t)ar.nx << 16) | ((uint64_t)ar.ny << 40));
+        uint64_t tb;
+        memcpy(&tb, &ar.tau, 8);
+        s = mix64(s, tb);
+        const D2Q9 &p = A.pps[a];
+        for (int k = 0; k < 9; k++) {
+            uintptr_t x = (uintptr_t)p.F[k], y = (uintptr_t)p.f[k];
+            s = mix64(s, std::min(x, y));
+            s = mix64(s, std::max(x, y));
+        }
+        s = mix64(s, (uintptr_t)p.solid);
+        s = mix64(s, (uintptr_t)p.rho);
+    }
+    auto line = [&](const Inf_Ff_store &L) {
+        s = mix64(s, (uint64_t)L.areac_n | ((uint64_t)L.areaf_n << 20) | ((uint64_t)L.Nc << 40));
+        s = mix64(s, (uint64_t)L.Nf);
+        const int *q = &L.line_C.xstart;
+        for (int i = 0; i < 8; i++) s = mix64(s, (uint64_t)(uint32_t)q[i]);
+    };
+    for (int b = 0; b < nline; b++) {
+        line(A.lc[b]);
+        line(A.lf[b]);
+        const F_k3 *ft = &A.t3[b].f_t1;
+        for (int sl = 0; sl < 3; sl++)
+            for (int k = 0; k < 9; k++) s = mix64(s, (uintptr_t)ft[sl].F[k]);
+        s = mix64(s, (uintptr_t)A.s3[b].Varibs_M);
+        s = mix64(s, (uintptr_t)A.s3[b].l);
+    }
+    for (int c = 0; c < nline2; c++) line(A.lccs[c]);
+    return s;
+}
+
+static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline, int nline2, uint64_t sig,
+                        Layout *&out)
+{
+    const mbl_config &cfg = h->cfg;
+    Layout *L = new Layout();
+    L->sig = sig;
+    L->LEVEL = cfg.LEVEL;
+    L->nblocks = nblocks;
+    L->number_line = nline;
+    L->number_line2 = nline2;
+    L->periodic = (cfg.bc_mode == MBL_BC_PERIODIC);
+    for (int l = 0; l < cfg.LEVEL; l++) L->mi[l] = cfg.mi[l];
+    for (int l = 0; l <= cfg.LEVEL; l++) {
+        L->ASP[l] = A.asp[l];
+        L->LTC[l] = A.ltc[l];
+        L->LS3[l] = A.ls3[l];
+        L->LCC[l] = A.lcc[l];
+    }
+    auto fail = [&](const char *why) {
+        fprintf(stderr, "mbl: invalid layout: %s\n", why);
+        L->release();
+        delete L;
+        return set_err(h, MBL_E_INVALID);
+    };
+    if (L->ASP[0] != 0 || L->ASP[cfg.LEVEL] != nblocks) return fail("ASP does not cover the blocks");
+    if (L->periodic && nblocks != 1) return fail("periodic mode needs exactly one block");
+    // blocks
+    L->hb.resize(nblocks);
+    std::vector<int> level_of(nblocks, -1);
+    for (int l = 0; l < cfg.LEVEL; l++)
+        for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) level_of[a] = l;
+    const int rows = h->tpb / 64;
+    for (int l = 0; l < cfg.LEVEL; l++) {
+        int vx = h->vx;
+        for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
+            while (vx > 1 && A.areas[a].nx % vx) vx >>= 1;
+        L->vx[l] = vx;
+        int tiles = 0;
+        for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) {
+            const AREA &ar = A.areas[a];
+            if (ar.nx < 2 || ar.ny < 2) return fail("block smaller than 2x2");
+            for (int k = 0; k < 9; k++)
+                if (!A.pps[a].F[k] || !A.pps[a].f[k] || A.pps[a].F[k] == A.pps[a].f[k]) return fail("F/f planes");
+            BlockDesc d = make_desc(ar, A.pps[a], l, nullptr, nullptr);
+            d.tiles_x = (ar.nx + 64 * vx - 1) / (64 * vx);
+            d.tile_begin = tiles;
+            tiles += d.tiles_x * ((ar.ny + rows - 1) / rows);
+            L->hb[a] = d;
+            if (validate_flags(h, ar, A.pps[a].solid, L->periodic, nullptr)) {
+                L->release();
+                delete L;
+                return MBL_E_INVALID;
+            }
+        }
+        L->tiles[l] = tiles;
+    }
+    // interface ops, batch by batch, in the order evolution.h issues them
+    std::vector<OpBuild> all;
+    auto check_cf_line = [&](int b) -> bool {
+        const Inf_Ff_store &lc = A.lc[b];
+        if (lc.areac_n < 0 || lc.areac_n >= nblocks || lc.areaf_n < 0 || lc.areaf_n >= nblocks) return false;
+        const AREA &ac = A.areas[lc.areac_n], &af = A.areas[lc.areaf_n];
+        int r = af.m / ac.m;
+        if (r < 1 || lc.Nf != r * (lc.Nc - 1) + 1) return false;
+        if (A.t3[b].N != lc.Nf || A.s3[b].N != lc.Nc) return false;
+        if (ac.tau == 1.0 || af.tau == 1.0) return false;
+        return true;
+    };
+    for (int b = 0; b < nline; b++)
+        if (!check_cf_line(b)) return fail("coarse/fine line");
+
+    auto make_T_or_CF = [&](int b, int type) -> OpBuild {
+        OpBuild o;
+        memset(&o.op, 0, sizeof(LineOp));
+        const Inf_Ff_store &lc = A.lc[b];
+        const AREA &af = A.areas[lc.areaf_n];
+        o.op.type = type;
+        o.op.n = A.t3[b].N;
+        o.op.dst_block = lc.areaf_n;
+        o.op.src_block = lc.areaf_n;
+        const F_k3 *ft = &A.t3[b].f_t1;
+        for (int sl = 0; sl < 3; sl++)
+            for (int k = 0; k < 9; k++) o.op.ring[sl][k] = ft[sl].F[k];
+        for (int ip = 0; ip < o.op.n; ip++) {
+            int i, j;
+            line_node(lc.line_F, ip, i, j);
+            if (!in_block(af, i, j)) o.op.n = -1;
+            size_t node = (size_t)j * af.nx + i;
+            o.dst.push_back((int)node);
+            o.mask.push_back(2);
+            o.wr.push_back({unit(lc.areaf_n, node, 1), ip * 2 + 1});
+        }
+        return o;
+    };
+    auto make_CC = [&](int c) -> OpBuild {
+        OpBuild o;
+        memset(&o.op, 0, sizeof(LineOp));
+        const Inf_Ff_store &l = A.lccs[c];
+        o.op.type = OP_CC;
+        o.op.n = l.Nc;
+        o.op.src_block = l.areac_n;
+        o.op.dst_block = l.areaf_n;
+        if (l.areac_n < 0 || l.areac_n >= nblocks || l.areaf_n < 0 || l.areaf_n >= nblocks) {
+            o.op.n = -1;
+            return o;
+        }
+        const AREA &a1 = A.areas[l.areac_n], &a2 = A.areas[l.areaf_n];
+        for (int ip = 0; ip < l.Nc; ip++) {
+            int i1, j1, i2, j2;
+            cc_nodes(l, ip, i1, j1, i2, j2);
+            if (!in_block(a1, i1, j1) || !in_block(a2, i2, j2)) {
+                o.op.n = -1;
+                return o;
+            }
+            size_t s = (size_t)j1 * a1.nx + i1, d = (size_t)j2 * a2.nx + i2;
+            o.src.push_back((int)s);
+            o.dst.push_back((int)d);
+            o.mask.push_back(3);
+            o.rd.push_back(unit(l.areac_n, s, 0));
+            o.rd.push_back(unit(l.areac_n, s, 1));
+            o.wr.push_back({unit(l.areaf_n, d, 0), ip * 2 + 0});
+            o.wr.push_back({unit(l.areaf_n, d, 1), ip * 2 + 1});
+        }
+        return o;
+    };
+    auto make_S = [&](int b) -> OpBuild {
+        OpBuild o;
+        memset(&o.op, 0, sizeof(LineOp));
+        const Inf_Ff_store &lc = A.lc[b];
+        const AREA &ac = A.areas[lc.areac_n], &af = A.areas[lc.areaf_n];
+        const FM_spline3 &S3 = A.s3[b];
+        int r = af.m / ac.m;
+        o.op.type = OP_SPLINE;
+        o.op.n = lc.Nf;
+        o.op.nc = lc.Nc;
+        o.op.ratio = r;
+        o.op.para = (af.tau - 1.0) / (r * (ac.tau - 1.0));
+        o.op.src_block = lc.areac_n;
+        o.op.dst_block = lc.areaf_n;
+        o.op.tl = S3.l;
+        o.op.tm = S3.m;
+        o.op.tu = S3.u;
+        o.op.varibs = S3.Varibs_M;
+        Lbd *const *gc = &S3.rho;
+        for (int c = 0; c < 12; c++) o.op.gcol[c] = gc[c];
+        const F_k3 *ft = &A.t3[b].f_t1;
+        for (int sl = 0; sl < 3; sl++)
+            for (int k = 0; k < 9; k++) o.op.ring[sl][k] = ft[sl].F[k];
+        for (int ip = 0; ip < lc.Nc; ip++) {
+            int i, j;
+            line_node(lc.line_C, ip, i, j);
+            if (!in_block(ac, i, j)) {
+                o.op.n = -1;
+                return o;
+            }
+            size_t s = (size_t)j * ac.nx + i;
+            o.src.push_back((int)s);
+            o.rd.push_back(unit(lc.areac_n, s, 0));
+            o.rd.push_back(unit(lc.areac_n, s, 1));
+        }
+        for (int ip = 0; ip < lc.Nf; ip++) {
+            int i, j;
+            line_node(lc.line_F, ip, i, j);
+            if (!in_block(af, i, j)) {
+                o.op.n = -1;
+                return o;
+            }
+            size_t d = (size_t)j * af.nx + i;
+            o.dst.push_back((int)d);
+            o.mask.push_back(1);
+            o.wr.push_back({unit(lc.areaf_n, d, 0), ip * 2 + 0});
+        }
+        return o;
+    };
+    auto make_FC = [&](int b) -> OpBuild {
+        OpBuild o;
+        memset(&o.op, 0, sizeof(LineOp));
+        const Inf_Ff_store &l = A.lf[b];
+        const AREA &ac = A.areas[l.areac_n], &af = A.areas[l.areaf_n];
+        int r = af.m / ac.m;
+        o.op.type = OP_FC;
+        o.op.n = l.Nc;
+        o.op.ratio = r;
+        o.op.para = r * (ac.tau - 1.0) / (af.tau - 1.0);
+        o.op.src_block = l.areaf_n;
+        o.op.dst_block = l.areac_n;
+        for (int ip = 0; ip < l.Nc; ip++) {
+            int ic, jc, i_f, j_f;
+            if (l.line_C.xstart == l.line_C.xend) {
+                ic = l.line_C.xstart;
+                jc = l.line_C.ystart + ip;
+                i_f = l.line_F.xstart;
+                j_f = l.line_F.ystart + r * ip;
+            } else {
+                ic = l.line_C.xstart + ip;
+                jc = l.line_C.ystart;
+                i_f = l.line_F.xstart + r * ip;
+                j_f = l.line_F.ystart;
+            }
+            if (!in_block(ac, ic, jc) || !in_block(af, i_f, j_f)) {
+                o.op.n = -1;
+                return o;
+            }
+            size_t s = (size_t)j_f * af.nx + i_f, d = (size_t)jc * ac.nx + ic;
+            o.src.push_back((int)s);
+            o.dst.push_back((int)d);
+            o.mask.push_back(3);
+            o.rd.push_back(unit(l.areaf_n, s, 0));
+            o.rd.push_back(unit(l.areaf_n, s, 1));
+            o.wr.push_back({unit(l.areac_n, d, 0), ip * 2 + 0});
+            o.wr.push_back({unit(l.areac_n, d, 1), ip * 2 + 1});
+        }
+        return o;
+    };
+
+    // Batches, each scheduled on its own; ops appended to `all` grouped by sub-batch.
+    auto add_batch = [&](std::vector<OpBuild> ops, Batch &batch) -> bool {
+        for (auto &o : ops)
+            if (o.op.n < 0) return false;
+        std::vector<int> lv = schedule(ops);
+        int nsub = 0;
+        for (int v : lv) nsub = std::max(nsub, v + 1);
+        for (int s = 0; s < nsub; s++) {
+            Sub sb;
+            sb.off = (int)all.size();
+            for (size_t j = 0; j < ops.size(); j++) {
+                if (lv[j] != s) continue;
+                sb.count++;
+                sb.max_n = std::max(sb.max_n, ops[j].op.n);
+                sb.max_nc = std::max(sb.max_nc, ops[j].op.nc);
+                sb.spline = (ops[j].op.type == OP_SPLINE);
+                all.push_back(std::move(ops[j]));
+            }
+            if (sb.count) batch.push_back(sb);
+        }
+        return true;
+    };
+    for (int l = 0; l < cfg.LEVEL; l++) {
+        std::vector<OpBuild> ops;
+        for (int b = L->LTC[l]; b < L->LTC[l + 1]; b++) ops.push_back(make_T_or_CF(b, OP_TEMPORAL));
+        if (!add_batch(std::move(ops), L->bT[l])) return fail("temporal line outside its block");
+        ops.clear();
+        if (nline2 != 0)
+            for (int c = L->LCC[l]; c < L->LCC[l + 1]; c++) ops.push_back(make_CC(c));
+        if (!add_batch(std::move(ops), L->bCC[l])) return fail("same-level line outside its blocks");
+        ops.clear();
+        if (l != cfg.LEVEL - 1)
+            for (int b = L->LS3[l]; b < L->LS3[l + 1]; b++) ops.push_back(make_S(b));
+        if (!add_batch(std::move(ops), L->bS[l])) return fail("spline line outside its blocks");
+        ops.clear();
+        if (l != 0)
+            for (int b = L->LTC[l]; b < L->LTC[l + 1]; b++) {
+                ops.push_back(make_T_or_CF(b, OP_CF));
+                ops.push_back(make_FC(b));
+            }
+        if (!add_batch(std::move(ops), L->bCF[l])) return fail("coarse/fine transfer line outside its blocks");
+    }
+    // upload tables
+    size_t nidx = 0, nmask = 0, nM = 0;
+    for (auto &o : all) {
+        nidx += o.src.size() + o.dst.size();
+        nmask += o.mask.size();
+        if (o.op.type == OP_SPLINE) nM += 12 * (size_t)o.op.nc;
+    }
+    if (hipMalloc(&L->db, sizeof(BlockDesc) * std::max(1, nblocks)) != hipSuccess ||
+        hipMalloc(&L->dops, sizeof(LineOp) * std::max<size_t>(1, all.size())) != hipSuccess ||
+        hipMalloc(&L->didx, sizeof(int) * std::max<size_t>(1, nidx)) != hipSuccess ||
+        hipMalloc(&L->dmask, std::max<size_t>(1, nmask)) != hipSuccess ||
+        hipMalloc(&L->dM64, sizeof(double) * std::max<size_t>(1, nM)) != hipSuccess) {
+        L->release();
+        delete L;
+        return set_err(h, MBL_E_NOMEM);
+    }
+    std::vector<int> hidx;
+    hidx.reserve(nidx);
+    std::vector<uint8_t> hmask;
+    hmask.reserve(nmask);
+    size_t mo = 0;
+    for (auto &o : all) {
+        LineOp op = o.op;
+        op.src_idx = o.src.empty() ? nullptr : L->didx + hidx.size();
+        hidx.insert(hidx.end(), o.src.begin(), o.src.end());
+        op.dst_idx = o.dst.empty() ? nullptr : L->didx + hidx.size();
+        hidx.insert(hidx.end(), o.dst.begin(), o.dst.end());
+        op.mask = L->dmask + hmask.size();
+        hmask.insert(hmask.end(), o.mask.begin(), o.mask.end());
+        if (op.type == OP_SPLINE) {
+            op.M64 = L->dM64 + mo;
+            mo += 12 * (size_t)op.nc;
+        }
+        L->hops.push_back(op);
+    }
+    bool ok = hipMemcpy(L->db, L->hb.data(), sizeof(BlockDesc) * nblocks, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !L->hops.empty())
+        ok = hipMemcpy(L->dops, L->hops.data(), sizeof(LineOp) * L->hops.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !hidx.empty())
+        ok = hipMemcpy(L->didx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !hmask.empty()) ok = hipMemcpy(L->dmask, hmask.data(), hmask.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        L->release();
+        delete L;
+        return set_err(h, MBL_E_HIP);
+    }
+    out = L;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// refinement cycle (evolution.h:2-65)
+// ---------------------------------------------------------------------------
+struct CycleState {
+    uint32_t pm;
+    bool launch;
+    int rc;
+};
+
+static int markers_of(const Layout *L, const Ff_Store *t3, int b0, int b1, int m[3])
+{
+    if (b0 >= b1) {
+        m[0] = 2;
+        m[1] = 1;
+        m[2] = 0;
+        return 0;
+    }
+    for (int p = 0; p < 3; p++) m[p] = t3[b0].marker[p];
+    for (int b = b0 + 1; b < b1; b++)
+        for (int p = 0; p < 3; p++)
+            if (t3[b].marker[p] != m[p]) return MBL_E_UNSUPPORTED;
+    for (int p = 0; p < 3; p++)
+        if (m[p] < 0 || m[p] > 2) return MBL_E_INVALID;
+    (void)L;
+    return 0;
+}
+
+static void run_ops(mbl_context *h, Layout *L, const Batch &B, CycleState &cs, int m0, int m1, int m2, double xm)
+{
+    if (!cs.launch || cs.rc) return;
+    const int mode = storage_mode(h->cfg);
+    for (const Sub &s : B) {
+        int rc = launch_line_ops(h->stream, mode, L->dops + s.off, s.count, s.max_n, L->db, cs.pm, m0, m1, m2, xm);
+        if (rc) cs.rc = rc;
+    }
+}
+
+static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState &cs)
+{
+    if (level == L->LEVEL || cs.rc) return; // level checked before mi[level] is read
+    const int mode = storage_mode(h->cfg);
+    const Consts c = make_consts(h->cfg);
+    for (int i = 0; i < L->mi[level]; i++) {
+        int m[3];
+        if (!(level == 0 || i == 0) && !L->bT[level].empty()) {
+            int rc = markers_of(L, t3, L->LTC[level], L->LTC[level + 1], m);
+            if (rc) {
+                cs.rc = rc;
+                return;
+            }
+            run_ops(h, L, L->bT[level], cs, m[0], m[1], m[2], 1.0 * i / L->mi[level]);
+        }
+        if (cs.launch && L->tiles[level] > 0) {
+            int rc = launch_stream_collide(h->stream, mode, L->vx[level], h->tpb, L->periodic, true, true,
+                                           L->db + L->ASP[level], L->ASP[level + 1] - L->ASP[level], L->tiles[level],
+                                           h->tpb / 64, cs.pm, c);
+            if (rc) cs.rc = rc;
+        }
+        cs.pm ^= (1u << level);
+        if (L->number_line2 != 0) run_ops(h, L, L->bCC[level], cs, 0, 0, 0, 0.0);
+        if (level != L->LEVEL - 1 && L->LS3[level + 1] > L->LS3[level]) {
+            int rc = markers_of(L, t3, L->LS3[level], L->LS3[level + 1], m);
+            if (rc) {
+                cs.rc = rc;
+                return;
+            }
+            if (cs.launch && !cs.rc)
+                for (const Sub &s : L->bS[level]) {
+                    rc = launch_spline(h->stream, mode, L->dops + s.off, s.count, s.max_nc, s.max_n, L->db, cs.pm, m[0]);
+                    if (rc) cs.rc = rc;
+                }
+            for (int b = L->LS3[level]; b < L->LS3[level + 1]; b++)
+                for (int p = 0; p < 3; p++) {
+                    int bb = t3[b].marker[p] - 1;
+                    t3[b].marker[p] = (bb == -1) ? 2 : bb;
+                }
+        }
+        cycle(h, L, level + 1, t3, cs);
+        if (level != 0 && i == L->mi[level] - 1 && !L->bCF[level].empty()) {
+            int rc = markers_of(L, t3, L->LTC[level], L->LTC[level + 1], m);
+            if (rc) {
+                cs.rc = rc;
+                return;
+            }
+            run_ops(h, L, L->bCF[level], cs, m[0], m[1], m[2], 0.0);
+        }
+    }
+}
+
+extern "C" void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lccp, Lbi level, mbl_dim3 block,
+                          mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d, Inf_Ff_store *line_Inf_c,
+                          Inf_Ff_store *line_Inf_f, Inf_Ff_store *line_Inf_cc, Ff_Store *line_t3,
+                          FM_spline3 *line_S3, mbl_handle_t handle)
+{
+    (void)n;
+    (void)block;
+    (void)grid;
+    (void)pps_h;
+    mbl_context *h = handle ? handle : g_current;
+    if (!h) return;
+    if (h->last_error) return;
+    const mbl_config &cfg = h->cfg;
+    if (!asp || !ls3cffcp || !ltcffcp || !lccp || !areas || !pps_d || level < 0 || level > cfg.LEVEL) {
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    const int nblocks = asp[cfg.LEVEL];
+    const int nline = ltcffcp[cfg.LEVEL];
+    const int nline2 = lccp[cfg.LEVEL];
+    if (nline != ls3cffcp[cfg.LEVEL] || (nline > 0 && (!line_Inf_c || !line_Inf_f || !line_t3 || !line_S3)) ||
+        (nline2 > 0 && !line_Inf_cc)) {
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    EvoArgs A{asp, ls3cffcp, ltcffcp, lccp, areas, pps_d, line_Inf_c, line_Inf_f, line_Inf_cc, line_t3, line_S3};
+    const uint64_t sig = layout_sig(h, A, nblocks, nline, nline2);
+    if (!h->layout || h->layout->sig != sig) {
+        if (h->layout) {
+            hipStreamSynchronize(h->stream);
+            h->layout->release();
+            delete h->layout;
+            h->layout = nullptr;
+        }
+        Layout *L = nullptr;
+        if (build_layout(h, A, nblocks, nline, nline2, sig, L)) return;
+        h->layout = L;
+    }
+    Layout *L = h->layout;
+    // current parity of every level from the caller's F pointers
+    uint32_t pm = 0;
+    for (int l = 0; l < cfg.LEVEL; l++) {
+        int par = -1;
+        for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) {
+            int p = (pps_d[a].F[0] == (Lbd *)L->hb[a].P[0][0]) ? 0 : ((pps_d[a].F[0] == (Lbd *)L->hb[a].P[1][0]) ? 1 : -1);
+            if (p < 0 || (par >= 0 && p != par)) {
+                set_err(h, MBL_E_INVALID);
+                return;
+            }
+            par = p;
+        }
+        if (par == 1) pm |= (1u << l);
+    }
+    // phase key: parity bits + markers of every line group
+    uint64_t key = pm;
+    int sh = 8;
+    for (int l = 0; l < cfg.LEVEL; l++) {
+        int m[3];
+        if (L->LS3[l + 1] > L->LS3[l]) {
+            if (markers_of(L, line_t3, L->LS3[l], L->LS3[l + 1], m)) {
+                set_err(h, MBL_E_UNSUPPORTED);
+                return;
+            }
+            key |= (uint64_t)(m[0] | (m[1] << 2) | (m[2] << 4)) << sh;
+        }
+        sh += 6;
+    }
+    key |= (uint64_t)level << 60;
+    CycleState cs{pm, true, 0};
+    if (cfg.use_graph) {
+        auto it = L->graphs.find(key);
+        if (it == L->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            std::vector<Ff_Store> saved(line_t3, line_t3 + nline);
+            if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                set_err(h, MBL_E_HIP);
+                return;
+            }
+            cycle(h, L, level, line_t3, cs);
+            hipError_t e = hipStreamEndCapture(h->stream, &g);
+            if (cs.rc || e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                std::copy(saved.begin(), saved.end(), line_t3);
+                set_err(h, cs.rc ? cs.rc : MBL_E_HIP);
+                return;
+            }
+            hipGraphDestroy(g);
+            L->graphs[key] = ge;
+            if (hipGraphLaunch(ge, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
+        } else {
+            cs.launch = false;
+            cycle(h, L, level, line_t3, cs);
+            if (hipGraphLaunch(it->second, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
+        }
+    } else {
+        cycle(h, L, level, line_t3, cs);
+    }
+    if (cs.rc) {
+        set_err(h, cs.rc);
+        return;
+    }
+    // hand the new state back: F <-> f for every level that took an odd
+    // number of substeps in this call
+    for (int l = level; l < cfg.LEVEL; l++) {
+        if (!((cs.pm ^ pm) & (1u << l))) continue;
+        for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
+            for (int k = 0; k < 9; k++) std::swap(pps_d[a].F[k], pps_d[a].f[k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// single-block stepping (bandwidth benchmark path)
+// ---------------------------------------------------------------------------
+extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!area || !pp || nsteps < 0) return set_err(h, MBL_E_INVALID);
+    Single &S = h->single;
+    uint64_t sig = 0x77;
+    for (int k = 0; k < 9; k++) {
+        uintptr_t x = (uintptr_t)pp->F[k], y = (uintptr_t)pp->f[k];
+        sig = mix64(sig, std::min(x, y));
+        sig = mix64(sig, std::max(x, y));
+    }
+    sig = mix64(sig, (uint64_t)area->nx | ((uint64_t)area->ny << 32));
+    uint64_t tb;
+    memcpy(&tb, &area->rv_tau, 8);
+    sig = mix64(sig, tb);
+    sig = mix64(sig, (uintptr_t)pp->solid);
+    sig = mix64(sig, (uintptr_t)pp->rho);
+    if (S.sig != sig || !S.dd) {
+        release_single(h);
+        const bool periodic = (h->cfg.bc_mode == MBL_BC_PERIODIC);
+        bool any_solid = true;
+        if (validate_flags(h, *area, pp->solid, periodic, &any_solid)) return MBL_E_INVALID;
+        S.hd = make_desc(*area, *pp, 0, nullptr, nullptr);
+        S.vx = h->vx;
+        while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
+        S.tpb = h->tpb;
+        S.rows = S.tpb / 64;
+        S.hd.tiles_x = (area->nx + 64 * S.vx - 1) / (64 * S.vx);
+        S.hd.tile_begin = 0;
+        S.tiles = S.hd.tiles_x * ((area->ny + S.rows - 1) / S.rows);
+        S.periodic = periodic;
+        S.flags = !(periodic && !any_solid);
+        HIPCHK(h, hipMalloc(&S.dd, sizeof(BlockDesc)));
+        HIPCHK(h, hipMemcpy(S.dd, &S.hd, sizeof(BlockDesc), hipMemcpyHostToDevice));
+        S.sig = sig;
+    }
+    int par = (pp->F[0] == (Lbd *)S.hd.P[0][0]) ? 0 : 1;
+    const int mode = storage_mode(h->cfg);
+    const Consts c = make_consts(h->cfg);
+    const bool macros = !S.periodic || S.flags;
+    for (int t = 0; t < nsteps; t++) {
+        int rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, 1, S.tiles,
+                                       S.rows, (uint32_t)par, c);
+        if (rc) return set_err(h, rc);
+        par ^= 1;
+        for (int k = 0; k < 9; k++) std::swap(pp->F[k], pp->f[k]);
+    }
+    return 0;
+}
+
+extern "C" int mbl_timer_begin(mbl_handle_t h)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    h->timing = true;
+    return 0;
+}
+
+extern "C" int mbl_timer_end(mbl_handle_t h, double *ms)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!h->timing) return set_err(h, MBL_E_INVALID);
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float f = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&f, h->ev0, h->ev1));
+    h->timing = false;
+    if (ms) *ms = f;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// initialisation entry points
+// ---------------------------------------------------------------------------
+extern "C" void init_General(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, Lbd *upx, Lbd *upy)
+{
+    (void)grid;
+    (void)block;
+    mbl_context *h = g_current;
+    if (!h) return;
+    if (area.nx <= 0 || area.ny <= 0 || area.m <= 0 || !pp.solid || !pp.rho) {
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    const mbl_config &c = h->cfg;
+    BlockDesc d = make_desc(area, pp, 0, upx, upy);
+    int rc = launch_init_general(h->stream, storage_mode(c), d, area, pp.x, pp.y, pp.forcex, pp.forcey, c.bc_mode,
+                                 c.init_mode, c.seed, c.sigma, (double)c.LX, (double)c.LY, c.R, c.CX, c.CY,
+                                 make_consts(c));
+    if (rc) set_err(h, rc);
+}
+
+static int one_op(mbl_context *h, const LineOp &op, const BlockDesc &bd, bool ring_fill)
+{
+    LineOp *dop = nullptr;
+    BlockDesc *dbd = nullptr;
+    HIPCHK(h, hipMalloc(&dop, sizeof(LineOp)));
+    HIPCHK(h, hipMalloc(&dbd, sizeof(BlockDesc)));
+    HIPCHK(h, hipMemcpy(dop, &op, sizeof(LineOp), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(dbd, &bd, sizeof(BlockDesc), hipMemcpyHostToDevice));
+    int rc = 0;
+    if (ring_fill) rc = launch_ring_fill(h->stream, storage_mode(h->cfg), dop, 1, op.n, dbd, 0u);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    hipFree(dop);
+    hipFree(dbd);
+    return set_err(h, rc);
+}
+
+extern "C" void init_Fstore(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, Inf_Ff_store line_Inf,
+                            Ff_Store line_t3)
+{
+    (void)grid;
+    (void)block;
+    mbl_context *h = g_current;
+    if (!h) return;
+    LineOp op;
+    memset(&op, 0, sizeof(op));
+    op.n = line_t3.N;
+    op.dst_block = 0;
+    const F_k3 *ft = &line_t3.f_t1;
+    for (int sl = 0; sl < 3; sl++)
+        for (int k = 0; k < 9; k++) op.ring[sl][k] = ft[sl].F[k];
+    std::vector<int> idx(op.n);
+    for (int ip = 0; ip < op.n; ip++) {
+        int i, j;
+        line_node(line_Inf.line_F, ip, i, j);
+        if (!in_block(area, i, j)) {
+            set_err(h, MBL_E_INVALID);
+            return;
+        }
+        idx[ip] = j * area.nx + i;
+    }
+    int *didx = nullptr;
+    if (hipMalloc(&didx, sizeof(int) * std::max(1, op.n)) != hipSuccess ||
+        hipMemcpy(didx, idx.data(), sizeof(int) * op.n, hipMemcpyHostToDevice) != hipSuccess) {
+        set_err(h, MBL_E_HIP);
+        return;
+    }
+    op.dst_idx = didx;
+    BlockDesc bd = make_desc(area, pp, 0, nullptr, nullptr);
+    one_op(h, op, bd, true);
+    hipFree(didx);
+}
+
+extern "C" void init_tri_A(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, Inf_Ff_store line_Inf,
+                           FM_spline3 line_S3)
+{
+    (void)grid;
+    (void)block;
+    (void)area;
+    (void)pp;
+    (void)line_Inf;
+    mbl_context *h = g_current;
+    if (!h) return;
+    int rc = launch_tri_A(h->stream, line_S3.l, line_S3.m, line_S3.u, line_S3.N);
+    if (rc) set_err(h, rc);
+}
+
+// ---------------------------------------------------------------------------
+// geometry of the momentum-exchange links (force_block.h:2-127)
+// ---------------------------------------------------------------------------
+static int host_links(mbl_context *h, const AREA &a, const int *d_solid, int *host_solid, std::vector<Int3> *out)
+{
+    const size_t n = (size_t)a.nx * a.ny;
+    std::vector<int> tmp;
+    int *f = host_solid;
+    if (!f) {
+        tmp.resize(n);
+        f = tmp.data();
+    }
+    HIPCHK(h, hipMemcpy(f, d_solid, n * sizeof(int), hipMemcpyDeviceToHost));
+    int count = 0;
+    for (int i = 0; i < a.nx; i++)
+        for (int j = 0; j < a.ny; j++) {
+            if (f[i + (size_t)j * a.nx] != SOLID_SBB) continue;
+            for (int k = 0; k < 9; k++) {
+                int id = i - EXh[k], jd = j - EYh[k];
+                if (id < 0 || id > a.nx - 1 || jd < 0 || jd > a.ny - 1) continue;
+                if (f[id + (size_t)jd * a.nx] == FLUID) {
+                    if (out) out->push_back(Int3{i, j, k});
+                    count++;
+                }
+            }
+        }
+    return count;
+}
+
+extern "C" Lbi sum_N_drag(mbl_dim3 block, mbl_dim3 grid, AREA area, D2Q9 pp)
+{
+    (void)block;
+    (void)grid;
+    mbl_context *h = g_current;
+    if (!h) return 0;
+    int n = host_links(h, area, pp.solid, nullptr, nullptr);
+    return n < 0 ? 0 : n;
+}
+
+extern "C" void fdrag_store(AREA area, D2Q9 h_pp, Int3 *d_pdrag, D2Q9 d_pp, Lbi N)
+{
+    mbl_context *h = g_current;
+    if (!h) return;
+    std::vector<Int3> links;
+    int n = host_links(h, area, d_pp.solid, h_pp.solid, &links);
+    if (n < 0) return;
+    if (n != N) { // force_block.h:122 assert(ii == N)
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    if (N > 0 && hipMemcpy(d_pdrag, links.data(), sizeof(Int3) * N, hipMemcpyHostToDevice) != hipSuccess)
+        set_err(h, MBL_E_HIP);
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+static std::string out_path(const mbl_context *h, const char *name)
+{
+    std::string d = h->cfg.out_dir;
+    if (d.empty()) return name;
+    if (d.back() != '/') d += '/';
+    return d + name;
+}
+
+extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    const size_t nl = h->pend_l2.size(), nd = h->pend_drag.size();
+    if (!nl && !nd) return 0;
+    if (nl) HIPCHK(h, hipMemcpyAsync(h->h_l2, h->d_l2, sizeof(double) * 3 * nl, hipMemcpyDeviceToHost, h->stream));
+    if (nd) HIPCHK(h, hipMemcpyAsync(h->h_drag, h->d_drag, sizeof(double) * 2 * nd, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const double U0_2 = std::sqrt(h->cfg.U0[0] * h->cfg.U0[0] + h->cfg.U0[1] * h->cfg.U0[1]);
+    FILE *fl = nullptr, *fd = nullptr;
+    if (h->cfg.write_files) {
+        if (nl) fl = fopen(out_path(h, "error_L2.dat").c_str(), "a");
+        if (nd) fd = fopen(out_path(h, "fdrag_t.dat").c_str(), "a");
+    }
+    int rc = 0;
+    for (size_t i = 0; i < nl; i++) {
+        double v = h->h_l2[3 * i];
+        if (fl) fprintf(fl, "%d\t%g\n", h->pend_l2[i], v); // error_L2.h:302
+        h->ser_l2_t.push_back(h->pend_l2[i]);
+        h->ser_l2.push_back(v);
+        h->last[0] = v;
+        if (!std::isfinite(v) && !std::isnan(0.0 * h->h_l2[3 * i + 2])) rc = MBL_E_NOT_FINITE;
+        if (!std::isfinite(h->h_l2[3 * i + 1])) rc = MBL_E_NOT_FINITE;
+    }
+    for (size_t i = 0; i < nd; i++) {
+        double fx = h->h_drag[2 * i], fy = h->h_drag[2 * i + 1];
+        double nrm = h->cfg.rho0 * U0_2 * U0_2 * h->pend_drag[i].m * h->cfg.R;
+        double cd = fx / nrm, cl = fy / nrm;
+        if (fd) fprintf(fd, "%d\t%g\t%g\t\n", h->pend_drag[i].t, cd, cl); // force_block.h:214
+        h->ser_drag_t.push_back(h->pend_drag[i].t);
+        h->ser_cd.push_back(cd);
+        h->ser_cl.push_back(cl);
+        h->last[1] = fx;
+        h->last[2] = fy;
+        h->last[3] = cd;
+        h->last[4] = cl;
+        if (!std::isfinite(fx) || !std::isfinite(fy)) rc = MBL_E_NOT_FINITE;
+    }
+    if (fl) fclose(fl);
+    if (fd) fclose(fd);
+    h->pend_l2.clear();
+    h->pend_drag.clear();
+    return set_err(h, rc);
+}
+
+static uint64_t resid_sig(int N, const AREA *areas, const D2Q9 *pps, Lbd **upsx, Lbd **upsy)
+{
+    uint64_t s = 0x4242;
+    for (int a = 0; a < N; a++) {
+        s = mix64(s, (uint64_t)areas[a].nx | ((uint64_t)areas[a].ny << 32));
+        s = mix64(s, (uint64_t)areas[a].m);
+        s = mix64(s, (uintptr_t)pps[a].ux);
+        s = mix64(s, (uintptr_t)pps[a].uy);
+        s = mix64(s, (uintptr_t)upsx[a]);
+        s = mix64(s, (uintptr_t)upsy[a]);
+    }
+    return s;
+}
+
+extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *grid, AREA *areas, D2Q9 *pps_d,
+                         Lbd **upsx, Lbd **upsy, double **h_uvp2_b, double **d_uvp2_b, double **h_uv2_b,
+                         double **d_uv2_b)
+{
+    (void)block;
+    (void)grid;
+    (void)h_uvp2_b;
+    (void)d_uvp2_b;
+    (void)h_uv2_b;
+    (void)d_uv2_b;
+    mbl_context *h = g_current;
+    if (!h) return;
+    if (N_areas <= 0 || !areas || !pps_d || !upsx || !upsy) {
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    Residual &R = h->resid;
+    uint64_t sig = resid_sig(N_areas, areas, pps_d, upsx, upsy);
+    if (R.sig != sig || !R.dd) {
+        release_resid(h);
+        R.hd.resize(N_areas);
+        int total = 0;
+        for (int a = 0; a < N_areas; a++) {
+            BlockDesc d = make_desc(areas[a], pps_d[a], 0, upsx[a], upsy[a]);
+            d.rtiles_x = (areas[a].nx + 63) / 64;
+            d.rtile_begin = total;
+            total += d.rtiles_x * ((areas[a].ny + 15) / 16);
+            R.hd[a] = d;
+        }
+        R.total = total;
+        if (hipMalloc(&R.dd, sizeof(BlockDesc) * N_areas) != hipSuccess ||
+            hipMalloc(&R.partials, sizeof(double) * 2 * std::max(1, total)) != hipSuccess ||
+            hipMemcpy(R.dd, R.hd.data(), sizeof(BlockDesc) * N_areas, hipMemcpyHostToDevice) != hipSuccess) {
+            set_err(h, MBL_E_HIP);
+            return;
+        }
+        R.sig = sig;
+    }
+    const size_t slot = h->pend_l2.size();
+    int rc = launch_residual(h->stream, R.dd, N_areas, R.total, R.partials, N_areas, h->d_l2 + 3 * slot);
+    if (rc) {
+        set_err(h, rc);
+        return;
+    }
+    h->pend_l2.push_back(t);
+    if (h->cfg.sync_diag || (int)h->pend_l2.size() >= h->cap) mbl_flush_diagnostics(h);
+}
+
+extern "C" void getDrag_block(Lbi t, Int3 *d_pdrag, AREA area, D2Q9 d_pp, double *force, Lbi N)
+{
+    mbl_context *h = g_current;
+    if (!h) return;
+    if (N < 0 || (N > 0 && !d_pdrag)) {
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    DragArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int k = 0; k < 9; k++) a.Fold[k] = (void *)d_pp.f[k];
+    a.links = d_pdrag;
+    a.n = N;
+    a.nx = area.nx;
+    const size_t slot = h->pend_drag.size();
+    a.out = h->d_drag + 2 * slot;
+    int rc = launch_drag(h->stream, storage_mode(h->cfg), a);
+    if (rc) {
+        set_err(h, rc);
+        return;
+    }
+    h->pend_drag.push_back(PendingDrag{t, area.m});
+    if (h->cfg.sync_diag || (int)h->pend_drag.size() >= h->cap) mbl_flush_diagnostics(h);
+    if (force) {
+        force[0] = h->last[1];
+        force[1] = h->last[2];
+    }
+}
+
+extern "C" int mbl_last_diagnostics(mbl_handle_t h, double out[5])
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    for (int i = 0; i < 5; i++) out[i] = h->last[i];
+    return 0;
+}
+
+extern "C" int mbl_diagnostics_series(mbl_handle_t h, int kind, int cap, int *t, double *a, double *b)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (kind == 0) {
+        int n = (int)h->ser_l2.size();
+        for (int i = 0; i < std::min(n, cap); i++) {
+            if (t) t[i] = h->ser_l2_t[i];
+            if (a) a[i] = h->ser_l2[i];
+        }
+        return n;
+    }
+    int n = (int)h->ser_cd.size();
+    for (int i = 0; i < std::min(n, cap); i++) {
+        if (t) t[i] = h->ser_drag_t[i];
+        if (a) a[i] = h->ser_cd[i];
+        if (b) b[i] = h->ser_cl[i];
+    }
+    return n;
+}
+
+extern "C" void mbl_diagnostics_clear(mbl_handle_t h)
+{
+    if (!h) h = g_current;
+    if (!h) return;
+    h->ser_l2_t.clear();
+    h->ser_l2.clear();
+    h->ser_drag_t.clear();
+    h->ser_cd.clear();
+    h->ser_cl.clear();
+}

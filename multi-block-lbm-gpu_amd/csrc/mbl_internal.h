@@ -1,0 +1,111 @@
+// mbl_internal.h -- descriptors shared by the HIP kernels (mbl_kernels.hip) and
+// the host engine (mbl_engine.hip). Not part of the public ABI.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/mbl.h"
+
+#ifdef MBL_FP64
+typedef double real_t; // compute type of the state update
+#else
+typedef float real_t;
+#endif
+
+namespace mbl {
+
+enum StoreMode { ST_NATIVE = 0, ST_F32S = 1, ST_F16S = 2, ST_BF16S = 3 };
+
+// Element size in bytes of one f/F plane entry for a storage mode.
+inline size_t store_elem_bytes(int mode)
+{
+    switch (mode) {
+    case ST_NATIVE: return sizeof(Lbd);
+    case ST_F32S: return 4;
+    default: return 2;
+    }
+}
+
+// Constants of the run read by the kernels (parameters.h:42-52 twins).
+struct Consts {
+    double rho0;
+    double U0x, U0y;
+    double LY;   // channel height in coarse nodes, for the parabolic inlet
+    double umax; // inlet centre-line velocity
+};
+
+// One block (an AREA with its D2Q9 planes) in device memory. P[0] is the
+// caller's F plane set when the table was built, P[1] the caller's f set; the
+// block's state lives in P[parity] where parity is the level's bit of the
+// mask passed to every launch.
+struct BlockDesc {
+    void *P[2][MBL_Q];
+    const int *flag;
+    Lbd *rho, *ux, *uy;
+    Lbd *upx, *upy;       // previous velocity (error_L2), may be null
+    int nx, ny;
+    int level;
+    int m;
+    int tiles_x;          // stream-collide tiles along x
+    int tile_begin;       // first stream-collide tile of the block inside its launch
+    int rtile_begin;      // first residual tile of the block inside the residual launch
+    int rtiles_x;
+    double base_y, dx;
+    double omega;         // 1/tau
+    double inv_m;         // residual weight 1/m
+};
+
+// Interface line operations (multi/FH/assit_multi.h). An op moves data along
+// one line between two blocks or between a block and a ring buffer.
+enum OpType { OP_TEMPORAL = 0, OP_CC = 1, OP_CF = 2, OP_FC = 3, OP_SPLINE = 4 };
+
+struct LineOp {
+    int type;
+    int n;            // entries (fine points for TEMPORAL/CF/SPLINE, Nc for CC/FC)
+    int src_block;    // table index of the block read (CC, FC, SPLINE coarse block)
+    int dst_block;    // table index of the block written
+    const int *src_idx; // node index in src block per entry (CC, FC; SPLINE: coarse line nodes, Nc of them)
+    const int *dst_idx; // node index in dst block per entry
+    const uint8_t *mask; // per entry: bit0 write macros (rho, u), bit1 write F
+    Lbd *ring[3][MBL_Q]; // ring slots (TEMPORAL, CF, SPLINE), walk order &f_t1
+    int nc;             // SPLINE: coarse points
+    int ratio;          // SPLINE/FC: areaf.m / areac.m (integer division, as the reference)
+    double para;        // FC: r*(tau_c-1)/(tau_f-1); SPLINE: (tau_f-1)/(r*(tau_c-1))
+    // SPLINE work arrays: caller's FM_spline3 (l, m, u read; Varibs_M and the
+    // gathered columns written) and the library's FP64 second-derivative columns.
+    const Lbd *tl, *tm, *tu;
+    Lbd *varibs;
+    Lbd *gcol[12];
+    double *M64;
+};
+
+struct DragArgs {
+    void *Fold[MBL_Q]; // planes holding the state one substep older (the caller's f)
+    const Int3 *links;
+    int n;
+    int nx;
+    double *out;       // [0] Fx, [1] Fy
+};
+
+// ---- launchers (mbl_kernels.hip), all asynchronous on `stream` ----
+int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
+                          const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows,
+                          uint32_t parity_mask, const Consts &c);
+int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &area, Lbd *x, Lbd *y,
+                        Lbd *forcex, Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed,
+                        double sigma, double LX, double LY, double R, double CX, double CY, const Consts &c);
+int launch_line_ops(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
+                    const BlockDesc *d_blocks, uint32_t parity_mask, int m0, int m1, int m2, double xm);
+int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
+                  const BlockDesc *d_blocks, uint32_t parity_mask, int slot);
+int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
+                     const BlockDesc *d_blocks, uint32_t parity_mask);
+int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N);
+int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total_rtiles,
+                    double *partials, int nblocks_total, double *d_out);
+int launch_drag(void *stream, int mode, const DragArgs &a);
+int launch_decode(void *stream, int mode, const void *plane, int k, size_t n, double *out);
+int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const double *in);
+
+} // namespace mbl

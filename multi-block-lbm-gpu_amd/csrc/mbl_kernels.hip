@@ -1,0 +1,1115 @@
+// mbl_kernels.hip -- HIP kernels of the multi-block D2Q9 BGK time-stepping
+// path for gfx950 (MI355X), and their asynchronous launchers.
+//
+// Kernels (reference behaviour restated, file:line):
+//   k_stream_collide  fused pull-stream + macros + BGK + bounce-back + ghost
+//                     carry-over of multi/FH/evolution_block.h:3-126 (stb1/2/3)
+//   k_init_general    lbm/init_block.h:2-51
+//   k_line_ops        temporal_interpolation (multi/FH/assit_multi.h:213-247),
+//                     c_c_boundary_transfer (:2-46), c_f_boundary_transfer
+//                     (:249-271), f_c_boundary_transfer (:273-314)
+//   k_spline_solve    init_Spline3 gather+RHS (:48-116) and the tridiagonal
+//                     solve that cusparseDgtsv2 performs (:193-205), as an
+//                     LDS-resident parallel cyclic reduction in FP64
+//   k_spline_eval     Spline3_S + calculate_Spline3_gpu (:118-185)
+//   k_ring_fill       init_Fstore (multi/init_multi.h:3-28)
+//   k_tri_A           init_tri_A (multi/init_multi.h:30-51)
+//   k_drag            kernel_sum_Force (lbm/force_block.h:129-178), FP64
+//   k_residual_*      error_L2_area + host sum (lbm/error_L2.h:3-91), FP64
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mbl_internal.h"
+
+namespace mbl {
+
+// D2Q9 lattice, lbm/lbm.h:43-45 ordering.
+__device__ constexpr int EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+__device__ constexpr int EY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+__device__ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+__device__ constexpr double W64[9] = {4. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 36, 1. / 36, 1. / 36, 1. / 36};
+
+// ---------------------------------------------------------------------------
+// Plane storage codecs. `ld` returns the stored quantity q in compute
+// precision: q = f for ST_NATIVE, q = f - w_k for the shifted modes.
+// ---------------------------------------------------------------------------
+template <int MODE> struct Codec;
+
+template <> struct Codec<ST_NATIVE> {
+    typedef Lbd T;
+    static constexpr bool SHIFT = false;
+    __device__ static inline real_t ld(T v) { return (real_t)v; }
+    __device__ static inline T st(real_t v) { return (T)v; }
+};
+
+#ifndef MBL_FP64
+template <> struct Codec<ST_F32S> {
+    typedef float T;
+    static constexpr bool SHIFT = true;
+    __device__ static inline float ld(T v) { return v; }
+    __device__ static inline T st(float v) { return v; }
+};
+
+template <> struct Codec<ST_F16S> {
+    typedef _Float16 T;
+    static constexpr bool SHIFT = true;
+    __device__ static inline float ld(T v) { return (float)v; }
+    __device__ static inline T st(float v) { return (_Float16)v; } // round to nearest even
+};
+
+template <> struct Codec<ST_BF16S> {
+    typedef uint16_t T;
+    static constexpr bool SHIFT = true;
+    __device__ static inline float ld(T v) { return __uint_as_float(((uint32_t)v) << 16); }
+    __device__ static inline T st(float v)
+    {
+        // round to nearest even on the upper 16 bits (finite inputs)
+        uint32_t u = __float_as_uint(v);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (T)(u >> 16);
+    }
+};
+#endif
+
+// f value in double of a stored entry of direction k
+template <int MODE> __device__ inline double dec64(typename Codec<MODE>::T v, int k)
+{
+    double q = (double)Codec<MODE>::ld(v);
+    return Codec<MODE>::SHIFT ? q + W64[k] : q;
+}
+
+template <int MODE> __device__ inline typename Codec<MODE>::T enc64(double f, int k)
+{
+    return Codec<MODE>::st((real_t)(Codec<MODE>::SHIFT ? f - W64[k] : f));
+}
+
+// lbm/assit_func.h:2-9 in double
+__device__ inline double feq64(int k, double rh, double ux, double uy)
+{
+    double eu = EX[k] * ux + EY[k] * uy;
+    double uv = ux * ux + uy * uy;
+    return W64[k] * rh * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
+}
+
+// Equilibrium in the stored domain, compute precision. drho = rho - 1 is
+// passed separately so the shifted form keeps its small magnitude.
+template <bool SHIFT>
+__device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t uy, real_t uu)
+{
+    const real_t w = (real_t)W64[k];
+    real_t eu = (real_t)EX[k] * ux + (real_t)EY[k] * uy;
+    real_t poly = (real_t)3.0 * eu + (real_t)4.5 * eu * eu - (real_t)1.5 * uu;
+    if (SHIFT) return w * (drho + rho * poly);
+    return w * rho * ((real_t)1.0 + poly);
+}
+
+// BGK on one node: q[9] in, Q[9] out, macros out (evolution_block.h:60-70, 100-111).
+template <bool SHIFT>
+__device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
+{
+    real_t s = q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7] + q[8];
+    real_t jx = q[1] - q[3] + q[5] - q[6] - q[7] + q[8];
+    real_t jy = q[2] - q[4] + q[5] + q[6] - q[7] - q[8];
+    real_t drho = SHIFT ? s : s - (real_t)1.0;
+    rho = SHIFT ? (real_t)1.0 + s : s;
+    real_t inv = (real_t)1.0 / rho;
+    ux = jx * inv;
+    uy = jy * inv;
+    real_t uu = ux * ux + uy * uy;
+#pragma unroll
+    for (int k = 0; k < 9; k++) Q[k] = q[k] - (q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu)) * omega;
+}
+
+template <bool SHIFT> __device__ inline void equilibrium(real_t rho, real_t ux, real_t uy, real_t (&Q)[9])
+{
+    real_t uu = ux * ux + uy * uy;
+#pragma unroll
+    for (int k = 0; k < 9; k++) Q[k] = qeq<SHIFT>(k, rho, rho - (real_t)1.0, ux, uy, uu);
+}
+
+// Bijective XCD-aware tile order: hardware workgroup b runs on XCD b % 8;
+// give each XCD one contiguous range of tiles so neighbouring tiles (which
+// share cache lines at their x seams) meet in the same L2.
+__device__ inline int xcd_tile(int b, int G)
+{
+    if (G < 8) return b;
+    int q = G >> 3, r = G & 7;
+    int x = b & 7, local = b >> 3;
+    return x * q + (x < r ? x : r) + local;
+}
+
+template <class T, int VX> struct Vec {
+    typedef T type __attribute__((ext_vector_type(VX)));
+};
+
+template <class T, int VX> __device__ inline void vload(const T *p, T (&o)[VX])
+{
+    if constexpr (VX == 1) {
+        o[0] = p[0];
+    } else {
+        typedef typename Vec<T, VX>::type V;
+        V v = *reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int i = 0; i < VX; i++) o[i] = v[i];
+    }
+}
+
+template <class T, int VX> __device__ inline void vstore(T *p, const T (&o)[VX])
+{
+    if constexpr (VX == 1) {
+        __builtin_nontemporal_store(o[0], p);
+    } else {
+        typedef typename Vec<T, VX>::type V;
+        V v;
+#pragma unroll
+        for (int i = 0; i < VX; i++) v[i] = o[i];
+        __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// General per-node update (boundary flags, block edges, outlet); scalar.
+// Returns false when the node keeps its F (nothing written by the caller's
+// rules besides the copy done here).
+// ---------------------------------------------------------------------------
+template <int MODE, bool PERIODIC>
+__device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>::T *const (&S)[9], int i, int j,
+                                 const Consts &c, real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
+{
+    typedef Codec<MODE> C;
+    const int nx = d.nx, ny = d.ny;
+    bool edge = !PERIODIC && (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
+    if (!edge) {
+        real_t q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            int id = i - EX[k], jd = j - EY[k];
+            if (PERIODIC) {
+                id = id < 0 ? nx - 1 : (id >= nx ? 0 : id);
+                jd = jd < 0 ? ny - 1 : (jd >= ny ? 0 : jd);
+            }
+            q[k] = C::ld(S[k][(size_t)jd * nx + id]);
+        }
+        collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
+    } else {
+        // outer-edge Dirichlet equilibrium (evolution_block.h:47-57)
+        rho = (real_t)c.rho0;
+        ux = (real_t)c.U0x;
+        uy = (real_t)c.U0y;
+        equilibrium<C::SHIFT>(rho, ux, uy, Q);
+    }
+}
+
+template <int MODE, bool PERIODIC, bool FLAGS, bool MACROS>
+__device__ inline void general_node(const BlockDesc &d, const typename Codec<MODE>::T *const (&S)[9],
+                                    typename Codec<MODE>::T *const (&D)[9], int i, int j, const Consts &c,
+                                    real_t omega)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const int nx = d.nx, ny = d.ny;
+    const size_t ixy = (size_t)j * nx + i;
+    const int flag = FLAGS ? d.flag[ixy] : FLUID;
+    if (flag == SOLID_SBB || flag == MBL_SOLID_WALL) {
+        // full-way bounce-back (evolution_block.h:113-117); pulls leaving the
+        // block take the node's own value (they never reach a fluid node).
+        T v[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            int id = i - EX[k], jd = j - EY[k];
+            if (PERIODIC) {
+                id = id < 0 ? nx - 1 : (id >= nx ? 0 : id);
+                jd = jd < 0 ? ny - 1 : (jd >= ny ? 0 : jd);
+            } else if (id < 0 || id >= nx || jd < 0 || jd >= ny) {
+                id = i;
+                jd = j;
+            }
+            v[k] = S[k][(size_t)jd * nx + id];
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k][ixy] = v[OPP[k]];
+        return;
+    }
+    if (flag == FLUID) {
+        real_t Q[9], rho, ux, uy;
+        fluidlike<MODE, PERIODIC>(d, S, i, j, c, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k][ixy] = C::st(Q[k]);
+        if (MACROS) {
+            d.rho[ixy] = (Lbd)rho;
+            d.ux[ixy] = (Lbd)ux;
+            d.uy[ixy] = (Lbd)uy;
+        }
+        return;
+    }
+    if (flag == BOUNDARY_INLET || flag == BOUNDARY_OUTLET) {
+        // channel ends by non-equilibrium extrapolation (Guo, Zheng & Shi
+        // 2002) from the post-collision state of the x+1 (inlet) or x-1
+        // (outlet) neighbour, recomputed here: inlet imposes u_in(y) with the
+        // neighbour's density, outlet imposes rho0 with the neighbour's velocity.
+        const bool inlet = (flag == BOUNDARY_INLET);
+        const size_t n = inlet ? ixy + 1 : ixy - 1;
+        real_t QN[9], rN, uN, vN;
+        if (d.flag[n] == C_BOUNDARY) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) QN[k] = C::ld(S[k][n]);
+            rN = (real_t)d.rho[n];
+            uN = (real_t)d.ux[n];
+            vN = (real_t)d.uy[n];
+        } else {
+            fluidlike<MODE, PERIODIC>(d, S, inlet ? i + 1 : i - 1, j, c, omega, QN, rN, uN, vN);
+        }
+        real_t rs, us, vs;
+        if (inlet) {
+            const double y = j * d.dx + d.base_y;
+            rs = rN;
+            us = (real_t)(4.0 * c.umax * (c.LY - 1.5 - y) * (y - 0.5) / (c.LY - 2.0) / (c.LY - 2.0));
+            vs = (real_t)0.0;
+        } else {
+            rs = (real_t)c.rho0;
+            us = uN;
+            vs = vN;
+        }
+        const real_t uuN = uN * uN + vN * vN, uus = us * us + vs * vs;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            real_t v = qeq<C::SHIFT>(k, rs, rs - (real_t)1.0, us, vs, uus) +
+                       (QN[k] - qeq<C::SHIFT>(k, rN, rN - (real_t)1.0, uN, vN, uuN));
+            D[k][ixy] = C::st(v);
+        }
+        if (MACROS) {
+            d.rho[ixy] = (Lbd)rs;
+            d.ux[ixy] = (Lbd)us;
+            d.uy[ixy] = (Lbd)vs;
+        }
+        return;
+    }
+    // C_BOUNDARY (and any other flag): F carried over unchanged
+#pragma unroll
+    for (int k = 0; k < 9; k++) D[k][ixy] = S[k][ixy];
+}
+
+// ---------------------------------------------------------------------------
+// Fused stream-collide. One wave covers 64*VX consecutive nodes of one row;
+// a workgroup covers tile_rows rows. Interior all-fluid runs use aligned
+// VX-wide loads of each source row plus one scalar for the x-shifted planes.
+// ---------------------------------------------------------------------------
+template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
+__global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs, int ndesc, int tile_rows,
+                                                        uint32_t parity_mask, Consts c)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    int lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].tile_begin <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BlockDesc &d = descs[lo];
+    const int lt = t - d.tile_begin;
+    const int tyi = lt / d.tiles_x;
+    const int txi = lt - tyi * d.tiles_x;
+    const int lane = threadIdx.x & 63;
+    const int j = tyi * tile_rows + (int)(threadIdx.x >> 6);
+    const int x0 = (txi * 64 + lane) * VX;
+    const int nx = d.nx, ny = d.ny;
+    if (j >= ny || x0 >= nx) return;
+    const int par = (parity_mask >> d.level) & 1;
+    const T *S[9];
+    T *D[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        S[k] = (const T *)d.P[par][k];
+        D[k] = (T *)d.P[par ^ 1][k];
+    }
+    const real_t omega = (real_t)d.omega;
+    const size_t r0 = (size_t)j * nx;
+
+    bool fast;
+    if (PERIODIC)
+        fast = true;
+    else
+        fast = (x0 >= 1) && (x0 + VX <= nx - 1) && (j >= 1) && (j <= ny - 2);
+    if (FLAGS && fast) {
+        int fl[VX];
+        vload<int, VX>(d.flag + r0 + x0, fl);
+#pragma unroll
+        for (int v = 0; v < VX; v++) fast = fast && (fl[v] == FLUID);
+    }
+    if (!fast) {
+        for (int v = 0; v < VX; v++)
+            if (x0 + v < nx) general_node<MODE, PERIODIC, FLAGS, MACROS>(d, S, D, x0 + v, j, c, omega);
+        return;
+    }
+
+    int jm = j - 1, jp = j + 1, xm = x0 - 1, xp = x0 + VX;
+    if (PERIODIC) {
+        jm = (j == 0) ? ny - 1 : jm;
+        jp = (j == ny - 1) ? 0 : jp;
+        xm = (x0 == 0) ? nx - 1 : xm;
+        xp = (xp >= nx) ? 0 : xp;
+    }
+    const size_t rm = (size_t)jm * nx, rp = (size_t)jp * nx;
+    real_t q[9][VX];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        // pull: q_k(x) = F_k(x - e_k); EY = +1 reads row j-1, EY = -1 row j+1
+        const size_t rb = (EY[k] == 1) ? rm : ((EY[k] == -1) ? rp : r0);
+        const T *base = S[k] + rb;
+        if (EX[k] == 0) {
+            T tv[VX];
+            vload<T, VX>(base + x0, tv);
+#pragma unroll
+            for (int v = 0; v < VX; v++) q[k][v] = C::ld(tv[v]);
+        } else if (EX[k] == 1) {
+            q[k][0] = C::ld(base[xm]);
+            if constexpr (VX > 1) {
+                T tv[VX];
+                vload<T, VX>(base + x0, tv);
+#pragma unroll
+                for (int v = 1; v < VX; v++) q[k][v] = C::ld(tv[v - 1]);
+            }
+        } else {
+            q[k][VX - 1] = C::ld(base[xp]);
+            if constexpr (VX > 1) {
+                T tv[VX];
+                vload<T, VX>(base + x0, tv);
+#pragma unroll
+                for (int v = 0; v < VX - 1; v++) q[k][v] = C::ld(tv[v + 1]);
+            }
+        }
+    }
+    T out[9][VX];
+    Lbd mr[VX], mx[VX], my[VX];
+#pragma unroll
+    for (int v = 0; v < VX; v++) {
+        real_t qq[9], Q[9], rho, ux, uy;
+#pragma unroll
+        for (int k = 0; k < 9; k++) qq[k] = q[k][v];
+        collide<C::SHIFT>(qq, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) out[k][v] = C::st(Q[k]);
+        mr[v] = (Lbd)rho;
+        mx[v] = (Lbd)ux;
+        my[v] = (Lbd)uy;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) vstore<T, VX>(D[k] + r0 + x0, out[k]);
+    if (MACROS) {
+        vstore<Lbd, VX>(d.rho + r0 + x0, mr);
+        vstore<Lbd, VX>(d.ux + r0 + x0, mx);
+        vstore<Lbd, VX>(d.uy + r0 + x0, my);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// init_General (lbm/init_block.h:2-51)
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// counter-based normal pair: SplitMix64 finaliser on (seed, i, j), Box-Muller
+__device__ inline void random_velocity(uint64_t seed, double sigma, int i, int j, double &ux, double &uy)
+{
+    uint64_t key = ((uint64_t)(uint32_t)j << 32) | (uint64_t)(uint32_t)i;
+    uint64_t a = splitmix64(seed + (2 * key + 1) * 0x9E3779B97F4A7C15ULL);
+    uint64_t b = splitmix64(seed + (2 * key + 2) * 0x9E3779B97F4A7C15ULL);
+    double u1 = (double)((a >> 11) + 1) * (1.0 / 9007199254740992.0);
+    double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+    double r = sigma * sqrt(-2.0 * log(u1));
+    double th = 6.283185307179586 * u2;
+    ux = r * cos(th);
+    uy = r * sin(th);
+}
+
+struct InitArgs {
+    BlockDesc d;
+    Lbd *x, *y, *fx, *fy;
+    int bc_mode, init_mode;
+    unsigned long long seed;
+    double sigma, LX, LY, R, CX, CY, base_x;
+    int m;
+    Consts c;
+};
+
+template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitArgs a)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const BlockDesc &d = a.d;
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= d.nx || j >= d.ny) return;
+    const size_t ixy = (size_t)j * d.nx + i;
+    const double dx = 1.0 / a.m;
+    const double X = i * dx + a.base_x;
+    const double Y = j * dx + d.base_y;
+    a.x[ixy] = (Lbd)X;
+    a.y[ixy] = (Lbd)Y;
+    double rho = a.c.rho0, ux = a.c.U0x, uy = a.c.U0y;
+    if (a.init_mode == MBL_INIT_REST) {
+        ux = 0.0;
+        uy = 0.0;
+    } else if (a.init_mode == MBL_INIT_RANDOM) {
+        random_velocity(a.seed, a.sigma, i, j, ux, uy);
+    }
+    d.rho[ixy] = (Lbd)rho;
+    d.ux[ixy] = (Lbd)ux;
+    d.uy[ixy] = (Lbd)uy;
+    if (d.upx) d.upx[ixy] = (Lbd)ux;
+    if (d.upy) d.upy[ixy] = (Lbd)uy;
+    a.fx[ixy] = (Lbd)0.0;
+    a.fy[ixy] = (Lbd)0.0;
+    int flag;
+    if ((X - a.CX) * (X - a.CX) + (Y - a.CY) * (Y - a.CY) < a.R * a.R)
+        flag = SOLID_SBB;
+    else if (a.bc_mode == MBL_BC_PERIODIC)
+        flag = FLUID;
+    else if (a.bc_mode == MBL_BC_CHANNEL) {
+        if (Y == 0.0 || Y == a.LY - 1.0)
+            flag = MBL_SOLID_WALL;
+        else if (X == 0.0)
+            flag = BOUNDARY_INLET;
+        else if (X == a.LX - 1.0)
+            flag = BOUNDARY_OUTLET;
+        else if (i == 0 || j == 0 || i == d.nx - 1 || j == d.ny - 1)
+            flag = C_BOUNDARY;
+        else
+            flag = FLUID;
+    } else {
+        if (Y == 0.0 || Y == a.LY - 1.0 || X == 0.0 || X == a.LX - 1.0)
+            flag = FLUID;
+        else if (i == 0 || j == 0 || i == d.nx - 1 || j == d.ny - 1)
+            flag = C_BOUNDARY;
+        else
+            flag = FLUID;
+    }
+    ((int *)d.flag)[ixy] = flag;
+    // F = f = feq(rho, u) in double, stored in the plane's domain
+    const double uv = ux * ux + uy * uy;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        double eu = EX[k] * ux + EY[k] * uy;
+        double poly = 3.0 * eu + 4.5 * eu * eu - 1.5 * uv;
+        double q = C::SHIFT ? W64[k] * ((rho - 1.0) + rho * poly) : W64[k] * rho * (1.0 + poly);
+        T v = C::st((real_t)q);
+        ((T *)d.P[0][k])[ixy] = v;
+        ((T *)d.P[1][k])[ixy] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interface line ops
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
+                                                  uint32_t parity_mask, int m0, int m1, int m2, double xm)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const LineOp &op = ops[blockIdx.y];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= op.n) return;
+    const uint8_t mask = op.mask[e];
+    if (!mask) return;
+    const BlockDesc &db = blocks[op.dst_block];
+    const int pd = (parity_mask >> db.level) & 1;
+    const int didx = op.dst_idx[e];
+    T *D[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) D[k] = (T *)db.P[pd][k];
+    if (op.type == OP_TEMPORAL) {
+        // 3-point Lagrange in time (assit_multi.h:232-245)
+        const double a = 0.5 * (xm - 1.0) * xm;
+        const double b = (xm - 1.0) * (xm + 1.0);
+        const double cc = 0.5 * xm * (xm + 1.0);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            double v = a * (double)op.ring[m0][k][e] - b * (double)op.ring[m1][k][e] + cc * (double)op.ring[m2][k][e];
+            D[k][didx] = enc64<MODE>(v, k);
+        }
+        return;
+    }
+    if (op.type == OP_CF) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k][didx] = enc64<MODE>((double)op.ring[m2][k][e], k);
+        return;
+    }
+    const BlockDesc &sb = blocks[op.src_block];
+    const int ps = (parity_mask >> sb.level) & 1;
+    const int sidx = op.src_idx[e];
+    const T *S[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) S[k] = (const T *)sb.P[ps][k];
+    if (mask & 1) {
+        db.rho[didx] = sb.rho[sidx];
+        db.ux[didx] = sb.ux[sidx];
+        db.uy[didx] = sb.uy[sidx];
+    }
+    if (!(mask & 2)) return;
+    if (op.type == OP_CC) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k][didx] = S[k][sidx];
+    } else { // OP_FC: non-equilibrium rescaling fine -> coarse (assit_multi.h:304-311)
+        const double r = (double)sb.rho[sidx], ux = (double)sb.ux[sidx], uy = (double)sb.uy[sidx];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            double ft = dec64<MODE>(S[k][sidx], k);
+            double fe = feq64(k, r, ux, uy);
+            D[k][didx] = enc64<MODE>(fe + op.para * (ft - fe), k);
+        }
+    }
+}
+
+// init_Fstore: every ring slot <- the fine edge's F (init_multi.h:3-28)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ring_fill(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
+                                                   uint32_t parity_mask)
+{
+    typedef typename Codec<MODE>::T T;
+    const LineOp &op = ops[blockIdx.y];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= op.n) return;
+    const BlockDesc &db = blocks[op.dst_block];
+    const int pd = (parity_mask >> db.level) & 1;
+    const int idx = op.dst_idx[e];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        Lbd v = (Lbd)dec64<MODE>(((const T *)db.P[pd][k])[idx], k);
+        op.ring[0][k][e] = v;
+        op.ring[1][k][e] = v;
+        op.ring[2][k][e] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spline: workgroup (column c, line op). Gathers one of the 12 columns of the
+// coarse line (rho, ux, uy, F0..F8), builds the as-coded right-hand side
+// 6g_i - 3g_{i-1} - 3g_{i+1} with zero end rows (assit_multi.h:82-107), and
+// solves tridiag(l, m, u) M = rhs by parallel cyclic reduction (Hockney 1965)
+// with the rows in registers and one LDS exchange per level, in FP64.
+// ---------------------------------------------------------------------------
+constexpr int SPL_TPB = 256;
+constexpr int SPL_MAX_NC = 2048;
+
+template <int MODE, int R>
+__global__ __launch_bounds__(SPL_TPB) void k_spline_solve(const LineOp *__restrict__ ops,
+                                                          const BlockDesc *__restrict__ blocks, uint32_t parity_mask)
+{
+    typedef typename Codec<MODE>::T T;
+    const LineOp &op = ops[blockIdx.y];
+    const int col = blockIdx.x;
+    const int N = op.nc;
+    const BlockDesc &sb = blocks[op.src_block];
+    const int ps = (parity_mask >> sb.level) & 1;
+    extern __shared__ double lds[]; // 4*N doubles: a | b | c | d
+    double *La = lds, *Lb = lds + N, *Lc = lds + 2 * N, *Ld = lds + 3 * N;
+    double a[R], b[R], c[R], dd[R], g[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ip = threadIdx.x + r * SPL_TPB;
+        g[r] = 0.0;
+        if (ip < N) {
+            const int node = op.src_idx[ip];
+            double v;
+            if (col == 0)
+                v = (double)sb.rho[node];
+            else if (col == 1)
+                v = (double)sb.ux[node];
+            else if (col == 2)
+                v = (double)sb.uy[node];
+            else
+                v = dec64<MODE>(((const T *)sb.P[ps][col - 3])[node], col - 3);
+            g[r] = v;
+            op.gcol[col][ip] = (Lbd)v;
+            Ld[ip] = v;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ip = threadIdx.x + r * SPL_TPB;
+        a[r] = 0.0;
+        b[r] = 1.0;
+        c[r] = 0.0;
+        dd[r] = 0.0;
+        if (ip < N) {
+            a[r] = (double)op.tl[ip];
+            b[r] = (double)op.tm[ip];
+            c[r] = (double)op.tu[ip];
+            dd[r] = (ip == 0 || ip == N - 1) ? 0.0 : (6.0 * g[r] - 3.0 * Ld[ip - 1] - 3.0 * Ld[ip + 1]);
+        }
+    }
+    for (int s = 1; s < N; s <<= 1) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ip = threadIdx.x + r * SPL_TPB;
+            if (ip < N) {
+                La[ip] = a[r];
+                Lb[ip] = b[r];
+                Lc[ip] = c[r];
+                Ld[ip] = dd[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ip = threadIdx.x + r * SPL_TPB;
+            if (ip >= N) continue;
+            const int im = ip - s, iq = ip + s;
+            double na = 0.0, nc = 0.0, nb = b[r], nd = dd[r];
+            if (im >= 0) {
+                const double alpha = -a[r] / Lb[im];
+                na = alpha * La[im];
+                nb += alpha * Lc[im];
+                nd += alpha * Ld[im];
+            }
+            if (iq < N) {
+                const double gamma = -c[r] / Lb[iq];
+                nc = gamma * Lc[iq];
+                nb += gamma * La[iq];
+                nd += gamma * Ld[iq];
+            }
+            a[r] = na;
+            b[r] = nb;
+            c[r] = nc;
+            dd[r] = nd;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ip = threadIdx.x + r * SPL_TPB;
+        if (ip < N) {
+            const double M = dd[r] / b[r];
+            op.M64[(size_t)col * N + ip] = M;
+            op.varibs[(size_t)col * N + ip] = (Lbd)M;
+        }
+    }
+}
+
+// assit_multi.h:118-134, the literal 0.1666667 included
+__device__ inline double spline3_s(double xx, double x_1, double h, double fi_1, double fi, double mi_1, double mi)
+{
+    double x = x_1 + h;
+    double a = mi_1 * (x - xx) * (x - xx) * (x - xx) * 0.1666667;
+    double b = mi * (xx - x_1) * (xx - x_1) * (xx - x_1) * 0.1666667;
+    double c = (fi_1 - mi_1 * h * h * 0.1666667) * (x - xx);
+    double d = (fi - mi * h * h * 0.1666667) * (xx - x_1);
+    return (a + b + c + d);
+}
+
+// calculate_Spline3_gpu (assit_multi.h:136-185): evaluate at fine point ip,
+// write the fine edge's rho, u and the rescaled F into ring slot `slot`.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
+                                                     uint32_t parity_mask, int slot)
+{
+    typedef typename Codec<MODE>::T T;
+    const LineOp &op = ops[blockIdx.y];
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= op.n) return;
+    const int N = op.nc;
+    const BlockDesc &sb = blocks[op.src_block];
+    const int ps = (parity_mask >> sb.level) & 1;
+    const double xx = (double)ip / (double)op.ratio;
+    const int I = (ceil(xx) == 0.0) ? (int)(ceil(xx) + 1.0) : (int)ceil(xx);
+    const int n0 = op.src_idx[I - 1], n1 = op.src_idx[I];
+    const double *M = op.M64;
+    const double rho = spline3_s(xx, I - 1.0, 1.0, (double)sb.rho[n0], (double)sb.rho[n1], M[I - 1], M[I]);
+    const double ux = spline3_s(xx, I - 1.0, 1.0, (double)sb.ux[n0], (double)sb.ux[n1], M[N + I - 1], M[N + I]);
+    const double uy =
+        spline3_s(xx, I - 1.0, 1.0, (double)sb.uy[n0], (double)sb.uy[n1], M[2 * N + I - 1], M[2 * N + I]);
+    if (op.mask[ip] & 1) {
+        const BlockDesc &db = blocks[op.dst_block];
+        const int idx = op.dst_idx[ip];
+        db.rho[idx] = (Lbd)rho;
+        db.ux[idx] = (Lbd)ux;
+        db.uy[idx] = (Lbd)uy;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const T *Sk = (const T *)sb.P[ps][k];
+        const double ft = spline3_s(xx, I - 1.0, 1.0, dec64<MODE>(Sk[n0], k), dec64<MODE>(Sk[n1], k),
+                                    M[(3 + k) * N + I - 1], M[(3 + k) * N + I]);
+        const double fe = feq64(k, rho, ux, uy);
+        op.ring[slot][k][ip] = (Lbd)(fe + op.para * (ft - fe));
+    }
+}
+
+// init_tri_A (multi/init_multi.h:30-51), final race-free state
+__global__ void k_tri_A(Lbd *l, Lbd *m, Lbd *u, int N)
+{
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= N) return;
+    m[ip] = (Lbd)2.0;
+    l[ip] = (ip == 0 || ip == 1 || ip == N - 1) ? (Lbd)0.0 : (Lbd)0.5;
+    u[ip] = (ip == 0 || ip == N - 2 || ip == N - 1) ? (Lbd)0.0 : (Lbd)0.5;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// sums v over the workgroup (blockDim.x multiple of 64, <= 1024); result in thread 0
+__device__ inline double group_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nw; i++) s += sh[i];
+    return s;
+}
+
+// Momentum exchange (force_block.h:129-178) from the previous state planes:
+// f_post[k][x_s] = Fold[k][x_f], f_post[opp k][x_f] = Fold[opp k][x_s].
+template <int MODE> __global__ __launch_bounds__(1024) void k_drag(DragArgs a)
+{
+    typedef typename Codec<MODE>::T T;
+    __shared__ double sh[16];
+    double fx = 0.0, fy = 0.0;
+    for (int e = threadIdx.x; e < a.n; e += blockDim.x) {
+        const Int3 L = a.links[e];
+        const int k = L.k, kd = OPP[k];
+        const size_t xs = (size_t)L.j * a.nx + L.i;
+        const size_t xf = (size_t)(L.j - EY[k]) * a.nx + (L.i - EX[k]);
+        const double fk = dec64<MODE>(((const T *)a.Fold[k])[xf], k);
+        const double fkd = dec64<MODE>(((const T *)a.Fold[kd])[xs], kd);
+        fx += (fk * EX[k] - fkd * EX[kd]);
+        fy += (fk * EY[k] - fkd * EY[kd]);
+    }
+    const double sx = group_sum(fx, sh);
+    const double sy = group_sum(fy, sh);
+    if (threadIdx.x == 0) {
+        a.out[0] = sx;
+        a.out[1] = sy;
+    }
+}
+
+// error_L2_area (error_L2.h:3-55): tiles of 64 x 16 nodes, FP64 partial sums
+// of |u - u_prev|^2 and |u|^2 per tile, u_prev <- u.
+__global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restrict__ descs, int ndesc, double *partials)
+{
+    __shared__ double sh[16];
+    const int t = blockIdx.x;
+    int lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (descs[mid].rtile_begin <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BlockDesc &d = descs[lo];
+    const int lt = t - d.rtile_begin;
+    const int ty = lt / d.rtiles_x, tx = lt - ty * d.rtiles_x;
+    const int i = tx * 64 + (threadIdx.x & 63);
+    double s1 = 0.0, s2 = 0.0;
+    if (i < d.nx) {
+        for (int r = 0; r < 4; r++) {
+            const int j = ty * 16 + r * 4 + (threadIdx.x >> 6);
+            if (j >= d.ny) break;
+            const size_t ixy = (size_t)j * d.nx + i;
+            const double ux = (double)d.ux[ixy], uy = (double)d.uy[ixy];
+            const double upx = (double)d.upx[ixy], upy = (double)d.upy[ixy];
+            s1 += (ux - upx) * (ux - upx) + (uy - upy) * (uy - upy);
+            s2 += ux * ux + uy * uy;
+            d.upx[ixy] = d.ux[ixy];
+            d.upy[ixy] = d.uy[ixy];
+        }
+    }
+    const double a1 = group_sum(s1, sh);
+    const double a2 = group_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        partials[2 * (size_t)t] = a1;
+        partials[2 * (size_t)t + 1] = a2;
+    }
+}
+
+// error_L2.h:279-302 host sum: per block in block order, weighted 1/m.
+__global__ __launch_bounds__(256) void k_residual_final(const BlockDesc *__restrict__ descs, int ndesc, int total,
+                                                        const double *partials, double *out)
+{
+    __shared__ double sh[16];
+    double all1 = 0.0, all2 = 0.0;
+    for (int a = 0; a < ndesc; a++) {
+        const int b0 = descs[a].rtile_begin;
+        const int b1 = (a + 1 < ndesc) ? descs[a + 1].rtile_begin : total;
+        double s1 = 0.0, s2 = 0.0;
+        for (int t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
+            s1 += partials[2 * (size_t)t];
+            s2 += partials[2 * (size_t)t + 1];
+        }
+        const double g1 = group_sum(s1, sh);
+        const double g2 = group_sum(s2, sh);
+        if (threadIdx.x == 0) {
+            all1 += g1 / descs[a].m;
+            all2 += g2 / descs[a].m;
+        }
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(all1) / sqrt(all2);
+        out[1] = all1;
+        out[2] = all2;
+    }
+}
+
+template <int MODE> __global__ void k_decode(const void *plane, int k, size_t n, double *out)
+{
+    typedef typename Codec<MODE>::T T;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = dec64<MODE>(((const T *)plane)[i], k);
+}
+
+template <int MODE> __global__ void k_encode(void *plane, int k, size_t n, const double *in)
+{
+    typedef typename Codec<MODE>::T T;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ((T *)plane)[i] = enc64<MODE>(in[i], k);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define MBL_LAUNCH_CHECK()                                                                                             \
+    do {                                                                                                               \
+        if (hipGetLastError() != hipSuccess) return MBL_E_HIP;                                                         \
+    } while (0)
+
+template <int MODE, int VX>
+static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d, int nd,
+                       int tiles, int rows, uint32_t pm, const Consts &c)
+{
+    dim3 g(tiles), b(tpb);
+    if (periodic) {
+        if (flags) {
+            if (macros)
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, nd, rows, pm, c);
+            else
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, nd, rows, pm, c);
+        } else {
+            if (macros)
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, nd, rows, pm, c);
+            else
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, nd, rows, pm, c);
+        }
+    } else {
+        if (macros)
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, nd, rows, pm, c);
+        else
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, nd, rows, pm, c);
+    }
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+template <int MODE>
+static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d, int nd,
+                   int tiles, int rows, uint32_t pm, const Consts &c)
+{
+    switch (vx) {
+    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, c);
+    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, c);
+    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, c);
+    default: return MBL_E_INVALID;
+    }
+}
+
+// Instantiates a templated launcher body once per storage mode of the build.
+#ifdef MBL_FP64
+#define MBL_MODE_SWITCH(mode, CALL)                                                                                    \
+    switch (mode) {                                                                                                    \
+    case ST_NATIVE: {                                                                                                  \
+        constexpr int M_ = ST_NATIVE;                                                                                  \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
+    default: return MBL_E_UNSUPPORTED;                                                                                 \
+    }
+#else
+#define MBL_MODE_SWITCH(mode, CALL)                                                                                    \
+    switch (mode) {                                                                                                    \
+    case ST_NATIVE: {                                                                                                  \
+        constexpr int M_ = ST_NATIVE;                                                                                  \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
+    case ST_F32S: {                                                                                                    \
+        constexpr int M_ = ST_F32S;                                                                                    \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
+    case ST_F16S: {                                                                                                    \
+        constexpr int M_ = ST_F16S;                                                                                    \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
+    case ST_BF16S: {                                                                                                   \
+        constexpr int M_ = ST_BF16S;                                                                                   \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
+    default: return MBL_E_UNSUPPORTED;                                                                                 \
+    }
+#endif
+
+int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
+                          const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows, uint32_t parity_mask,
+                          const Consts &c)
+{
+    if (total_tiles <= 0) return 0;
+    if (tpb != 64 * tile_rows || tpb > 256) return MBL_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, return sc_mode<M_>(s, vx, tpb, periodic, flags, macros, d_descs, ndesc, total_tiles,
+                                             tile_rows, parity_mask, c));
+    return 0;
+}
+
+int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &area, Lbd *x, Lbd *y, Lbd *forcex,
+                        Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed, double sigma, double LX,
+                        double LY, double R, double CX, double CY, const Consts &c)
+{
+    InitArgs a;
+    a.d = d;
+    a.x = x;
+    a.y = y;
+    a.fx = forcex;
+    a.fy = forcey;
+    a.bc_mode = bc_mode;
+    a.init_mode = init_mode;
+    a.seed = seed;
+    a.sigma = sigma;
+    a.LX = LX;
+    a.LY = LY;
+    a.R = R;
+    a.CX = CX;
+    a.CY = CY;
+    a.base_x = area.base_x;
+    a.m = area.m;
+    a.c = c;
+    dim3 g((area.nx + 15) / 16, (area.ny + 15) / 16), b(256);
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_init_general<M_>), g, b, 0, s, a));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_line_ops(void *stream, int mode, const LineOp *d_ops, int nops, int max_n, const BlockDesc *d_blocks,
+                    uint32_t parity_mask, int m0, int m1, int m2, double xm)
+{
+    if (nops <= 0 || max_n <= 0) return 0;
+    dim3 g((max_n + 255) / 256, nops), b(256);
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_line_ops<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, m0, m1, m2,
+                                             xm));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n, const BlockDesc *d_blocks,
+                     uint32_t parity_mask)
+{
+    if (nops <= 0 || max_n <= 0) return 0;
+    dim3 g((max_n + 255) / 256, nops), b(256);
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_ring_fill<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+template <int MODE>
+static int spline_solve_r(hipStream_t s, int R, const LineOp *ops, int nops, size_t lds, const BlockDesc *blocks,
+                          uint32_t pm)
+{
+    dim3 g(12, nops), b(SPL_TPB);
+    switch (R) {
+    case 1: hipLaunchKernelGGL((k_spline_solve<MODE, 1>), g, b, lds, s, ops, blocks, pm); break;
+    case 2: hipLaunchKernelGGL((k_spline_solve<MODE, 2>), g, b, lds, s, ops, blocks, pm); break;
+    case 4: hipLaunchKernelGGL((k_spline_solve<MODE, 4>), g, b, lds, s, ops, blocks, pm); break;
+    case 8: hipLaunchKernelGGL((k_spline_solve<MODE, 8>), g, b, lds, s, ops, blocks, pm); break;
+    default: return MBL_E_UNSUPPORTED;
+    }
+    return 0;
+}
+
+int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
+                  const BlockDesc *d_blocks, uint32_t parity_mask, int slot)
+{
+    if (nops <= 0) return 0;
+    if (max_nc > SPL_MAX_NC) return MBL_E_UNSUPPORTED;
+    int R = 1;
+    while (R * SPL_TPB < max_nc) R <<= 1;
+    const size_t lds = 4 * sizeof(double) * (size_t)max_nc;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = 0;
+    MBL_MODE_SWITCH(mode, rc = spline_solve_r<M_>(s, R, d_ops, nops, lds, d_blocks, parity_mask));
+    if (rc) return rc;
+    MBL_LAUNCH_CHECK();
+    dim3 g((max_nf + 255) / 256, nops), b(256);
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_eval<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, slot));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N)
+{
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_tri_A, dim3((N + 255) / 256), dim3(256), 0, (hipStream_t)stream, l, m, u, N);
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total_rtiles, double *partials,
+                    int nblocks_total, double *d_out)
+{
+    (void)nblocks_total;
+    hipStream_t s = (hipStream_t)stream;
+    if (total_rtiles > 0) hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, ndesc, partials);
+    MBL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, s, d_descs, ndesc, total_rtiles, partials, d_out);
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_drag(void *stream, int mode, const DragArgs &a)
+{
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_drag<M_>), dim3(1), dim3(1024), 0, s, a));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_decode(void *stream, int mode, const void *plane, int k, size_t n, double *out)
+{
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 g((unsigned)((n + 255) / 256)), b(256);
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_decode<M_>), g, b, 0, s, plane, k, n, out));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const double *in)
+{
+    if (n == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 g((unsigned)((n + 255) / 256)), b(256);
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_encode<M_>), g, b, 0, s, plane, k, n, in));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+} // namespace mbl

@@ -1,0 +1,120 @@
+"""Multi-block runs through the reference entry points (init_General,
+init_Fstore, init_tri_A, sum_N_drag, fdrag_store, evolution, error_L2,
+getDrag_block) of the HIP library against the FP64 oracle's whole-run
+restatement of Multi-block-LBM.cu:main.
+
+The FP64 build must agree with the oracle to rounding (1e-11 relative on
+every field and diagnostic); the FP32 build is held to the tolerances written
+in each test.
+"""
+import numpy as np
+import pytest
+
+import mbl
+from mbl import abi
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def tiny2(lib):
+    # 2 levels: 60x24-cell channel, fine block 20x12 cells, cylinder D=5 coarse
+    return mbl.Geometry.channel(lib, 61, 25, [(10, 30, 6, 18)], R=2.5, CX=18.0, CY=12.0, Re=20.0)
+
+
+def tiny3(lib):
+    # 3 levels: 80x32-cell channel
+    return mbl.Geometry.channel(lib, 81, 33, [(12, 44, 6, 26), (16, 36, 10, 22)], R=2.0, CX=22.0, CY=16.0,
+                                Re=40.0)
+
+
+def run_pair(lib, geom, steps, **kw):
+    sim = mbl.MultiBlockSim(lib, geom, **kw)
+    o = oracle.OracleSim(geom.g, sim.cfg)
+    sim.step(steps)
+    o.run(steps)
+    return sim, o
+
+
+def assert_fields_close(sim, o, rtol, atol_f):
+    worst = 0.0
+    for a in range(sim.na):
+        assert np.array_equal(sim.read(a, "solid"), o.get(a, "solid").astype(np.int32))
+        for fld in ("rho", "ux", "uy"):
+            g, r = sim.read(a, fld).astype(np.float64), o.get(a, fld)
+            scale = max(1e-3, np.max(np.abs(r)))
+            err = np.max(np.abs(g - r)) / scale
+            worst = max(worst, err)
+            assert err < rtol, (a, fld, err)
+        for k in range(9):
+            g, r = sim.read(a, ("F", k)), o.get(a, ("F", k))
+            err = np.max(np.abs(g - r))
+            assert err < atol_f, (a, k, err)
+    return worst
+
+
+def assert_series_close(sim, o, rtol):
+    l2, dr = sim.series()
+    for name, g, r in (("residual", l2["residual"], o.series(0)), ("Cd", dr["Cd"], o.series(3)),
+                       ("Cl", dr["Cl"], o.series(4))):
+        assert len(g) == len(r), name
+        scale = max(1e-12, np.max(np.abs(r)))
+        err = np.max(np.abs(g - r)) / scale
+        print(name, "max rel err", err)
+        assert err < rtol, (name, err)
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_tiny2_fp64_matches_oracle(lib64, use_graph):
+    geom = tiny2(lib64)
+    sim, o = run_pair(lib64, geom, 40, use_graph=use_graph)
+    assert sim.n_drag == len(o.links())
+    assert_fields_close(sim, o, 1e-11, 1e-12)
+    assert_series_close(sim, o, 1e-10)
+    for b in range(sim.nl):
+        assert list(sim.t3[b].marker) == o.markers(b)
+    sim.close()
+
+
+def test_tiny3_fp64_matches_oracle(lib64):
+    geom = tiny3(lib64)
+    sim, o = run_pair(lib64, geom, 24)
+    assert_fields_close(sim, o, 1e-11, 1e-12)
+    assert_series_close(sim, o, 1e-10)
+    sim.close()
+
+
+def test_geo_2_5_reference_mode_fp64(lib64):
+    """The reference's own default layout and boundary rule (equilibrium on all outer edges)."""
+    geom = mbl.Geometry.preset(lib64, "GEO_2_5")
+    sim, o = run_pair(lib64, geom, 6)
+    assert sim.n_drag == 384
+    assert_fields_close(sim, o, 1e-11, 1e-12)
+    assert_series_close(sim, o, 1e-10)
+    sim.close()
+
+
+def test_tiny2_fp32_close_to_oracle(lib32):
+    geom = tiny2(lib32)
+    sim, o = run_pair(lib32, geom, 40)
+    worst = assert_fields_close(sim, o, 5e-4, 5e-6)
+    print("fp32 worst field rel err", worst)
+    sim.close()
+
+
+def test_graph_replay_bitwise_equal(lib32):
+    geom = tiny2(lib32)
+    a = mbl.MultiBlockSim(lib32, geom, use_graph=0)
+    b = mbl.MultiBlockSim(lib32, geom, use_graph=1)
+    for _ in range(3):
+        a.step(7)
+        b.step(7)
+        for blk in range(a.na):
+            for k in range(9):
+                assert np.array_equal(a.read(blk, ("F", k)), b.read(blk, ("F", k)))
+            assert np.array_equal(a.read(blk, "ux"), b.read(blk, "ux"))
+    sa, sb = a.series(), b.series()
+    assert np.array_equal(sa[0]["residual"], sb[0]["residual"])
+    assert np.array_equal(sa[1]["Cd"], sb[1]["Cd"])
+    a.close()
+    b.close()

@@ -1,0 +1,94 @@
+"""Single periodic block (the bandwidth-benchmark workload) on the GPU against
+the FP64 oracle (oracle/oracle.c: stb1/stb2/stb3 of
+multi/FH/evolution_block.h:3-126 with periodic pulls).
+
+Tolerances, after 20 steps at tau = 0.6 from u ~ N(0, 0.01^2):
+  FP64 planes            |F - F_oracle| <= 1e-13
+  FP32 planes (f)        <= 2e-6   (FP32 rounding of f ~ 0.1-0.45, accumulated)
+  FP32 planes (f - w)    <= 2e-7
+  FP16 planes (f - w)    <= 2e-4   (binary16: 11-bit significand of |f - w| <~ 0.05)
+  BF16 planes (f - w)    <= 2e-3   (bfloat16: 8-bit significand)
+"""
+import numpy as np
+import pytest
+
+import mbl
+from mbl import abi
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, STEPS, TAU = 96, 64, 20, 0.6
+
+
+@pytest.fixture(scope="module")
+def ref():
+    o = oracle.OraclePeriodic(NX, NY, tau=TAU)
+    o.run(STEPS)
+    F = np.stack([o.get(("F", k)) for k in range(9)])
+    o.close()
+    return F
+
+
+def run_gpu(lib, storage=abi.STORE_NATIVE, tuning=None, nx=NX, ny=NY, steps=STEPS):
+    b = mbl.PeriodicBlock(lib, nx, ny, tau=TAU, storage=storage, tuning=tuning)
+    b.step(steps)
+    b.sync()
+    F = np.stack([b.read_F(k) for k in range(9)])
+    b.close()
+    return F
+
+
+def test_init_matches_oracle(lib64):
+    o = oracle.OraclePeriodic(NX, NY, tau=TAU)
+    F0 = np.stack([o.get(("F", k)) for k in range(9)])
+    g = run_gpu(lib64, steps=0)
+    assert np.max(np.abs(g - F0)) < 1e-15
+
+
+def test_fp64_matches_oracle(lib64, ref):
+    g = run_gpu(lib64)
+    err = np.max(np.abs(g - ref))
+    print("fp64 max err", err)
+    assert err < 1e-13
+
+
+@pytest.mark.parametrize("storage,tol", [(abi.STORE_NATIVE, 2e-6), (abi.STORE_F32S, 2e-7),
+                                         (abi.STORE_F16S, 2e-4), (abi.STORE_BF16S, 2e-3)])
+def test_fp32_storage_modes(lib32, ref, storage, tol):
+    g = run_gpu(lib32, storage=storage)
+    err = np.max(np.abs(g - ref))
+    print("storage", storage, "max err", err)
+    assert err < tol
+
+
+@pytest.mark.parametrize("vx", [1, 2, 4])
+@pytest.mark.parametrize("tpb", [64, 128, 256])
+def test_tilings_bitwise_equal(lib32, vx, tpb):
+    base = run_gpu(lib32, tuning=(1, 256), nx=128, ny=40, steps=7)
+    g = run_gpu(lib32, tuning=(vx, tpb), nx=128, ny=40, steps=7)
+    assert np.array_equal(base, g)
+
+
+def test_mass_and_momentum_conserved(lib64):
+    b = mbl.PeriodicBlock(lib64, NX, NY, tau=TAU)
+    F0 = np.stack([b.read_F(k) for k in range(9)])
+    b.step(50)
+    b.sync()
+    F1 = np.stack([b.read_F(k) for k in range(9)])
+    b.close()
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])[:, None, None]
+    ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])[:, None, None]
+    assert abs(F1.sum() - F0.sum()) < 1e-10 * F0.sum()
+    assert abs((F1 * ex).sum() - (F0 * ex).sum()) < 1e-12 * NX * NY
+    assert abs((F1 * ey).sum() - (F0 * ey).sum()) < 1e-12 * NX * NY
+
+
+def test_odd_width_periodic(lib32, lib64):
+    """nx not a multiple of the vector width falls back to one node per thread."""
+    o = oracle.OraclePeriodic(37, 23, tau=TAU)
+    o.run(9)
+    F = np.stack([o.get(("F", k)) for k in range(9)])
+    o.close()
+    g = run_gpu(lib64, nx=37, ny=23, steps=9)
+    assert np.max(np.abs(g - F)) < 1e-13
