@@ -53,6 +53,23 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
             "mlups": nodes * steps / wall / 1e6, "nodes": nodes}
 
 
+def copy_calibration(mib):
+    """Five device-to-device copies of a known size: what FETCH_SIZE / WRITE_SIZE
+    read for them calibrates the counters against bytes actually moved."""
+    import ctypes
+    import mbl
+
+    lib = mbl.load_library(False)
+    n = mib << 20
+    a, b = lib.mbl_malloc(n), lib.mbl_malloc(n)
+    lib.mbl_memset_d(a, 1, n)
+    for _ in range(5):
+        lib.mbl_memcpy_d2d(b, a, n)
+    lib.mbl_free(a)
+    lib.mbl_free(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
+
+
 def cpu_baseline(target_s):
     """FP64 oracle on a 4096 x 4096 periodic sample, steps sized to ~target_s seconds."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -80,7 +97,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", action="store_true", help="also time the 16-bit storage modes")
+    ap.add_argument("--vx", type=int, default=0, help="nodes per thread (0: library default)")
+    ap.add_argument("--tpb", type=int, default=0, help="threads per workgroup (0: library default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design, not a run)", file=sys.stderr)
@@ -90,7 +114,8 @@ def main():
     lib = mbl.load_library(False)
     print(f"# bench: periodic {args.n}^2 storage={STORAGE_NAMES[args.storage]} steps={args.steps}", file=sys.stderr,
           flush=True)
-    r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage)
+    tuning = (args.vx, args.tpb) if (args.vx or args.tpb) else None
+    r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage, tuning)
     bpn = BYTES_PER_NODE[args.storage]
     achieved = bpn * r["nodes"] / (r["per_launch_ms"] * 1e-3) / 1e9
     out = {

@@ -73,10 +73,12 @@ typedef unsigned int Lbui;
 
 /* Node flags. FLUID, SOLID_SBB, BOUNDARY_INLET, BOUNDARY_OUTLET and
  * C_BOUNDARY keep the reference's values (lbm/lbm.h:26-32). The reference
- * declares INLET/OUTLET without using them; here they are the Dirichlet
- * equilibrium inlet (parabolic profile) and the zero-gradient outlet of the
- * channel boundary mode. MBL_SOLID_WALL (new) is a full-way bounce-back node
- * that does not take part in the drag sum. */
+ * declares INLET/OUTLET without using them; here they are the channel ends,
+ * both set by non-equilibrium extrapolation from the neighbour's
+ * post-collision state: the inlet imposes the parabolic velocity with the
+ * density of its x+1 neighbour, the outlet imposes rho0 with the velocity of
+ * its x-1 neighbour (zero-gradient velocity). MBL_SOLID_WALL (new) is a
+ * full-way bounce-back node that does not take part in the drag sum. */
 #define FLUID 0
 #define SOLID_SBB 1
 #define MBL_SOLID_WALL 2
@@ -161,7 +163,7 @@ enum {
 
 enum {
     MBL_BC_REFERENCE = 0, /* equilibrium (rho0, U0) on every outer domain edge, as the reference */
-    MBL_BC_CHANNEL = 1,   /* walls y=0, y=LY-1; parabolic equilibrium inlet x=0; zero-gradient outlet x=LX-1 */
+    MBL_BC_CHANNEL = 1,   /* walls y=0, y=LY-1; parabolic velocity inlet x=0; pressure outlet x=LX-1 */
     MBL_BC_PERIODIC = 2   /* single block, periodic in x and y */
 };
 
@@ -343,6 +345,14 @@ int mbl_last_diagnostics(mbl_handle_t h, double out[5]);
 int mbl_diagnostics_series(mbl_handle_t h, int kind, int cap, int *t, double *a, double *b);
 void mbl_diagnostics_clear(mbl_handle_t h);
 
+/* Velocity L2 distance from the analytic plane Poiseuille profile
+ * u_a(y) = 4 U0[0] (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2 (the profile left
+ * commented at lbm/init_block.h:20), over the FLUID nodes of all blocks with
+ * the 1/m block weights of error_L2:
+ * sqrt(sum_a sum ((ux-u_a)^2 + uy^2)/m_a) / sqrt(sum_a sum u_a^2/m_a).
+ * Synchronous; does not touch u_prev. */
+int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 *pps_d, double *out);
+
 /* Advances one block by `nsteps` fused stream-collide steps on its own
  * (no interface work), swapping pp->F / pp->f after every step. This is the
  * path of the single-block periodic bandwidth benchmark. */
@@ -354,8 +364,46 @@ int mbl_timer_begin(mbl_handle_t h);
 int mbl_timer_end(mbl_handle_t h, double *elapsed_ms);
 
 /* Launch tuning of the fused stream-collide kernel: nodes per thread along x
- * (1, 2 or 4) and threads per workgroup (64..1024). 0 keeps the default. */
+ * (1, 2, 4 or 8; a level whose block widths it does not divide falls back to a
+ * divisor) and threads per workgroup (64, 128, 192 or 256). 0 keeps the
+ * current value. */
 int mbl_set_tuning(mbl_handle_t h, int nodes_per_thread, int threads_per_group);
+/* Persistent grid of the fused stream-collide launches: `groups` workgroups
+ * each walk one contiguous run of tiles (0: one workgroup per tile). */
+int mbl_set_persistent(mbl_handle_t h, int groups);
+
+/* ---- batched independent cases (no reference counterpart) ----
+ * Many two-level channel cases of different size advance together: every
+ * case's level-0 blocks form one fused stream-collide launch, every case's
+ * refined block the other, and every interface line of every case shares the
+ * per-level interface launches of evolution(). The whole refinement cycle of
+ * all cases is one captured hipGraph per phase. */
+typedef struct mbl_case_spec {
+    int LX, LY;       /* channel nodes in coarse units (x = 0..LX-1, y = 0..LY-1) */
+    P4 region;        /* the ratio-2 refined block, coarse coordinates */
+    double R, CX, CY; /* cylinder radius and centre, coarse units */
+    double Re, U;     /* Reynolds number on D = 2R; inlet centre-line velocity */
+} mbl_case_spec;
+
+typedef struct mbl_batch *mbl_batch_t;
+
+/* Builds, allocates (one device allocation) and initialises every case at
+ * rest with channel boundaries. The handle's storage mode is used. */
+int mbl_batch_create(mbl_handle_t h, int ncases, const mbl_case_spec *cases, mbl_batch_t *out);
+void mbl_batch_destroy(mbl_batch_t b);
+/* Advances every case by nsteps coarse steps; the momentum-exchange force on
+ * every cylinder is reduced on the device after each step (FP64). */
+int mbl_batch_step(mbl_batch_t b, int nsteps);
+/* Node updates (all levels, all cases) per coarse step. */
+double mbl_batch_node_updates(mbl_batch_t b);
+/* Cd and Cl of every case (normalised as getDrag_block) for the last `nt`
+ * recorded steps, case-major: cd[c*nt + s]; returns the number of steps
+ * available (at most the history capacity, 4096). */
+int mbl_batch_forces(mbl_batch_t b, int nt, double *cd, double *cl);
+/* Times `trial_steps` steps for every (nodes per thread, threads per group,
+ * persistent groups) candidate and keeps the fastest; the trial steps advance
+ * the cases. Returns 0 and the choice and its time per step. */
+int mbl_batch_autotune(mbl_batch_t b, int trial_steps, int *vx, int *tpb, int *groups, double *ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,7 @@ struct mbl_context {
     int device = 0;
     int last_error = 0;
     int vx = 4, tpb = 256;
+    int groups = 0; // persistent grid size of the stream-collide launches (0: one workgroup per tile)
     Layout *layout = nullptr;
     Single single;
     Residual resid;
@@ -294,19 +295,34 @@ extern "C" int mbl_set_stream(mbl_handle_t h, void *s)
     return 0;
 }
 
-extern "C" int mbl_set_tuning(mbl_handle_t h, int vx, int tpb)
+static void drop_layouts(mbl_context *h)
 {
-    if (!h) return MBL_E_NO_HANDLE;
-    if (vx && vx != 1 && vx != 2 && vx != 4) return MBL_E_INVALID;
-    if (tpb && (tpb % 64 || tpb < 64 || tpb > 256)) return MBL_E_INVALID;
-    if (vx) h->vx = vx;
-    if (tpb) h->tpb = tpb;
+    hipStreamSynchronize(h->stream);
     release_single(h);
     if (h->layout) {
         h->layout->release();
         delete h->layout;
         h->layout = nullptr;
     }
+}
+
+extern "C" int mbl_set_persistent(mbl_handle_t h, int groups)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    if (groups < 0) return MBL_E_INVALID;
+    h->groups = groups;
+    drop_layouts(h); // captured graphs hold the old launch shape
+    return 0;
+}
+
+extern "C" int mbl_set_tuning(mbl_handle_t h, int vx, int tpb)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    if (vx && vx != 1 && vx != 2 && vx != 4 && vx != 8) return MBL_E_INVALID;
+    if (tpb && (tpb % 64 || tpb < 64 || tpb > 256)) return MBL_E_INVALID;
+    if (vx) h->vx = vx;
+    if (tpb) h->tpb = tpb;
+    drop_layouts(h);
     return 0;
 }
 
@@ -323,19 +339,33 @@ extern "C" void mbl_free(void *p)
 {
     if (p) hipFree(p);
 }
+// The copy helpers order themselves after the current handle's queued work
+// (its stream is non-blocking, so a plain hipMemcpy would not wait for it).
+static int wait_current()
+{
+    if (g_current && hipStreamSynchronize(g_current->stream) != hipSuccess) return MBL_E_HIP;
+    return 0;
+}
 extern "C" int mbl_memcpy_h2d(void *d, const void *s, size_t n)
 {
+    if (wait_current()) return MBL_E_HIP;
     return hipMemcpy(d, s, n, hipMemcpyHostToDevice) == hipSuccess ? 0 : MBL_E_HIP;
 }
 extern "C" int mbl_memcpy_d2h(void *d, const void *s, size_t n)
 {
+    if (wait_current()) return MBL_E_HIP;
     return hipMemcpy(d, s, n, hipMemcpyDeviceToHost) == hipSuccess ? 0 : MBL_E_HIP;
 }
 extern "C" int mbl_memcpy_d2d(void *d, const void *s, size_t n)
 {
+    if (wait_current()) return MBL_E_HIP;
     return hipMemcpy(d, s, n, hipMemcpyDeviceToDevice) == hipSuccess ? 0 : MBL_E_HIP;
 }
-extern "C" int mbl_memset_d(void *d, int v, size_t n) { return hipMemset(d, v, n) == hipSuccess ? 0 : MBL_E_HIP; }
+extern "C" int mbl_memset_d(void *d, int v, size_t n)
+{
+    if (wait_current()) return MBL_E_HIP;
+    return (hipMemset(d, v, n) == hipSuccess && hipDeviceSynchronize() == hipSuccess) ? 0 : MBL_E_HIP;
+}
 
 extern "C" size_t mbl_plane_bytes(mbl_handle_t h, int nx, int ny)
 {
@@ -416,10 +446,11 @@ extern "C" int mbl_write_distribution(mbl_handle_t h, const D2Q9 *pp, int which,
 // ---------------------------------------------------------------------------
 // block descriptors
 // ---------------------------------------------------------------------------
-static BlockDesc make_desc(const AREA &a, const D2Q9 &pp, int level, Lbd *upx, Lbd *upy)
+static BlockDesc make_desc(const mbl_config &cfg, const AREA &a, const D2Q9 &pp, int level, Lbd *upx, Lbd *upy)
 {
     BlockDesc d;
     memset(&d, 0, sizeof(d));
+    d.bc = make_consts(cfg);
     for (int k = 0; k < 9; k++) {
         d.P[0][k] = (void *)pp.F[k];
         d.P[1][k] = (void *)pp.f[k];
@@ -448,6 +479,9 @@ static int validate_flags(mbl_context *h, const AREA &a, const int *d_solid, boo
 {
     const size_t n = (size_t)a.nx * a.ny;
     std::vector<int> f(n);
+    // the handle's stream does not synchronise with the null stream: wait for
+    // the work that wrote the flags before the host copy
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(f.data(), d_solid, n * sizeof(int), hipMemcpyDeviceToHost));
     bool solid = false;
     for (int j = 0; j < a.ny; j++) {
@@ -576,11 +610,13 @@ struct EvoArgs {
     Inf_Ff_store *lc, *lf, *lccs;
     Ff_Store *t3;
     FM_spline3 *s3;
+    const Consts *bc = nullptr; // per-block boundary constants (batched cases); null: the handle's
 };
 
 static uint64_t layout_sig(mbl_context *h, const EvoArgs &A, int nblocks, int nline, int nline2)
 {
     uint64_t s = 0x1234;
+    s = mix64(s, (uintptr_t)A.bc);
     s = mix64(s, (uint64_t)nblocks);
     s = mix64(s, (uint64_t)nline);
     s = mix64(s, (uint64_t)nline2);
@@ -667,7 +703,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
             if (ar.nx < 2 || ar.ny < 2) return fail("block smaller than 2x2");
             for (int k = 0; k < 9; k++)
                 if (!A.pps[a].F[k] || !A.pps[a].f[k] || A.pps[a].F[k] == A.pps[a].f[k]) return fail("F/f planes");
-            BlockDesc d = make_desc(ar, A.pps[a], l, nullptr, nullptr);
+            BlockDesc d = make_desc(cfg, ar, A.pps[a], l, nullptr, nullptr);
+            if (A.bc) d.bc = A.bc[a];
             d.tiles_x = (ar.nx + 64 * vx - 1) / (64 * vx);
             d.tile_begin = tiles;
             tiles += d.tiles_x * ((ar.ny + rows - 1) / rows);
@@ -972,7 +1009,6 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
 {
     if (level == L->LEVEL || cs.rc) return; // level checked before mi[level] is read
     const int mode = storage_mode(h->cfg);
-    const Consts c = make_consts(h->cfg);
     for (int i = 0; i < L->mi[level]; i++) {
         int m[3];
         if (!(level == 0 || i == 0) && !L->bT[level].empty()) {
@@ -986,7 +1022,7 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
         if (cs.launch && L->tiles[level] > 0) {
             int rc = launch_stream_collide(h->stream, mode, L->vx[level], h->tpb, L->periodic, true, true,
                                            L->db + L->ASP[level], L->ASP[level + 1] - L->ASP[level], L->tiles[level],
-                                           h->tpb / 64, cs.pm, c);
+                                           h->tpb / 64, cs.pm, h->groups);
             if (rc) cs.rc = rc;
         }
         cs.pm ^= (1u << level);
@@ -1020,6 +1056,8 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
     }
 }
 
+static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2);
+
 extern "C" void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lccp, Lbi level, mbl_dim3 block,
                           mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d, Inf_Ff_store *line_Inf_c,
                           Inf_Ff_store *line_Inf_f, Inf_Ff_store *line_Inf_cc, Ff_Store *line_t3,
@@ -1046,6 +1084,14 @@ extern "C" void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lcc
         return;
     }
     EvoArgs A{asp, ls3cffcp, ltcffcp, lccp, areas, pps_d, line_Inf_c, line_Inf_f, line_Inf_cc, line_t3, line_S3};
+    evolution_impl(h, A, level, nblocks, nline, nline2);
+}
+
+static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2)
+{
+    const mbl_config &cfg = h->cfg;
+    D2Q9 *pps_d = A.pps;
+    Ff_Store *line_t3 = A.t3;
     const uint64_t sig = layout_sig(h, A, nblocks, nline, nline2);
     if (!h->layout || h->layout->sig != sig) {
         if (h->layout) {
@@ -1157,7 +1203,7 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
         const bool periodic = (h->cfg.bc_mode == MBL_BC_PERIODIC);
         bool any_solid = true;
         if (validate_flags(h, *area, pp->solid, periodic, &any_solid)) return MBL_E_INVALID;
-        S.hd = make_desc(*area, *pp, 0, nullptr, nullptr);
+        S.hd = make_desc(h->cfg, *area, *pp, 0, nullptr, nullptr);
         S.vx = h->vx;
         while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
         S.tpb = h->tpb;
@@ -1173,11 +1219,10 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     }
     int par = (pp->F[0] == (Lbd *)S.hd.P[0][0]) ? 0 : 1;
     const int mode = storage_mode(h->cfg);
-    const Consts c = make_consts(h->cfg);
     const bool macros = !S.periodic || S.flags;
     for (int t = 0; t < nsteps; t++) {
         int rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, 1, S.tiles,
-                                       S.rows, (uint32_t)par, c);
+                                       S.rows, (uint32_t)par, h->groups);
         if (rc) return set_err(h, rc);
         par ^= 1;
         for (int k = 0; k < 9; k++) std::swap(pp->F[k], pp->f[k]);
@@ -1220,7 +1265,7 @@ extern "C" void init_General(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, 
         return;
     }
     const mbl_config &c = h->cfg;
-    BlockDesc d = make_desc(area, pp, 0, upx, upy);
+    BlockDesc d = make_desc(h->cfg, area, pp, 0, upx, upy);
     int rc = launch_init_general(h->stream, storage_mode(c), d, area, pp.x, pp.y, pp.forcex, pp.forcey, c.bc_mode,
                                  c.init_mode, c.seed, c.sigma, (double)c.LX, (double)c.LY, c.R, c.CX, c.CY,
                                  make_consts(c));
@@ -1274,7 +1319,7 @@ extern "C" void init_Fstore(mbl_dim3 grid, mbl_dim3 block, AREA area, D2Q9 pp, I
         return;
     }
     op.dst_idx = didx;
-    BlockDesc bd = make_desc(area, pp, 0, nullptr, nullptr);
+    BlockDesc bd = make_desc(h->cfg, area, pp, 0, nullptr, nullptr);
     one_op(h, op, bd, true);
     hipFree(didx);
 }
@@ -1305,6 +1350,7 @@ static int host_links(mbl_context *h, const AREA &a, const int *d_solid, int *ho
         tmp.resize(n);
         f = tmp.data();
     }
+    HIPCHK(h, hipStreamSynchronize(h->stream)); // init_General may still be writing the flags
     HIPCHK(h, hipMemcpy(f, d_solid, n * sizeof(int), hipMemcpyDeviceToHost));
     int count = 0;
     for (int i = 0; i < a.nx; i++)
@@ -1380,8 +1426,8 @@ extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
         h->ser_l2_t.push_back(h->pend_l2[i]);
         h->ser_l2.push_back(v);
         h->last[0] = v;
-        if (!std::isfinite(v) && !std::isnan(0.0 * h->h_l2[3 * i + 2])) rc = MBL_E_NOT_FINITE;
-        if (!std::isfinite(h->h_l2[3 * i + 1])) rc = MBL_E_NOT_FINITE;
+        // a non-finite sum means a non-finite velocity somewhere; 0/0 of an all-rest field is not an error
+        if (!std::isfinite(h->h_l2[3 * i + 1]) || !std::isfinite(h->h_l2[3 * i + 2])) rc = MBL_E_NOT_FINITE;
     }
     for (size_t i = 0; i < nd; i++) {
         double fx = h->h_drag[2 * i], fy = h->h_drag[2 * i + 1];
@@ -1404,7 +1450,7 @@ extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
     return set_err(h, rc);
 }
 
-static uint64_t resid_sig(int N, const AREA *areas, const D2Q9 *pps, Lbd **upsx, Lbd **upsy)
+static uint64_t resid_sig(int N, const AREA *areas, const D2Q9 *pps, Lbd *const *upsx, Lbd *const *upsy)
 {
     uint64_t s = 0x4242;
     for (int a = 0; a < N; a++) {
@@ -1412,11 +1458,14 @@ static uint64_t resid_sig(int N, const AREA *areas, const D2Q9 *pps, Lbd **upsx,
         s = mix64(s, (uint64_t)areas[a].m);
         s = mix64(s, (uintptr_t)pps[a].ux);
         s = mix64(s, (uintptr_t)pps[a].uy);
-        s = mix64(s, (uintptr_t)upsx[a]);
-        s = mix64(s, (uintptr_t)upsy[a]);
+        s = mix64(s, upsx ? (uintptr_t)upsx[a] : 0);
+        s = mix64(s, upsy ? (uintptr_t)upsy[a] : 0);
+        s = mix64(s, (uintptr_t)pps[a].solid);
     }
     return s;
 }
+
+static int ensure_resid(mbl_context *h, int N_areas, const AREA *areas, const D2Q9 *pps_d, Lbd **upsx, Lbd **upsy);
 
 extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *grid, AREA *areas, D2Q9 *pps_d,
                          Lbd **upsx, Lbd **upsy, double **h_uvp2_b, double **d_uvp2_b, double **h_uv2_b,
@@ -1434,6 +1483,20 @@ extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *gri
         set_err(h, MBL_E_INVALID);
         return;
     }
+    if (ensure_resid(h, N_areas, areas, pps_d, upsx, upsy)) return;
+    Residual &R = h->resid;
+    const size_t slot = h->pend_l2.size();
+    int rc = launch_residual(h->stream, R.dd, N_areas, R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + 3 * slot);
+    if (rc) {
+        set_err(h, rc);
+        return;
+    }
+    h->pend_l2.push_back(t);
+    if (h->cfg.sync_diag || (int)h->pend_l2.size() >= h->cap) mbl_flush_diagnostics(h);
+}
+
+static int ensure_resid(mbl_context *h, int N_areas, const AREA *areas, const D2Q9 *pps_d, Lbd **upsx, Lbd **upsy)
+{
     Residual &R = h->resid;
     uint64_t sig = resid_sig(N_areas, areas, pps_d, upsx, upsy);
     if (R.sig != sig || !R.dd) {
@@ -1441,7 +1504,7 @@ extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *gri
         R.hd.resize(N_areas);
         int total = 0;
         for (int a = 0; a < N_areas; a++) {
-            BlockDesc d = make_desc(areas[a], pps_d[a], 0, upsx[a], upsy[a]);
+            BlockDesc d = make_desc(h->cfg, areas[a], pps_d[a], 0, upsx ? upsx[a] : nullptr, upsy ? upsy[a] : nullptr);
             d.rtiles_x = (areas[a].nx + 63) / 64;
             d.rtile_begin = total;
             total += d.rtiles_x * ((areas[a].ny + 15) / 16);
@@ -1451,19 +1514,29 @@ extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *gri
         if (hipMalloc(&R.dd, sizeof(BlockDesc) * N_areas) != hipSuccess ||
             hipMalloc(&R.partials, sizeof(double) * 2 * std::max(1, total)) != hipSuccess ||
             hipMemcpy(R.dd, R.hd.data(), sizeof(BlockDesc) * N_areas, hipMemcpyHostToDevice) != hipSuccess) {
-            set_err(h, MBL_E_HIP);
-            return;
+            return set_err(h, MBL_E_HIP);
         }
         R.sig = sig;
     }
-    const size_t slot = h->pend_l2.size();
-    int rc = launch_residual(h->stream, R.dd, N_areas, R.total, R.partials, N_areas, h->d_l2 + 3 * slot);
-    if (rc) {
-        set_err(h, rc);
-        return;
-    }
-    h->pend_l2.push_back(t);
-    if (h->cfg.sync_diag || (int)h->pend_l2.size() >= h->cap) mbl_flush_diagnostics(h);
+    return 0;
+}
+
+extern "C" int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 *pps_d, double *out)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (N_areas <= 0 || !areas || !pps_d || !out) return set_err(h, MBL_E_INVALID);
+    if (ensure_resid(h, N_areas, areas, pps_d, nullptr, nullptr)) return h->last_error;
+    Residual &R = h->resid;
+    double *d_out = nullptr;
+    HIPCHK(h, hipMalloc(&d_out, 3 * sizeof(double)));
+    int rc = launch_residual(h->stream, R.dd, N_areas, R.total, R.partials, 1, h->cfg.U0[0], (double)h->cfg.LY, d_out);
+    double v[3] = {0, 0, 0};
+    if (!rc && hipMemcpyAsync(v, d_out, sizeof(v), hipMemcpyDeviceToHost, h->stream) != hipSuccess) rc = MBL_E_HIP;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) rc = MBL_E_HIP;
+    hipFree(d_out);
+    *out = v[0];
+    return set_err(h, rc);
 }
 
 extern "C" void getDrag_block(Lbi t, Int3 *d_pdrag, AREA area, D2Q9 d_pp, double *force, Lbi N)
@@ -1533,4 +1606,370 @@ extern "C" void mbl_diagnostics_clear(mbl_handle_t h)
     h->ser_drag_t.clear();
     h->ser_cd.clear();
     h->ser_cl.clear();
+}
+
+// ---------------------------------------------------------------------------
+// batched independent cases
+// ---------------------------------------------------------------------------
+struct mbl_batch {
+    mbl_context *h = nullptr;
+    int n = 0;
+    std::vector<mbl_case_spec> cs;
+    std::vector<AREA> areas;
+    std::vector<D2Q9> pps;
+    std::vector<Consts> bc;
+    std::vector<Inf_Ff_store> lc, lf, lcc;
+    std::vector<Ff_Store> t3;
+    std::vector<FM_spline3> s3;
+    int asp[3] = {0, 0, 0}, ls3[3] = {0, 0, 0}, ltc[3] = {0, 0, 0}, lccp[3] = {0, 0, 0};
+    char *pool = nullptr;         // one device allocation for every plane, ring and line array
+    Int3 *d_links = nullptr;
+    DragArgs *d_drag[2] = {nullptr, nullptr}; // per case, for the two pointer orders of the refined block
+    void *fine_F0 = nullptr;       // refined block of case 0: its F[0] at creation (parity reference)
+    double *d_hist = nullptr;      // force history: [cap][n][2]
+    int cap = 4096, nhist = 0, head = 0;
+    std::vector<double> norm;      // rho0 U^2 m R per case
+};
+
+static int batch_build(mbl_batch *B)
+{
+    mbl_context *h = B->h;
+    const int n = B->n;
+    const int mode = storage_mode(h->cfg);
+    const size_t eb = store_elem_bytes(mode);
+    // per-case geometry through the channel builder and geo_init
+    std::vector<AREA> ca(5 * n);
+    std::vector<Inf_Ff_store> clc(4 * n), clf(4 * n), clcc(8 * n);
+    for (int c = 0; c < n; c++) {
+        const mbl_case_spec &sp = B->cs[c];
+        mbl_channel_spec s;
+        memset(&s, 0, sizeof(s));
+        s.LX = sp.LX;
+        s.LY = sp.LY;
+        s.levels = 2;
+        s.region[1] = sp.region;
+        s.Re = sp.Re;
+        s.U = sp.U;
+        s.R = sp.R;
+        mbl_geometry g;
+        P4 ad[8], adef[5];
+        P3i cf[16], cc[16];
+        if (mbl_geometry_channel(&s, &g, ad, cf, cc) || g.number_area != 5 || g.number_line != 4 ||
+            g.number_line2 != 8)
+            return MBL_E_INVALID;
+        if (mbl_geo_init(&g, adef, &ca[5 * c], &clc[4 * c], &clf[4 * c], &clcc[8 * c])) return MBL_E_INVALID;
+    }
+    // global block order: all level-0 blocks (case-major), then all refined blocks
+    auto gid = [n](int c, int q) { return q < 4 ? 4 * c + q : 4 * n + c; };
+    B->areas.resize(5 * n);
+    B->bc.resize(5 * n);
+    for (int c = 0; c < n; c++)
+        for (int q = 0; q < 5; q++) {
+            B->areas[gid(c, q)] = ca[5 * c + q];
+            Consts k;
+            k.rho0 = 1.0;
+            k.U0x = B->cs[c].U;
+            k.U0y = 0.0;
+            k.LY = (double)B->cs[c].LY;
+            k.umax = B->cs[c].U;
+            B->bc[gid(c, q)] = k;
+        }
+    B->lc.resize(4 * n);
+    B->lf.resize(4 * n);
+    B->lcc.resize(8 * n);
+    for (int c = 0; c < n; c++) {
+        for (int b = 0; b < 4; b++) {
+            Inf_Ff_store x = clc[4 * c + b], y = clf[4 * c + b];
+            x.areac_n = gid(c, x.areac_n);
+            x.areaf_n = gid(c, x.areaf_n);
+            y.areac_n = gid(c, y.areac_n);
+            y.areaf_n = gid(c, y.areaf_n);
+            B->lc[4 * c + b] = x;
+            B->lf[4 * c + b] = y;
+        }
+        for (int b = 0; b < 8; b++) {
+            Inf_Ff_store x = clcc[8 * c + b];
+            x.areac_n = gid(c, x.areac_n);
+            x.areaf_n = gid(c, x.areaf_n);
+            B->lcc[8 * c + b] = x;
+        }
+    }
+    B->asp[1] = 4 * n;
+    B->asp[2] = 5 * n;
+    B->ls3[1] = 4 * n;
+    B->ls3[2] = 4 * n;
+    B->ltc[2] = 4 * n;
+    B->lccp[1] = 8 * n;
+    B->lccp[2] = 8 * n;
+    // one pool: planes of every block, then ring slots and spline arrays of every line
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t total = 0;
+    for (const AREA &a : B->areas) {
+        size_t nn = (size_t)a.nx * a.ny;
+        total += al(nn * sizeof(int)) + 7 * al(nn * sizeof(Lbd)) + 18 * al(nn * eb);
+    }
+    for (const Inf_Ff_store &L : B->lc) total += 27 * al(L.Nf * sizeof(Lbd)) + 28 * al(L.Nc * sizeof(Lbd));
+    HIPCHK(h, hipMalloc(&B->pool, total));
+    char *p = B->pool;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += al(bytes);
+        return q;
+    };
+    B->pps.resize(5 * n);
+    for (size_t a = 0; a < B->areas.size(); a++) {
+        size_t nn = (size_t)B->areas[a].nx * B->areas[a].ny;
+        D2Q9 &pp = B->pps[a];
+        pp.solid = (int *)take(nn * sizeof(int));
+        Lbd **v = &pp.x;
+        for (int q = 0; q < 7; q++) v[q] = (Lbd *)take(nn * sizeof(Lbd));
+        for (int k = 0; k < 9; k++) {
+            pp.f[k] = (Lbd *)take(nn * eb);
+            pp.F[k] = (Lbd *)take(nn * eb);
+        }
+    }
+    B->t3.resize(4 * n);
+    B->s3.resize(4 * n);
+    for (size_t b = 0; b < B->lc.size(); b++) {
+        Ff_Store &t = B->t3[b];
+        t.N = B->lc[b].Nf;
+        t.marker[0] = 2;
+        t.marker[1] = 1;
+        t.marker[2] = 0;
+        F_k3 *ft = &t.f_t1;
+        for (int sl = 0; sl < 3; sl++)
+            for (int k = 0; k < 9; k++) ft[sl].F[k] = (Lbd *)take(t.N * sizeof(Lbd));
+        FM_spline3 &S = B->s3[b];
+        S.N = B->lc[b].Nc;
+        S.l = (Lbd *)take(S.N * sizeof(Lbd));
+        S.m = (Lbd *)take(S.N * sizeof(Lbd));
+        S.u = (Lbd *)take(S.N * sizeof(Lbd));
+        S.Varibs_M = (Lbd *)take(12 * S.N * sizeof(Lbd));
+        Lbd **fm = &S.rho;
+        for (int q = 0; q < 12; q++) fm[q] = (Lbd *)take(S.N * sizeof(Lbd));
+    }
+    // initial state (init_General with the case's cylinder and channel)
+    for (int c = 0; c < n; c++)
+        for (int q = 0; q < 5; q++) {
+            int a = gid(c, q);
+            const mbl_case_spec &sp = B->cs[c];
+            BlockDesc d = make_desc(h->cfg, B->areas[a], B->pps[a], 0, nullptr, nullptr);
+            d.bc = B->bc[a];
+            int rc = launch_init_general(h->stream, mode, d, B->areas[a], B->pps[a].x, B->pps[a].y, B->pps[a].forcex,
+                                         B->pps[a].forcey, MBL_BC_CHANNEL, MBL_INIT_REST, 0, 0.0, (double)sp.LX,
+                                         (double)sp.LY, sp.R, sp.CX, sp.CY, d.bc);
+            if (rc) return rc;
+        }
+    // rings and tridiagonal arrays: one ring-fill op per line
+    {
+        std::vector<LineOp> ops(4 * n);
+        std::vector<int> idx;
+        for (int b = 0; b < 4 * n; b++) {
+            LineOp &op = ops[b];
+            memset(&op, 0, sizeof(op));
+            op.n = B->t3[b].N;
+            op.dst_block = B->lc[b].areaf_n;
+            const F_k3 *ft = &B->t3[b].f_t1;
+            for (int sl = 0; sl < 3; sl++)
+                for (int k = 0; k < 9; k++) op.ring[sl][k] = ft[sl].F[k];
+            const AREA &af = B->areas[op.dst_block];
+            op.src_idx = (const int *)(uintptr_t)idx.size(); // offset, fixed up below
+            for (int ip = 0; ip < op.n; ip++) {
+                int i, j;
+                line_node(B->lc[b].line_F, ip, i, j);
+                idx.push_back(j * af.nx + i);
+            }
+            int rc = launch_tri_A(h->stream, B->s3[b].l, B->s3[b].m, B->s3[b].u, B->s3[b].N);
+            if (rc) return rc;
+        }
+        int *didx = nullptr;
+        LineOp *dops = nullptr;
+        BlockDesc *dbl = nullptr;
+        std::vector<BlockDesc> blocks(5 * n);
+        for (int a = 0; a < 5 * n; a++) blocks[a] = make_desc(h->cfg, B->areas[a], B->pps[a], 0, nullptr, nullptr);
+        HIPCHK(h, hipMalloc(&didx, sizeof(int) * std::max<size_t>(1, idx.size())));
+        HIPCHK(h, hipMalloc(&dops, sizeof(LineOp) * ops.size()));
+        HIPCHK(h, hipMalloc(&dbl, sizeof(BlockDesc) * blocks.size()));
+        int maxn = 0;
+        for (auto &op : ops) {
+            op.dst_idx = didx + (uintptr_t)op.src_idx;
+            op.src_idx = nullptr;
+            maxn = std::max(maxn, op.n);
+        }
+        HIPCHK(h, hipMemcpy(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(dops, ops.data(), sizeof(LineOp) * ops.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(dbl, blocks.data(), sizeof(BlockDesc) * blocks.size(), hipMemcpyHostToDevice));
+        int rc = launch_ring_fill(h->stream, mode, dops, (int)ops.size(), maxn, dbl, 0u);
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        hipFree(didx);
+        hipFree(dops);
+        hipFree(dbl);
+        if (rc) return rc;
+    }
+    // drag links of every refined block
+    std::vector<Int3> links;
+    std::vector<int> off(n + 1, 0);
+    for (int c = 0; c < n; c++) {
+        int a = gid(c, 4);
+        std::vector<Int3> L;
+        if (host_links(h, B->areas[a], B->pps[a].solid, nullptr, &L) < 0) return MBL_E_HIP;
+        links.insert(links.end(), L.begin(), L.end());
+        off[c + 1] = (int)links.size();
+    }
+    HIPCHK(h, hipMalloc(&B->d_links, sizeof(Int3) * std::max<size_t>(1, links.size())));
+    if (!links.empty())
+        HIPCHK(h, hipMemcpy(B->d_links, links.data(), sizeof(Int3) * links.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMalloc(&B->d_hist, sizeof(double) * 2 * (size_t)n * B->cap));
+    B->norm.resize(n);
+    for (int par = 0; par < 2; par++) {
+        std::vector<DragArgs> args(n);
+        for (int c = 0; c < n; c++) {
+            int a = gid(c, 4);
+            DragArgs &d = args[c];
+            memset(&d, 0, sizeof(d));
+            // after an odd number of refined-level substeps F/f are exchanged; the
+            // planes one substep older are the set not holding the state
+            for (int k = 0; k < 9; k++) d.Fold[k] = par ? (void *)B->pps[a].F[k] : (void *)B->pps[a].f[k];
+            d.links = B->d_links + off[c];
+            d.n = off[c + 1] - off[c];
+            d.nx = B->areas[a].nx;
+            B->norm[c] = 1.0 * B->cs[c].U * B->cs[c].U * B->areas[a].m * B->cs[c].R;
+        }
+        HIPCHK(h, hipMalloc(&B->d_drag[par], sizeof(DragArgs) * n));
+        HIPCHK(h, hipMemcpy(B->d_drag[par], args.data(), sizeof(DragArgs) * n, hipMemcpyHostToDevice));
+    }
+    B->fine_F0 = (void *)B->pps[gid(0, 4)].F[0];
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mbl_batch_create(mbl_handle_t h, int ncases, const mbl_case_spec *cases, mbl_batch_t *out)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!out || !cases || ncases <= 0) return set_err(h, MBL_E_INVALID);
+    if (h->cfg.LEVEL != 2 || h->cfg.mi[0] != 1 || h->cfg.mi[1] != 2 || h->cfg.bc_mode != MBL_BC_CHANNEL)
+        return set_err(h, MBL_E_INVALID);
+    *out = nullptr;
+    mbl_batch *B = new mbl_batch();
+    B->h = h;
+    B->n = ncases;
+    B->cs.assign(cases, cases + ncases);
+    int rc = batch_build(B);
+    if (rc) {
+        mbl_batch_destroy(B);
+        return set_err(h, rc);
+    }
+    *out = B;
+    return 0;
+}
+
+extern "C" void mbl_batch_destroy(mbl_batch_t B)
+{
+    if (!B) return;
+    if (B->h) {
+        hipStreamSynchronize(B->h->stream);
+        drop_layouts(B->h);
+    }
+    if (B->pool) hipFree(B->pool);
+    if (B->d_links) hipFree(B->d_links);
+    if (B->d_hist) hipFree(B->d_hist);
+    for (int p = 0; p < 2; p++)
+        if (B->d_drag[p]) hipFree(B->d_drag[p]);
+    delete B;
+}
+
+extern "C" double mbl_batch_node_updates(mbl_batch_t B)
+{
+    if (!B) return 0.0;
+    double u = 0.0;
+    for (const AREA &a : B->areas) u += (double)a.nx * a.ny * a.m;
+    return u;
+}
+
+extern "C" int mbl_batch_step(mbl_batch_t B, int nsteps)
+{
+    if (!B) return MBL_E_INVALID;
+    mbl_context *h = B->h;
+    if (h->last_error) return h->last_error;
+    EvoArgs A{B->asp, B->ls3, B->ltc, B->lccp, B->areas.data(), B->pps.data(), B->lc.data(), B->lf.data(),
+              B->lcc.data(), B->t3.data(), B->s3.data()};
+    A.bc = B->bc.data();
+    const int mode = storage_mode(h->cfg);
+    for (int t = 0; t < nsteps; t++) {
+        evolution_impl(h, A, 0, 5 * B->n, 4 * B->n, 8 * B->n);
+        if (h->last_error) return h->last_error;
+        const int par = (B->pps[4 * B->n].F[0] == (Lbd *)B->fine_F0) ? 0 : 1;
+        int rc = launch_drag_batch(h->stream, mode, B->d_drag[par], B->n, B->d_hist + 2 * (size_t)B->head * B->n);
+        if (rc) return set_err(h, rc);
+        B->head = (B->head + 1) % B->cap;
+        B->nhist = std::min(B->nhist + 1, B->cap);
+    }
+    return 0;
+}
+
+extern "C" int mbl_batch_forces(mbl_batch_t B, int nt, double *cd, double *cl)
+{
+    if (!B) return MBL_E_INVALID;
+    mbl_context *h = B->h;
+    const int n = B->n;
+    std::vector<double> hist((size_t)2 * n * B->cap);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(hist.data(), B->d_hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost));
+    const int m = std::min(nt, B->nhist);
+    for (int s = 0; s < m; s++) {
+        int slot = (B->head - m + s + 2 * B->cap) % B->cap;
+        for (int c = 0; c < n; c++) {
+            double fx = hist[2 * ((size_t)slot * n + c)], fy = hist[2 * ((size_t)slot * n + c) + 1];
+            if (cd) cd[(size_t)c * nt + s] = fx / B->norm[c];
+            if (cl) cl[(size_t)c * nt + s] = fy / B->norm[c];
+        }
+    }
+    return m;
+}
+
+extern "C" int mbl_batch_autotune(mbl_batch_t B, int trial_steps, int *vx_out, int *tpb_out, int *groups_out,
+                                  double *ms_out)
+{
+    if (!B || trial_steps <= 0) return MBL_E_INVALID;
+    mbl_context *h = B->h;
+    int cu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
+        cu = prop.multiProcessorCount;
+    const int vxs[3] = {1, 2, 4}, tpbs[3] = {64, 128, 256};
+    const int gs[4] = {0, 4 * cu, 8 * cu, 16 * cu};
+    double best = 1e30;
+    int bv = h->vx, bt = h->tpb, bg = h->groups;
+    for (int v : vxs)
+        for (int t : tpbs)
+            for (int g : gs) {
+                h->vx = v;
+                h->tpb = t;
+                h->groups = g;
+                drop_layouts(h);
+                if (mbl_batch_step(B, 1)) return h->last_error; // builds tables and captures the graphs
+                HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+                if (mbl_batch_step(B, trial_steps)) return h->last_error;
+                HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+                HIPCHK(h, hipEventSynchronize(h->ev1));
+                float ms = 0.f;
+                HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                double per = ms / trial_steps;
+                if (per < best) {
+                    best = per;
+                    bv = v;
+                    bt = t;
+                    bg = g;
+                }
+            }
+    h->vx = bv;
+    h->tpb = bt;
+    h->groups = bg;
+    drop_layouts(h);
+    if (vx_out) *vx_out = bv;
+    if (tpb_out) *tpb_out = bt;
+    if (groups_out) *groups_out = bg;
+    if (ms_out) *ms_out = best;
+    return 0;
 }

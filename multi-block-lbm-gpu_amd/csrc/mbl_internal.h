@@ -40,6 +40,7 @@ struct Consts {
 // block's state lives in P[parity] where parity is the level's bit of the
 // mask passed to every launch.
 struct BlockDesc {
+    Consts bc;            // boundary constants of the block's case
     void *P[2][MBL_Q];
     const int *flag;
     Lbd *rho, *ux, *uy;
@@ -91,7 +92,7 @@ struct DragArgs {
 // ---- launchers (mbl_kernels.hip), all asynchronous on `stream` ----
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows,
-                          uint32_t parity_mask, const Consts &c);
+                          uint32_t parity_mask, int groups);
 int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &area, Lbd *x, Lbd *y,
                         Lbd *forcex, Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed,
                         double sigma, double LX, double LY, double R, double CX, double CY, const Consts &c);
@@ -102,9 +103,10 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
 int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
                      const BlockDesc *d_blocks, uint32_t parity_mask);
 int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N);
-int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total_rtiles,
-                    double *partials, int nblocks_total, double *d_out);
+int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total_rtiles, double *partials,
+                    int mode, double umax, double LY, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
+int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
 int launch_decode(void *stream, int mode, const void *plane, int k, size_t n, double *out);
 int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const double *in);
 

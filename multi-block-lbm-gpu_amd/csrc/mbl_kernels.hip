@@ -35,8 +35,13 @@ __device__ constexpr double W64[9] = {4. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 9, 1.
 // ---------------------------------------------------------------------------
 template <int MODE> struct Codec;
 
+// Pointers into HBM are cast to the global address space so the backend emits
+// global_load/global_store (SGPR base + VGPR offset) rather than flat accesses.
+#define GLOBAL_AS __attribute__((address_space(1)))
+
 template <> struct Codec<ST_NATIVE> {
     typedef Lbd T;
+    typedef GLOBAL_AS Lbd GT;
     static constexpr bool SHIFT = false;
     __device__ static inline real_t ld(T v) { return (real_t)v; }
     __device__ static inline T st(real_t v) { return (T)v; }
@@ -45,6 +50,7 @@ template <> struct Codec<ST_NATIVE> {
 #ifndef MBL_FP64
 template <> struct Codec<ST_F32S> {
     typedef float T;
+    typedef GLOBAL_AS float GT;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return v; }
     __device__ static inline T st(float v) { return v; }
@@ -52,6 +58,7 @@ template <> struct Codec<ST_F32S> {
 
 template <> struct Codec<ST_F16S> {
     typedef _Float16 T;
+    typedef GLOBAL_AS _Float16 GT;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return (float)v; }
     __device__ static inline T st(float v) { return (_Float16)v; } // round to nearest even
@@ -59,6 +66,7 @@ template <> struct Codec<ST_F16S> {
 
 template <> struct Codec<ST_BF16S> {
     typedef uint16_t T;
+    typedef GLOBAL_AS uint16_t GT;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return __uint_as_float(((uint32_t)v) << 16); }
     __device__ static inline T st(float v)
@@ -142,19 +150,20 @@ template <class T, int VX> struct Vec {
     typedef T type __attribute__((ext_vector_type(VX)));
 };
 
-template <class T, int VX> __device__ inline void vload(const T *p, T (&o)[VX])
+template <class T, int VX, class P> __device__ inline void vload(P p, T (&o)[VX])
 {
     if constexpr (VX == 1) {
         o[0] = p[0];
     } else {
         typedef typename Vec<T, VX>::type V;
-        V v = *reinterpret_cast<const V *>(p);
+        typedef const GLOBAL_AS V GV;
+        V v = *reinterpret_cast<GV *>(p);
 #pragma unroll
         for (int i = 0; i < VX; i++) o[i] = v[i];
     }
 }
 
-template <class T, int VX> __device__ inline void vstore(T *p, const T (&o)[VX])
+template <class T, int VX, class P> __device__ inline void vstore(P p, const T (&o)[VX])
 {
     if constexpr (VX == 1) {
         __builtin_nontemporal_store(o[0], p);
@@ -163,7 +172,8 @@ template <class T, int VX> __device__ inline void vstore(T *p, const T (&o)[VX])
         V v;
 #pragma unroll
         for (int i = 0; i < VX; i++) v[i] = o[i];
-        __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+        typedef GLOBAL_AS V GV;
+        __builtin_nontemporal_store(v, reinterpret_cast<GV *>(p));
     }
 }
 
@@ -173,7 +183,7 @@ template <class T, int VX> __device__ inline void vstore(T *p, const T (&o)[VX])
 // rules besides the copy done here).
 // ---------------------------------------------------------------------------
 template <int MODE, bool PERIODIC>
-__device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>::T *const (&S)[9], int i, int j,
+__device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9], int i, int j,
                                  const Consts &c, real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
 {
     typedef Codec<MODE> C;
@@ -201,8 +211,8 @@ __device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>:
 }
 
 template <int MODE, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ inline void general_node(const BlockDesc &d, const typename Codec<MODE>::T *const (&S)[9],
-                                    typename Codec<MODE>::T *const (&D)[9], int i, int j, const Consts &c,
+__device__ inline void general_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
+                                    typename Codec<MODE>::GT *const (&D)[9], int i, int j, const Consts &c,
                                     real_t omega)
 {
     typedef Codec<MODE> C;
@@ -295,12 +305,10 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
 // VX-wide loads of each source row plus one scalar for the x-shifted planes.
 // ---------------------------------------------------------------------------
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs, int ndesc, int tile_rows,
-                                                        uint32_t parity_mask, Consts c)
+__device__ inline void sc_tile(const BlockDesc *__restrict__ descs, int ndesc, int tile_rows, uint32_t parity_mask, int t)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
-    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     int lo = 0, hi = ndesc - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -310,6 +318,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
             hi = mid - 1;
     }
     const BlockDesc &d = descs[lo];
+    const Consts &c = d.bc;
     const int lt = t - d.tile_begin;
     const int tyi = lt / d.tiles_x;
     const int txi = lt - tyi * d.tiles_x;
@@ -319,12 +328,13 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
     const int nx = d.nx, ny = d.ny;
     if (j >= ny || x0 >= nx) return;
     const int par = (parity_mask >> d.level) & 1;
-    const T *S[9];
-    T *D[9];
+    typedef typename C::GT GT;
+    const GT *S[9];
+    GT *D[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        S[k] = (const T *)d.P[par][k];
-        D[k] = (T *)d.P[par ^ 1][k];
+        S[k] = (const GT *)d.P[par][k];
+        D[k] = (GT *)d.P[par ^ 1][k];
     }
     const real_t omega = (real_t)d.omega;
     const size_t r0 = (size_t)j * nx;
@@ -336,7 +346,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
         fast = (x0 >= 1) && (x0 + VX <= nx - 1) && (j >= 1) && (j <= ny - 2);
     if (FLAGS && fast) {
         int fl[VX];
-        vload<int, VX>(d.flag + r0 + x0, fl);
+        vload<int, VX>((const GLOBAL_AS int *)d.flag + r0 + x0, fl);
 #pragma unroll
         for (int v = 0; v < VX; v++) fast = fast && (fl[v] == FLUID);
     }
@@ -359,7 +369,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
     for (int k = 0; k < 9; k++) {
         // pull: q_k(x) = F_k(x - e_k); EY = +1 reads row j-1, EY = -1 row j+1
         const size_t rb = (EY[k] == 1) ? rm : ((EY[k] == -1) ? rp : r0);
-        const T *base = S[k] + rb;
+        const GT *base = S[k] + rb;
         if (EX[k] == 0) {
             T tv[VX];
             vload<T, VX>(base + x0, tv);
@@ -400,10 +410,29 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
 #pragma unroll
     for (int k = 0; k < 9; k++) vstore<T, VX>(D[k] + r0 + x0, out[k]);
     if (MACROS) {
-        vstore<Lbd, VX>(d.rho + r0 + x0, mr);
-        vstore<Lbd, VX>(d.ux + r0 + x0, mx);
-        vstore<Lbd, VX>(d.uy + r0 + x0, my);
+        vstore<Lbd, VX>((GLOBAL_AS Lbd *)d.rho + r0 + x0, mr);
+        vstore<Lbd, VX>((GLOBAL_AS Lbd *)d.ux + r0 + x0, mx);
+        vstore<Lbd, VX>((GLOBAL_AS Lbd *)d.uy + r0 + x0, my);
     }
+}
+
+// One tile per workgroup when the grid covers every tile; otherwise a
+// persistent grid in which workgroup b (after the XCD-aware reordering) walks
+// one contiguous chunk of tiles, so x-neighbouring tiles stay on one XCD.
+template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
+__global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs, int ndesc, int tile_rows,
+                                                        uint32_t parity_mask, int total_tiles)
+{
+    const int G = (int)gridDim.x;
+    const int b = xcd_tile((int)blockIdx.x, G);
+    if (G >= total_tiles) {
+        if (b < total_tiles) sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, ndesc, tile_rows, parity_mask, b);
+        return;
+    }
+    const int per = (total_tiles + G - 1) / G;
+    const int t0 = b * per;
+    const int t1 = min(total_tiles, t0 + per);
+    for (int t = t0; t < t1; t++) sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, ndesc, tile_rows, parity_mask, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -804,9 +833,39 @@ template <int MODE> __global__ __launch_bounds__(1024) void k_drag(DragArgs a)
     }
 }
 
+// Momentum exchange of many cases at once: workgroup c reduces case c's
+// links into out[2c], out[2c+1] (same per-link terms as k_drag).
+template <int MODE> __global__ __launch_bounds__(256) void k_drag_batch(const DragArgs *__restrict__ args, double *out)
+{
+    typedef typename Codec<MODE>::T T;
+    __shared__ double sh[16];
+    const DragArgs &a = args[blockIdx.x];
+    double fx = 0.0, fy = 0.0;
+    for (int e = threadIdx.x; e < a.n; e += blockDim.x) {
+        const Int3 L = a.links[e];
+        const int k = L.k, kd = OPP[k];
+        const size_t xs = (size_t)L.j * a.nx + L.i;
+        const size_t xf = (size_t)(L.j - EY[k]) * a.nx + (L.i - EX[k]);
+        const double fk = dec64<MODE>(((const T *)a.Fold[k])[xf], k);
+        const double fkd = dec64<MODE>(((const T *)a.Fold[kd])[xs], kd);
+        fx += (fk * EX[k] - fkd * EX[kd]);
+        fy += (fk * EY[k] - fkd * EY[kd]);
+    }
+    const double sx = group_sum(fx, sh);
+    const double sy = group_sum(fy, sh);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sx;
+        out[2 * blockIdx.x + 1] = sy;
+    }
+}
+
 // error_L2_area (error_L2.h:3-55): tiles of 64 x 16 nodes, FP64 partial sums
 // of |u - u_prev|^2 and |u|^2 per tile, u_prev <- u.
-__global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restrict__ descs, int ndesc, double *partials)
+// mode 0: error_L2_area (error_L2.h:3-55); mode 1: L2 distance of (ux, uy) on
+// FLUID nodes from the analytic plane Poiseuille profile
+// u_a(y) = 4 umax (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2, without touching u_prev.
+__global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restrict__ descs, int ndesc, double *partials,
+                                                        int mode, double umax, double LY)
 {
     __shared__ double sh[16];
     const int t = blockIdx.x;
@@ -829,11 +888,18 @@ __global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restr
             if (j >= d.ny) break;
             const size_t ixy = (size_t)j * d.nx + i;
             const double ux = (double)d.ux[ixy], uy = (double)d.uy[ixy];
-            const double upx = (double)d.upx[ixy], upy = (double)d.upy[ixy];
-            s1 += (ux - upx) * (ux - upx) + (uy - upy) * (uy - upy);
-            s2 += ux * ux + uy * uy;
-            d.upx[ixy] = d.ux[ixy];
-            d.upy[ixy] = d.uy[ixy];
+            if (mode == 0) {
+                const double upx = (double)d.upx[ixy], upy = (double)d.upy[ixy];
+                s1 += (ux - upx) * (ux - upx) + (uy - upy) * (uy - upy);
+                s2 += ux * ux + uy * uy;
+                d.upx[ixy] = d.ux[ixy];
+                d.upy[ixy] = d.uy[ixy];
+            } else if (d.flag[ixy] == FLUID) {
+                const double y = j * d.dx + d.base_y;
+                const double ua = 4.0 * umax * (LY - 1.5 - y) * (y - 0.5) / (LY - 2.0) / (LY - 2.0);
+                s1 += (ux - ua) * (ux - ua) + uy * uy;
+                s2 += ua * ua;
+            }
         }
     }
     const double a1 = group_sum(s1, sh);
@@ -896,26 +962,26 @@ template <int MODE> __global__ void k_encode(void *plane, int k, size_t n, const
 
 template <int MODE, int VX>
 static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d, int nd,
-                       int tiles, int rows, uint32_t pm, const Consts &c)
+                       int tiles, int rows, uint32_t pm, int groups)
 {
-    dim3 g(tiles), b(tpb);
+    dim3 g(groups > 0 && groups < tiles ? groups : tiles), b(tpb);
     if (periodic) {
         if (flags) {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, nd, rows, pm, c);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, nd, rows, pm, tiles);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, nd, rows, pm, c);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, nd, rows, pm, tiles);
         } else {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, nd, rows, pm, c);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, nd, rows, pm, tiles);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, nd, rows, pm, c);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, nd, rows, pm, tiles);
         }
     } else {
         if (macros)
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, nd, rows, pm, c);
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, nd, rows, pm, tiles);
         else
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, nd, rows, pm, c);
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, nd, rows, pm, tiles);
     }
     MBL_LAUNCH_CHECK();
     return 0;
@@ -923,12 +989,13 @@ static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool m
 
 template <int MODE>
 static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d, int nd,
-                   int tiles, int rows, uint32_t pm, const Consts &c)
+                   int tiles, int rows, uint32_t pm, int groups)
 {
     switch (vx) {
-    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, c);
-    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, c);
-    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, c);
+    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
+    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
+    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
+    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
     default: return MBL_E_INVALID;
     }
 }
@@ -968,13 +1035,13 @@ static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bo
 
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows, uint32_t parity_mask,
-                          const Consts &c)
+                          int groups)
 {
     if (total_tiles <= 0) return 0;
     if (tpb != 64 * tile_rows || tpb > 256) return MBL_E_INVALID;
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, return sc_mode<M_>(s, vx, tpb, periodic, flags, macros, d_descs, ndesc, total_tiles,
-                                             tile_rows, parity_mask, c));
+                                             tile_rows, parity_mask, groups));
     return 0;
 }
 
@@ -1073,11 +1140,12 @@ int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N)
 }
 
 int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total_rtiles, double *partials,
-                    int nblocks_total, double *d_out)
+                    int mode, double umax, double LY, double *d_out)
 {
-    (void)nblocks_total;
     hipStream_t s = (hipStream_t)stream;
-    if (total_rtiles > 0) hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, ndesc, partials);
+    if (total_rtiles > 0)
+        hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, ndesc, partials, mode, umax,
+                           LY);
     MBL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, s, d_descs, ndesc, total_rtiles, partials, d_out);
     MBL_LAUNCH_CHECK();
@@ -1088,6 +1156,15 @@ int launch_drag(void *stream, int mode, const DragArgs &a)
 {
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_drag<M_>), dim3(1), dim3(1024), 0, s, a));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out)
+{
+    if (ncases <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_drag_batch<M_>), dim3(ncases), dim3(256), 0, s, d_args, out));
     MBL_LAUNCH_CHECK();
     return 0;
 }

@@ -14,8 +14,8 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import (AREA, D2Q9, F_k3, FM_spline3, Ff_Store, Inf_Ff_store, Int3, P3i, P4, mbl_channel_spec,
-                  mbl_config, mbl_dim3, mbl_geometry)
+from .abi import (AREA, D2Q9, F_k3, FM_spline3, Ff_Store, Inf_Ff_store, Int3, P3i, P4, mbl_case_spec,
+                  mbl_channel_spec, mbl_config, mbl_dim3, mbl_geometry)
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
@@ -87,6 +87,14 @@ def load_library(fp64=False):
         "mbl_timer_begin": (ip, [vp]),
         "mbl_timer_end": (ip, [vp, dp]),
         "mbl_set_tuning": (ip, [vp, ip, ip]),
+        "mbl_poiseuille_l2": (ip, [vp, ip, C.POINTER(AREA), C.POINTER(D2Q9), dp]),
+        "mbl_set_persistent": (ip, [vp, ip]),
+        "mbl_batch_create": (ip, [vp, ip, C.POINTER(mbl_case_spec), C.POINTER(vp)]),
+        "mbl_batch_destroy": (None, [vp]),
+        "mbl_batch_step": (ip, [vp, ip]),
+        "mbl_batch_node_updates": (C.c_double, [vp]),
+        "mbl_batch_forces": (ip, [vp, ip, dp, dp]),
+        "mbl_batch_autotune": (ip, [vp, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -291,6 +299,13 @@ class MultiBlockSim:
     def flush(self):
         check(self.lib, self.lib.mbl_flush_diagnostics(self.h), "flush")
 
+    def poiseuille_l2(self):
+        """Velocity L2 distance from the analytic channel profile (FLUID nodes, 1/m weights)."""
+        out = C.c_double()
+        check(self.lib, self.lib.mbl_poiseuille_l2(self.h, self.na, self.areas, self.pps_d, C.byref(out)),
+              "poiseuille_l2")
+        return out.value
+
     def series(self):
         """{'t','residual'} and {'t','Cd','Cl'} recorded so far (after a flush)."""
         self.flush()
@@ -408,6 +423,88 @@ class PeriodicBlock:
     def close(self):
         if getattr(self, "h", None):
             self.lib.mbl_block_free(C.byref(self.pp), None, None)
+            self.lib.mbl_handle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# batched independent cases
+# ---------------------------------------------------------------------------
+def case_spec(Dc, blockage, Re, U=0.1):
+    """Two-level cylinder case in coarse units: diameter Dc (an integer number of
+    coarse cells, 2*Dc finest), channel height Dc/blockage, length 20 Dc, refined
+    block 4 Dc x 2 Dc with the cylinder one diameter behind its upstream edge."""
+    H = int(round(Dc / blockage))
+    LX, LY = 20 * Dc + 1, H + 1
+    yc = (LY - 1) // 2
+    region = P4(3 * Dc, 7 * Dc, yc - Dc, yc + Dc)
+    return mbl_case_spec(LX, LY, region, 0.5 * Dc, 4.0 * Dc, float(yc), float(Re), float(U))
+
+
+def c4_specs(n=256, seed=2024):
+    """Seeded sweep: Re ~ U[40, 200], D in {10, 16, 24, 32, 40} finest cells,
+    blockage D/H ~ U[0.1, 0.3]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        Dfine = int(rng.choice([10, 16, 24, 32, 40]))
+        out.append(case_spec(Dfine // 2, float(rng.uniform(0.1, 0.3)), float(rng.uniform(40.0, 200.0))))
+    return out
+
+
+class Batch:
+    """Many independent two-level cases advanced by shared launches (mbl_batch_*)."""
+
+    def __init__(self, lib, specs, storage=abi.STORE_NATIVE, use_graph=1):
+        self.lib = lib
+        cfg = default_config(lib, LEVEL=2, number_line2=8, bc_mode=abi.BC_CHANNEL, init_mode=abi.INIT_REST,
+                             storage=storage, use_graph=use_graph, write_files=0)
+        cfg.mi[0], cfg.mi[1] = 1, 2
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib, lib.mbl_handle_create(C.byref(h), C.byref(cfg)), "handle")
+        self.h = h
+        self.specs = (mbl_case_spec * len(specs))(*specs)
+        self.n = len(specs)
+        b = C.c_void_p()
+        check(lib, lib.mbl_batch_create(h, self.n, self.specs, C.byref(b)), "batch create")
+        self.b = b
+
+    def step(self, n):
+        check(self.lib, self.lib.mbl_batch_step(self.b, n), "batch step")
+
+    def sync(self):
+        check(self.lib, self.lib.mbl_synchronize(self.h), "synchronize")
+
+    def node_updates(self):
+        return self.lib.mbl_batch_node_updates(self.b)
+
+    def forces(self, nt):
+        cd = np.zeros(self.n * nt)
+        cl = np.zeros(self.n * nt)
+        m = self.lib.mbl_batch_forces(self.b, nt, cd.ctypes.data_as(C.POINTER(C.c_double)),
+                                      cl.ctypes.data_as(C.POINTER(C.c_double)))
+        if m < 0:
+            check(self.lib, m, "batch forces")
+        return cd.reshape(self.n, nt)[:, :m], cl.reshape(self.n, nt)[:, :m]
+
+    def autotune(self, trial_steps=10):
+        v, t, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(self.lib, self.lib.mbl_batch_autotune(self.b, trial_steps, C.byref(v), C.byref(t), C.byref(g),
+                                                    C.byref(ms)), "autotune")
+        return {"vx": v.value, "tpb": t.value, "groups": g.value, "ms_per_step": ms.value}
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.mbl_batch_destroy(self.b)
+            self.b = None
+        if getattr(self, "h", None):
             self.lib.mbl_handle_destroy(self.h)
             self.h = None
 

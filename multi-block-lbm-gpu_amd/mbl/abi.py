@@ -99,6 +99,11 @@ class mbl_geometry(C.Structure):
                 ("LX", C.c_int), ("LY", C.c_int), ("Re", C.c_double), ("U0_2", C.c_double), ("R", C.c_double)]
 
 
+class mbl_case_spec(C.Structure):
+    _fields_ = [("LX", C.c_int), ("LY", C.c_int), ("region", P4), ("R", C.c_double), ("CX", C.c_double),
+                ("CY", C.c_double), ("Re", C.c_double), ("U", C.c_double)]
+
+
 class mbl_channel_spec(C.Structure):
     _fields_ = [("LX", C.c_int), ("LY", C.c_int), ("levels", C.c_int), ("region", P4 * MAX_LEVEL),
                 ("Re", C.c_double), ("U", C.c_double), ("R", C.c_double)]
@@ -114,5 +119,7 @@ EXPORTS = [
     "mbl_geometry_preset", "mbl_geometry_channel", "init_General", "init_Fstore", "init_tri_A",
     "evolution", "sum_N_drag", "fdrag_store", "getDrag_block", "error_L2", "mbl_last_diagnostics",
     "mbl_diagnostics_series", "mbl_diagnostics_clear", "mbl_step_block", "mbl_timer_begin",
-    "mbl_timer_end", "mbl_set_tuning",
+    "mbl_timer_end", "mbl_set_tuning", "mbl_poiseuille_l2", "mbl_set_persistent",
+    "mbl_batch_create", "mbl_batch_destroy", "mbl_batch_step", "mbl_batch_node_updates", "mbl_batch_forces",
+    "mbl_batch_autotune",
 ]

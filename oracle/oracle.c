@@ -946,6 +946,29 @@ int orc_sim_diag(orc_sim *s, double out[3])
     return 0;
 }
 
+/* EXT: velocity L2 distance from the analytic plane Poiseuille profile over
+ * FLUID nodes, 1/m block weights (the normalisation of error_L2.h:297-302) */
+double orc_sim_poiseuille_l2(const orc_sim *s)
+{
+    double num = 0.0, den = 0.0;
+    for (int a = 0; a < s->number_area; a++) {
+        const orc_block *b = &s->blk[a];
+        double s1 = 0.0, s2 = 0.0;
+        for (int j = 0; j < b->area.ny; j++)
+            for (int i = 0; i < b->area.nx; i++) {
+                int ixy = i + j * b->area.nx;
+                if (b->pp.solid[ixy] != FLUID) continue;
+                double ua = inlet_u(&s->cfg, b->pp.y[ixy]);
+                double ux = b->pp.ux[ixy], uy = b->pp.uy[ixy];
+                s1 += (ux - ua) * (ux - ua) + uy * uy;
+                s2 += ua * ua;
+            }
+        num += s1 / b->area.m;
+        den += s2 / b->area.m;
+    }
+    return sqrt(num) / sqrt(den);
+}
+
 int orc_sim_evolve(orc_sim *s, int n)
 {
     for (int t = 0; t < n; t++) {

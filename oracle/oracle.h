@@ -42,6 +42,8 @@ int orc_sim_run(orc_sim *s, int n);
 int orc_sim_evolve(orc_sim *s, int n);
 /* error_L2 + getDrag_block on the current state: residual, Fx, Fy */
 int orc_sim_diag(orc_sim *s, double out[3]);
+/* velocity L2 distance from the analytic channel profile (FLUID nodes) */
+double orc_sim_poiseuille_l2(const orc_sim *s);
 int orc_sim_steps_done(const orc_sim *s);
 int orc_sim_num_areas(const orc_sim *s);
 int orc_sim_num_lines(const orc_sim *s, int which); /* 0: CF lines, 1: CC lines */

@@ -44,6 +44,7 @@ def load():
         "orc_sim_run": (ip, [vp, ip]),
         "orc_sim_evolve": (ip, [vp, ip]),
         "orc_sim_diag": (ip, [vp, dp]),
+        "orc_sim_poiseuille_l2": (C.c_double, [vp]),
         "orc_temporal_value": (C.c_double, [C.c_double] * 4),
         "orc_sim_steps_done": (ip, [vp]),
         "orc_sim_num_areas": (ip, [vp]),
@@ -111,6 +112,9 @@ class OracleSim:
         out = np.zeros(3)
         self.lib.orc_sim_diag(self.p, _dptr(out))
         return out
+
+    def poiseuille_l2(self):
+        return self.lib.orc_sim_poiseuille_l2(self.p)
 
     def shape(self, a):
         return self.areas[a].ny, self.areas[a].nx

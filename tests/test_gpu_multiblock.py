@@ -25,7 +25,7 @@ def tiny2(lib):
 def tiny3(lib):
     # 3 levels: 80x32-cell channel
     return mbl.Geometry.channel(lib, 81, 33, [(12, 44, 6, 26), (16, 36, 10, 22)], R=2.0, CX=22.0, CY=16.0,
-                                Re=40.0)
+                                Re=20.0)
 
 
 def run_pair(lib, geom, steps, **kw):
@@ -54,14 +54,17 @@ def assert_fields_close(sim, o, rtol, atol_f):
 
 
 def assert_series_close(sim, o, rtol):
+    """Per-step residual, Cd and Cl against the oracle. Errors are relative to
+    the largest |Cd| (Cl of a symmetric flow is ~1e-14, so it is compared on the
+    Cd scale) and to the largest residual."""
     l2, dr = sim.series()
-    for name, g, r in (("residual", l2["residual"], o.series(0)), ("Cd", dr["Cd"], o.series(3)),
-                       ("Cl", dr["Cl"], o.series(4))):
+    cd_scale = np.max(np.abs(o.series(3))) if len(o.series(3)) else 1.0
+    for name, g, r, scale in (("residual", l2["residual"], o.series(0), np.max(np.abs(o.series(0)))),
+                              ("Cd", dr["Cd"], o.series(3), cd_scale), ("Cl", dr["Cl"], o.series(4), cd_scale)):
         assert len(g) == len(r), name
-        scale = max(1e-12, np.max(np.abs(r)))
-        err = np.max(np.abs(g - r)) / scale
-        print(name, "max rel err", err)
-        assert err < rtol, (name, err)
+        err = np.abs(g - r) / max(scale, 1e-300)
+        print(name, "max rel err", err.max(), "at step", int(err.argmax()))
+        assert err.max() < rtol, (name, err.max(), int(err.argmax()))
 
 
 @pytest.mark.parametrize("use_graph", [0, 1])
@@ -99,6 +102,49 @@ def test_tiny2_fp32_close_to_oracle(lib32):
     sim, o = run_pair(lib32, geom, 40)
     worst = assert_fields_close(sim, o, 5e-4, 5e-6)
     print("fp32 worst field rel err", worst)
+    sim.close()
+
+
+def test_poiseuille_l2_matches_oracle(lib64):
+    """Empty two-level channel: the analytic-profile L2 of the GPU state equals
+    the oracle's, and decreases as the flow develops."""
+    # tau_c = 0.575, tau_f = 0.65: both below 1, so the f_neq rescale factors stay O(1)
+    geom = mbl.Geometry.channel(lib64, 81, 21, [(20, 50, 6, 14)], R=2.5, CX=-1000.0, CY=-1000.0, Re=20.0)
+    sim = mbl.MultiBlockSim(lib64, geom)
+    o = oracle.OracleSim(geom.g, sim.cfg)
+    vals = []
+    for n in (50, 300):
+        sim.step(n)
+        o.run(n)
+        g, r = sim.poiseuille_l2(), o.poiseuille_l2()
+        assert g == pytest.approx(r, rel=1e-10)
+        vals.append(g)
+    assert vals[1] < vals[0]
+    sim.close()
+
+
+def test_nan_is_reported(lib32):
+    geom = tiny2(lib32)
+    sim = mbl.MultiBlockSim(lib32, geom, flush_interval=4)
+    sim.step(2)
+    F = sim.read(0, ("F", 1))
+    F[5, 7] = np.nan
+    sim.write_F(0, 1, F)
+    with pytest.raises(mbl.MblError):
+        sim.step(8)
+        sim.flush()
+    assert sim.last_error() == -4
+    sim.close()
+
+
+def test_invalid_layout_rejected(lib32):
+    """A solid node on a block edge cannot be handled by the fused pull (it would
+    read outside the block): evolution() rejects the layout."""
+    geom = mbl.Geometry.channel(lib32, 61, 25, [(10, 30, 6, 18)], R=2.5, CX=10.0, CY=12.0, Re=20.0)
+    sim = mbl.MultiBlockSim(lib32, geom)
+    with pytest.raises(mbl.MblError):
+        sim.step(1)
+    assert sim.last_error() == -1
     sim.close()
 
 
