@@ -170,7 +170,8 @@ enum {
 enum {
     MBL_INIT_UNIFORM = 0, /* rho0, U0 everywhere (reference init_General) */
     MBL_INIT_REST = 1,    /* rho0, u = 0 */
-    MBL_INIT_RANDOM = 2   /* rho0, (u, v) ~ N(0, sigma^2) from a counter-based generator (seed) */
+    MBL_INIT_RANDOM = 2,  /* rho0, (u, v) ~ N(0, sigma^2) from a counter-based generator (seed) */
+    MBL_INIT_PROFILE = 3  /* rho0, the parabolic inlet profile u_in(y) on every non-solid node, rest in solids */
 };
 
 typedef struct mbl_config {
@@ -404,6 +405,41 @@ int mbl_batch_forces(mbl_batch_t b, int nt, double *cd, double *cl);
  * persistent groups) candidate and keeps the fastest; the trial steps advance
  * the cases. Returns 0 and the choice and its time per step. */
 int mbl_batch_autotune(mbl_batch_t b, int trial_steps, int *vx, int *tpb, int *groups, double *ms_per_step);
+
+/* ---- learned warm start (no reference counterpart) ----
+ * A convolutional encoder-decoder over an H x W raster (H, W multiples of 8):
+ * 4 levels of base, 2*base, 4*base, 4*base channels (base a multiple of 32),
+ * 3x3 convolutions with ReLU, 2x2 max pooling, nearest upsampling with skip
+ * concatenation, and a linear 1x1 head. Activations are BF16 (NHWC), every
+ * convolution runs on BF16 MFMA with FP32 accumulation, master weights and
+ * Adam state are FP32. Host tensors are [N][H][W][channels] FP32. */
+typedef struct mbl_net *mbl_net_t;
+int mbl_net_create(mbl_handle_t h, int H, int W, int in_ch, int out_ch, int base, int max_batch,
+                   unsigned long long seed, mbl_net_t *out);
+void mbl_net_destroy(mbl_net_t n);
+long long mbl_net_num_params(mbl_net_t n);
+/* get = 1 copies the FP32 parameters to host, get = 0 loads them from host */
+int mbl_net_params(mbl_net_t n, float *host, int get);
+int mbl_net_forward(mbl_net_t n, const float *in, int N, float *out);
+/* mean squared error over N x H x W x out_ch and its gradient (FP32, host, may be NULL) */
+int mbl_net_loss_grad(mbl_net_t n, const float *in, const float *target, int N, double *loss, float *grad);
+/* Adam (lr, betas 0.9/0.999) over `epochs` shuffled passes of `nsamples` samples held in HBM */
+int mbl_net_train(mbl_net_t n, const float *in, const float *target, int nsamples, int batch, int epochs, float lr,
+                  unsigned long long seed, double *epoch_loss);
+
+/* The AREA and the device D2Q9 of block q of case c (q = 0..3 the coarse
+ * blocks, 4 the refined block); pp->F holds the current state. For reading
+ * planes with mbl_read_distribution and for checkpoints. */
+int mbl_batch_block(mbl_batch_t b, int c, int q, AREA *area, D2Q9 *pp);
+/* (rho, ux, uy) of every case of a batch resampled onto an H x W raster
+ * covering its channel [0, LX-1] x [0, LY-1] (bilinear, finest block wins);
+ * out is [ncases][H][W][3] FP32 on the host. */
+int mbl_batch_raster(mbl_batch_t b, int H, int W, float *out);
+/* Re-initialises every case from rasters [ncases][H][W][3] (rho, ux, uy):
+ * F = feq(rho, u) + the first-order Chapman-Enskog non-equilibrium part
+ * -(tau - 1) dt w_k rho / cs^2 (e_k e_k - cs^2 I) : grad u, with the velocity
+ * gradient of the raster; solid nodes at rest. Interface rings are refilled. */
+int mbl_batch_init_from_raster(mbl_batch_t b, int H, int W, const float *fields);
 
 #ifdef __cplusplus
 }

@@ -205,6 +205,12 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
         return MBL_E_HIP;
     }
     h->stream = h->own_stream;
+    // default launch shape of the fused kernel per plane storage, from launch-shape sweeps on
+    // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 1 node/thread for 2-byte
+    // planes, 128 threads per workgroup
+    const int st = storage_mode(h->cfg);
+    h->vx = (st == ST_F16S || st == ST_BF16S) ? 1 : 4;
+    h->tpb = 128;
     if (hipMalloc(&h->d_l2, sizeof(double) * 3 * h->cap) != hipSuccess ||
         hipMalloc(&h->d_drag, sizeof(double) * 2 * h->cap) != hipSuccess ||
         hipHostMalloc(&h->h_l2, sizeof(double) * 3 * h->cap, hipHostMallocDefault) != hipSuccess ||
@@ -1629,7 +1635,66 @@ struct mbl_batch {
     double *d_hist = nullptr;      // force history: [cap][n][2]
     int cap = 4096, nhist = 0, head = 0;
     std::vector<double> norm;      // rho0 U^2 m R per case
+    // raster exchange: block/case tables (built on first use) and a staging buffer
+    RasterBlock *d_rblk = nullptr;
+    RasterCase *d_rcase = nullptr;
+    size_t max_nodes = 0;
+    float *d_raster = nullptr;
+    size_t raster_bytes = 0;
 };
+
+// The three ring slots of every coarse/fine line <- the fine edge's current
+// F (init_Fstore, multi/init_multi.h:3-28, for all lines of all cases at once).
+static int batch_ring_fill(mbl_batch *B)
+{
+    mbl_context *h = B->h;
+    const int n = B->n;
+    const int mode = storage_mode(h->cfg);
+    std::vector<LineOp> ops(4 * n);
+    std::vector<int> idx;
+    int maxn = 0;
+    for (int b = 0; b < 4 * n; b++) {
+        LineOp &op = ops[b];
+        memset(&op, 0, sizeof(op));
+        op.n = B->t3[b].N;
+        op.dst_block = B->lc[b].areaf_n;
+        const F_k3 *ft = &B->t3[b].f_t1;
+        for (int sl = 0; sl < 3; sl++)
+            for (int k = 0; k < 9; k++) op.ring[sl][k] = ft[sl].F[k];
+        const AREA &af = B->areas[op.dst_block];
+        op.src_idx = (const int *)(uintptr_t)idx.size(); // offset, fixed up below
+        for (int ip = 0; ip < op.n; ip++) {
+            int i, j;
+            line_node(B->lc[b].line_F, ip, i, j);
+            idx.push_back(j * af.nx + i);
+        }
+        maxn = std::max(maxn, op.n);
+    }
+    std::vector<BlockDesc> blocks(5 * n);
+    for (int a = 0; a < 5 * n; a++) blocks[a] = make_desc(h->cfg, B->areas[a], B->pps[a], 0, nullptr, nullptr);
+    const size_t bi = sizeof(int) * std::max<size_t>(1, idx.size());
+    const size_t bo = sizeof(LineOp) * ops.size();
+    const size_t bb = sizeof(BlockDesc) * blocks.size();
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char *tmp = nullptr;
+    HIPCHK(h, hipMalloc(&tmp, al(bi) + al(bo) + bb));
+    int *didx = (int *)tmp;
+    LineOp *dops = (LineOp *)(tmp + al(bi));
+    BlockDesc *dbl = (BlockDesc *)(tmp + al(bi) + al(bo));
+    for (auto &op : ops) {
+        op.dst_idx = didx + (uintptr_t)op.src_idx;
+        op.src_idx = nullptr;
+    }
+    int rc = 0;
+    if (hipMemcpyAsync(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(dops, ops.data(), bo, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(dbl, blocks.data(), bb, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = MBL_E_HIP;
+    if (!rc) rc = launch_ring_fill(h->stream, mode, dops, (int)ops.size(), maxn, dbl, 0u);
+    if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = MBL_E_HIP;
+    hipFree(tmp);
+    return rc;
+}
 
 static int batch_build(mbl_batch *B)
 {
@@ -1756,56 +1821,16 @@ static int batch_build(mbl_batch *B)
             BlockDesc d = make_desc(h->cfg, B->areas[a], B->pps[a], 0, nullptr, nullptr);
             d.bc = B->bc[a];
             int rc = launch_init_general(h->stream, mode, d, B->areas[a], B->pps[a].x, B->pps[a].y, B->pps[a].forcex,
-                                         B->pps[a].forcey, MBL_BC_CHANNEL, MBL_INIT_REST, 0, 0.0, (double)sp.LX,
+                                         B->pps[a].forcey, MBL_BC_CHANNEL, MBL_INIT_PROFILE, 0, 0.0, (double)sp.LX,
                                          (double)sp.LY, sp.R, sp.CX, sp.CY, d.bc);
             if (rc) return rc;
         }
-    // rings and tridiagonal arrays: one ring-fill op per line
-    {
-        std::vector<LineOp> ops(4 * n);
-        std::vector<int> idx;
-        for (int b = 0; b < 4 * n; b++) {
-            LineOp &op = ops[b];
-            memset(&op, 0, sizeof(op));
-            op.n = B->t3[b].N;
-            op.dst_block = B->lc[b].areaf_n;
-            const F_k3 *ft = &B->t3[b].f_t1;
-            for (int sl = 0; sl < 3; sl++)
-                for (int k = 0; k < 9; k++) op.ring[sl][k] = ft[sl].F[k];
-            const AREA &af = B->areas[op.dst_block];
-            op.src_idx = (const int *)(uintptr_t)idx.size(); // offset, fixed up below
-            for (int ip = 0; ip < op.n; ip++) {
-                int i, j;
-                line_node(B->lc[b].line_F, ip, i, j);
-                idx.push_back(j * af.nx + i);
-            }
-            int rc = launch_tri_A(h->stream, B->s3[b].l, B->s3[b].m, B->s3[b].u, B->s3[b].N);
-            if (rc) return rc;
-        }
-        int *didx = nullptr;
-        LineOp *dops = nullptr;
-        BlockDesc *dbl = nullptr;
-        std::vector<BlockDesc> blocks(5 * n);
-        for (int a = 0; a < 5 * n; a++) blocks[a] = make_desc(h->cfg, B->areas[a], B->pps[a], 0, nullptr, nullptr);
-        HIPCHK(h, hipMalloc(&didx, sizeof(int) * std::max<size_t>(1, idx.size())));
-        HIPCHK(h, hipMalloc(&dops, sizeof(LineOp) * ops.size()));
-        HIPCHK(h, hipMalloc(&dbl, sizeof(BlockDesc) * blocks.size()));
-        int maxn = 0;
-        for (auto &op : ops) {
-            op.dst_idx = didx + (uintptr_t)op.src_idx;
-            op.src_idx = nullptr;
-            maxn = std::max(maxn, op.n);
-        }
-        HIPCHK(h, hipMemcpy(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
-        HIPCHK(h, hipMemcpy(dops, ops.data(), sizeof(LineOp) * ops.size(), hipMemcpyHostToDevice));
-        HIPCHK(h, hipMemcpy(dbl, blocks.data(), sizeof(BlockDesc) * blocks.size(), hipMemcpyHostToDevice));
-        int rc = launch_ring_fill(h->stream, mode, dops, (int)ops.size(), maxn, dbl, 0u);
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        hipFree(didx);
-        hipFree(dops);
-        hipFree(dbl);
+    // tridiagonal arrays and rings of every line
+    for (int b = 0; b < 4 * n; b++) {
+        int rc = launch_tri_A(h->stream, B->s3[b].l, B->s3[b].m, B->s3[b].u, B->s3[b].N);
         if (rc) return rc;
     }
+    if (int rc = batch_ring_fill(B)) return rc;
     // drag links of every refined block
     std::vector<Int3> links;
     std::vector<int> off(n + 1, 0);
@@ -1876,6 +1901,9 @@ extern "C" void mbl_batch_destroy(mbl_batch_t B)
     if (B->d_hist) hipFree(B->d_hist);
     for (int p = 0; p < 2; p++)
         if (B->d_drag[p]) hipFree(B->d_drag[p]);
+    if (B->d_rblk) hipFree(B->d_rblk);
+    if (B->d_rcase) hipFree(B->d_rcase);
+    if (B->d_raster) hipFree(B->d_raster);
     delete B;
 }
 
@@ -1926,6 +1954,102 @@ extern "C" int mbl_batch_forces(mbl_batch_t B, int nt, double *cd, double *cl)
         }
     }
     return m;
+}
+
+// Block and case tables of the raster kernels and a staging buffer of
+// n x H x W x 3 floats. Plane pointers are taken as they are now; the
+// initialiser writes both plane sets, so their order does not matter.
+static int batch_raster_prepare(mbl_batch *B, int H, int W)
+{
+    mbl_context *h = B->h;
+    const int n = B->n;
+    if (H < 2 || W < 2 || (size_t)H * W > (1u << 26)) return set_err(h, MBL_E_INVALID);
+    if (!B->d_rblk) {
+        std::vector<RasterBlock> rb(5 * n);
+        std::vector<RasterCase> rc(n);
+        for (int a = 0; a < 5 * n; a++) {
+            RasterBlock &r = rb[a];
+            memset(&r, 0, sizeof(r));
+            const D2Q9 &pp = B->pps[a];
+            for (int k = 0; k < 9; k++) {
+                r.P[0][k] = pp.F[k];
+                r.P[1][k] = pp.f[k];
+            }
+            r.flag = pp.solid;
+            r.rho = pp.rho;
+            r.ux = pp.ux;
+            r.uy = pp.uy;
+            r.bx = B->areas[a].base_x;
+            r.by = B->areas[a].base_y;
+            r.tau = B->areas[a].tau;
+            r.m = B->areas[a].m;
+            r.nx = B->areas[a].nx;
+            r.ny = B->areas[a].ny;
+            r.cas = a < 4 * n ? a / 4 : a - 4 * n;
+            B->max_nodes = std::max(B->max_nodes, (size_t)r.nx * r.ny);
+        }
+        for (int c = 0; c < n; c++) {
+            RasterCase &k = rc[c];
+            k.nblk = 5;
+            k.blk[0] = 4 * n + c; // the refined block first
+            for (int q = 0; q < 4; q++) k.blk[q + 1] = 4 * c + q;
+            k.LX = B->cs[c].LX;
+            k.LY = B->cs[c].LY;
+        }
+        HIPCHK(h, hipMalloc(&B->d_rblk, sizeof(RasterBlock) * rb.size()));
+        HIPCHK(h, hipMalloc(&B->d_rcase, sizeof(RasterCase) * rc.size()));
+        HIPCHK(h, hipMemcpy(B->d_rblk, rb.data(), sizeof(RasterBlock) * rb.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(B->d_rcase, rc.data(), sizeof(RasterCase) * rc.size(), hipMemcpyHostToDevice));
+    }
+    const size_t bytes = sizeof(float) * 3 * (size_t)n * H * W;
+    if (bytes > B->raster_bytes) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (B->d_raster) hipFree(B->d_raster);
+        B->d_raster = nullptr;
+        B->raster_bytes = 0;
+        HIPCHK(h, hipMalloc(&B->d_raster, bytes));
+        B->raster_bytes = bytes;
+    }
+    return 0;
+}
+
+extern "C" int mbl_batch_raster(mbl_batch_t B, int H, int W, float *out)
+{
+    if (!B || !out) return MBL_E_INVALID;
+    mbl_context *h = B->h;
+    if (int rc = batch_raster_prepare(B, H, W)) return rc;
+    int rc = launch_raster(h->stream, B->d_rblk, B->d_rcase, B->n, H, W, B->d_raster);
+    if (rc) return set_err(h, rc);
+    HIPCHK(h, hipMemcpyAsync(out, B->d_raster, sizeof(float) * 3 * (size_t)B->n * H * W, hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mbl_batch_init_from_raster(mbl_batch_t B, int H, int W, const float *fields)
+{
+    if (!B || !fields) return MBL_E_INVALID;
+    mbl_context *h = B->h;
+    if (int rc = batch_raster_prepare(B, H, W)) return rc;
+    HIPCHK(h, hipMemcpyAsync(B->d_raster, fields, sizeof(float) * 3 * (size_t)B->n * H * W, hipMemcpyHostToDevice,
+                             h->stream));
+    int rc = launch_init_raster(h->stream, storage_mode(h->cfg), B->d_rblk, 5 * B->n, B->max_nodes, B->d_rcase, H, W,
+                                B->d_raster);
+    if (rc) return set_err(h, rc);
+    if ((rc = batch_ring_fill(B))) return set_err(h, rc);
+    // a restart: the force history starts again
+    B->nhist = 0;
+    B->head = 0;
+    return 0;
+}
+
+extern "C" int mbl_batch_block(mbl_batch_t B, int c, int q, AREA *area, D2Q9 *pp)
+{
+    if (!B || c < 0 || c >= B->n || q < 0 || q > 4) return MBL_E_INVALID;
+    const int a = q < 4 ? 4 * c + q : 4 * B->n + c;
+    if (area) *area = B->areas[a];
+    if (pp) *pp = B->pps[a];
+    return 0;
 }
 
 extern "C" int mbl_batch_autotune(mbl_batch_t B, int trial_steps, int *vx_out, int *tpb_out, int *groups_out,

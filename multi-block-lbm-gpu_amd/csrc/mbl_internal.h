@@ -89,6 +89,25 @@ struct DragArgs {
     double *out;       // [0] Fx, [1] Fy
 };
 
+// One block of a batched case for the raster resampler and the raster
+// initialiser: node (i, j) sits at coarse coordinates (bx + i/m, by + j/m).
+struct RasterBlock {
+    void *P[2][MBL_Q]; // both plane sets (the initialiser writes both)
+    const int *flag;
+    Lbd *rho, *ux, *uy;
+    double bx, by;
+    double tau;
+    int m, nx, ny;
+    int cas;           // case index
+};
+
+// A case's blocks, finest first, and its channel extent in coarse nodes.
+struct RasterCase {
+    int blk[5];
+    int nblk;
+    double LX, LY;
+};
+
 // ---- launchers (mbl_kernels.hip), all asynchronous on `stream` ----
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows,
@@ -107,6 +126,10 @@ int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total
                     int mode, double umax, double LY, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
+int launch_raster(void *stream, const RasterBlock *d_blocks, const RasterCase *d_cases, int ncases, int H, int W,
+                  float *d_out);
+int launch_init_raster(void *stream, int mode, const RasterBlock *d_blocks, int nblocks, size_t max_nodes,
+                       const RasterCase *d_cases, int H, int W, const float *d_fields);
 int launch_decode(void *stream, int mode, const void *plane, int k, size_t n, double *out);
 int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const double *in);
 

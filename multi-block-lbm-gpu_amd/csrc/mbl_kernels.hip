@@ -483,20 +483,6 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
     const double Y = j * dx + d.base_y;
     a.x[ixy] = (Lbd)X;
     a.y[ixy] = (Lbd)Y;
-    double rho = a.c.rho0, ux = a.c.U0x, uy = a.c.U0y;
-    if (a.init_mode == MBL_INIT_REST) {
-        ux = 0.0;
-        uy = 0.0;
-    } else if (a.init_mode == MBL_INIT_RANDOM) {
-        random_velocity(a.seed, a.sigma, i, j, ux, uy);
-    }
-    d.rho[ixy] = (Lbd)rho;
-    d.ux[ixy] = (Lbd)ux;
-    d.uy[ixy] = (Lbd)uy;
-    if (d.upx) d.upx[ixy] = (Lbd)ux;
-    if (d.upy) d.upy[ixy] = (Lbd)uy;
-    a.fx[ixy] = (Lbd)0.0;
-    a.fy[ixy] = (Lbd)0.0;
     int flag;
     if ((X - a.CX) * (X - a.CX) + (Y - a.CY) * (Y - a.CY) < a.R * a.R)
         flag = SOLID_SBB;
@@ -522,6 +508,24 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
             flag = FLUID;
     }
     ((int *)d.flag)[ixy] = flag;
+    double rho = a.c.rho0, ux = a.c.U0x, uy = a.c.U0y;
+    if (a.init_mode == MBL_INIT_REST) {
+        ux = 0.0;
+        uy = 0.0;
+    } else if (a.init_mode == MBL_INIT_RANDOM) {
+        random_velocity(a.seed, a.sigma, i, j, ux, uy);
+    } else if (a.init_mode == MBL_INIT_PROFILE) {
+        const bool solid = (flag == SOLID_SBB || flag == MBL_SOLID_WALL);
+        ux = solid ? 0.0 : 4.0 * a.c.umax * (a.LY - 1.5 - Y) * (Y - 0.5) / (a.LY - 2.0) / (a.LY - 2.0);
+        uy = 0.0;
+    }
+    d.rho[ixy] = (Lbd)rho;
+    d.ux[ixy] = (Lbd)ux;
+    d.uy[ixy] = (Lbd)uy;
+    if (d.upx) d.upx[ixy] = (Lbd)ux;
+    if (d.upy) d.upy[ixy] = (Lbd)uy;
+    a.fx[ixy] = (Lbd)0.0;
+    a.fy[ixy] = (Lbd)0.0;
     // F = f = feq(rho, u) in double, stored in the plane's domain
     const double uv = ux * ux + uy * uy;
 #pragma unroll
@@ -953,6 +957,120 @@ template <int MODE> __global__ void k_encode(void *plane, int k, size_t n, const
 }
 
 // ---------------------------------------------------------------------------
+// Raster exchange of batched cases (learned warm start; no reference
+// counterpart). A raster of H x W pixels spans the channel's coarse nodes
+// [0, LX-1] x [0, LY-1]; pixel (py, px) sits at X = px (LX-1)/(W-1),
+// Y = py (LY-1)/(H-1).
+// ---------------------------------------------------------------------------
+
+// Bilinear (rho, ux, uy) of a block at coarse coordinates (X, Y); false when
+// the point lies outside the block.
+__device__ inline bool block_sample(const RasterBlock &b, double X, double Y, double v[3])
+{
+    const double fx = (X - b.bx) * b.m, fy = (Y - b.by) * b.m;
+    const double eps = 1e-9;
+    if (fx < -eps || fy < -eps || fx > b.nx - 1 + eps || fy > b.ny - 1 + eps) return false;
+    int i0 = (int)floor(fx), j0 = (int)floor(fy);
+    i0 = min(max(i0, 0), b.nx - 2);
+    j0 = min(max(j0, 0), b.ny - 2);
+    const double tx = fx - i0, ty = fy - j0;
+    const size_t n00 = (size_t)j0 * b.nx + i0, n10 = n00 + 1, n01 = n00 + b.nx, n11 = n01 + 1;
+    const Lbd *src[3] = {b.rho, b.ux, b.uy};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const Lbd *s = src[q];
+        v[q] = (1 - ty) * ((1 - tx) * (double)s[n00] + tx * (double)s[n10]) +
+               ty * ((1 - tx) * (double)s[n01] + tx * (double)s[n11]);
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_raster(const RasterBlock *__restrict__ blocks,
+                                                const RasterCase *__restrict__ cases, int H, int W,
+                                                float *__restrict__ out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= H * W) return;
+    const RasterCase &c = cases[blockIdx.y];
+    const int px = p % W, py = p / W;
+    const double X = px * (c.LX - 1.0) / (W - 1), Y = py * (c.LY - 1.0) / (H - 1);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < c.nblk; q++)
+        if (block_sample(blocks[c.blk[q]], X, Y, v)) break;
+    float *o = out + ((size_t)blockIdx.y * H * W + p) * 3;
+    o[0] = (float)v[0];
+    o[1] = (float)v[1];
+    o[2] = (float)v[2];
+}
+
+// Bilinear sample of channel q of a case's raster at raster coordinates (gx, gy).
+__device__ inline double raster_at(const float *r, int H, int W, double gx, double gy, int q)
+{
+    gx = fmin(fmax(gx, 0.0), W - 1.0);
+    gy = fmin(fmax(gy, 0.0), H - 1.0);
+    int i0 = min((int)gx, W - 2), j0 = min((int)gy, H - 2);
+    const double tx = gx - i0, ty = gy - j0;
+    const float *r00 = r + ((size_t)j0 * W + i0) * 3;
+    const float *r01 = r00 + (size_t)W * 3;
+    return (1 - ty) * ((1 - tx) * r00[q] + tx * r00[3 + q]) + ty * ((1 - tx) * r01[q] + tx * r01[3 + q]);
+}
+
+// F = f = feq(rho, u) - (tau - 1) w_k rho / cs^2 (e_k e_k - cs^2 I) : grad u on
+// every node of every block, from the case's raster (first-order
+// Chapman-Enskog state after collision; grad u in the block's lattice units is
+// the coarse-unit gradient divided by m). Solid and wall nodes rest at rho = 1.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_init_raster(const RasterBlock *__restrict__ blocks,
+                                                     const RasterCase *__restrict__ cases, int H, int W,
+                                                     const float *__restrict__ fields)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const RasterBlock &b = blocks[blockIdx.y];
+    const size_t nn = (size_t)b.nx * b.ny;
+    const size_t ixy = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ixy >= nn) return;
+    const RasterCase &c = cases[b.cas];
+    const float *r = fields + (size_t)b.cas * H * W * 3;
+    const int i = (int)(ixy % b.nx), j = (int)(ixy / b.nx);
+    const double X = b.bx + (double)i / b.m, Y = b.by + (double)j / b.m;
+    const int flag = b.flag[ixy];
+    double rho = 1.0, ux = 0.0, uy = 0.0, dux_dx = 0.0, dux_dy = 0.0, duy_dx = 0.0, duy_dy = 0.0;
+    if (flag != SOLID_SBB && flag != MBL_SOLID_WALL) {
+        const double sx = (W - 1) / (c.LX - 1.0), sy = (H - 1) / (c.LY - 1.0); // raster pixels per coarse unit
+        const double gx = X * sx, gy = Y * sy;
+        rho = raster_at(r, H, W, gx, gy, 0);
+        ux = raster_at(r, H, W, gx, gy, 1);
+        uy = raster_at(r, H, W, gx, gy, 2);
+        const double xp = fmin(gx + 1.0, W - 1.0), xm = fmax(gx - 1.0, 0.0);
+        const double yp = fmin(gy + 1.0, H - 1.0), ym = fmax(gy - 1.0, 0.0);
+        const double hx = (xp - xm) / sx * b.m, hy = (yp - ym) / sy * b.m; // in the block's lattice units
+        dux_dx = (raster_at(r, H, W, xp, gy, 1) - raster_at(r, H, W, xm, gy, 1)) / hx;
+        duy_dx = (raster_at(r, H, W, xp, gy, 2) - raster_at(r, H, W, xm, gy, 2)) / hx;
+        dux_dy = (raster_at(r, H, W, gx, yp, 1) - raster_at(r, H, W, gx, ym, 1)) / hy;
+        duy_dy = (raster_at(r, H, W, gx, yp, 2) - raster_at(r, H, W, gx, ym, 2)) / hy;
+    }
+    b.rho[ixy] = (Lbd)rho;
+    b.ux[ixy] = (Lbd)ux;
+    b.uy[ixy] = (Lbd)uy;
+    const double uv = ux * ux + uy * uy;
+    const double div = dux_dx + duy_dy;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const double eu = EX[k] * ux + EY[k] * uy;
+        const double poly = 3.0 * eu + 4.5 * eu * eu - 1.5 * uv;
+        // e_a e_b d_a u_b - cs^2 div u
+        const double qg = EX[k] * EX[k] * dux_dx + EX[k] * EY[k] * (dux_dy + duy_dx) + EY[k] * EY[k] * duy_dy -
+                          div / 3.0;
+        const double neq = -(b.tau - 1.0) * W64[k] * rho * 3.0 * qg;
+        const double q = (C::SHIFT ? W64[k] * ((rho - 1.0) + rho * poly) : W64[k] * rho * (1.0 + poly)) + neq;
+        const T v = C::st((real_t)q);
+        ((T *)b.P[0][k])[ixy] = v;
+        ((T *)b.P[1][k])[ixy] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 #define MBL_LAUNCH_CHECK()                                                                                             \
@@ -1185,6 +1303,28 @@ int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const do
     hipStream_t s = (hipStream_t)stream;
     dim3 g((unsigned)((n + 255) / 256)), b(256);
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_encode<M_>), g, b, 0, s, plane, k, n, in));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_raster(void *stream, const RasterBlock *d_blocks, const RasterCase *d_cases, int ncases, int H, int W,
+                  float *d_out)
+{
+    if (ncases <= 0 || H < 2 || W < 2) return MBL_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 g((unsigned)((H * W + 255) / 256), ncases), b(256);
+    hipLaunchKernelGGL(k_raster, g, b, 0, s, d_blocks, d_cases, H, W, d_out);
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_init_raster(void *stream, int mode, const RasterBlock *d_blocks, int nblocks, size_t max_nodes,
+                       const RasterCase *d_cases, int H, int W, const float *d_fields)
+{
+    if (nblocks <= 0 || H < 2 || W < 2) return MBL_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 g((unsigned)((max_nodes + 255) / 256), nblocks), b(256);
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_init_raster<M_>), g, b, 0, s, d_blocks, d_cases, H, W, d_fields));
     MBL_LAUNCH_CHECK();
     return 0;
 }

@@ -1,0 +1,55 @@
+// mbl_net.h -- internal declarations of the warm-start network kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mbl {
+namespace net {
+
+typedef uint16_t bf16; // bfloat16 bits
+
+struct ConvArgs {
+    const bf16 *X;
+    int ldx, xoff;      // input leading dimension (channels per pixel) and channel offset
+    int N, H, W, Cin;   // Cin % 32 == 0
+    const bf16 *Wt;     // [Cout][ksize*ksize][Cin]
+    const float *bias;  // [Cout] or null
+    int Cout;           // % 16 == 0
+    int ksize;          // 3 or 1
+    bf16 *Y;            // BF16 output or null
+    int ldy, yoff;
+    float *Yf;          // FP32 output or null
+    int ldyf;
+    const bf16 *M;      // multiply by (M > 0) or null
+    int ldm, moff;
+    int relu;
+};
+
+struct WgradArgs {
+    const bf16 *dY;
+    int ldd, doff;
+    const bf16 *X;
+    int ldx, xoff;
+    int N, H, W, Cin, Cout, ksize;
+    float *partial;     // [nsplit][Cout][ksize*ksize][Cin]
+};
+
+void conv(hipStream_t s, const ConvArgs &a);
+void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit);
+void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int accumulate);
+void bias_grad(hipStream_t s, const bf16 *dY, int ldd, int doff, int P, int C, float *db);
+void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb);
+void to_bf16(hipStream_t s, const float *src, size_t n, bf16 *dst);
+void maxpool(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg);
+void maxpool_bwd(hipStream_t s, const bf16 *dY, const uint8_t *arg, int N, int H, int W, int C, bf16 *dX);
+void upsample(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, int ld, int off);
+void upsample_bwd(hipStream_t s, const bf16 *dY, int ld, int off, int N, int H, int W, int C, bf16 *dX);
+void copy_ch(hipStream_t s, const bf16 *src, int lds, int soff, bf16 *dst, int ldd, int doff, int C, int P, int add);
+int mse(hipStream_t s, const float *Y, int ldy, const float *T, int C, int P, bf16 *dY, int ldg, float *partial);
+void adam(hipStream_t s, float *w, const float *g, float *m, float *v, size_t n, float lr, int t);
+void pack_input(hipStream_t s, const float *src, int Creal, int P, bf16 *dst, int Cpad);
+
+} // namespace net
+} // namespace mbl

@@ -1,0 +1,393 @@
+// mbl_net_kernels.hip -- kernels of the warm-start network (a small conv
+// encoder-decoder) for gfx950: BF16 activations in NHWC layout, BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, FP32 master weights.
+//
+// MFMA operand layout used here (16x16 output tile, K = 32 per instruction):
+// lane l supplies row (l % 16) of A and column (l % 16) of B, eight K values
+// each from group g = l / 16; the accumulator lane l holds rows 4g..4g+3 of
+// column l % 16. Both operands take their K values from the same (g, element)
+// positions, so the K order inside a group does not affect the product.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mbl_net.h"
+
+namespace mbl {
+namespace net {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ inline float bf2f(bf16 v) { return __uint_as_float(((uint32_t)v) << 16); }
+__device__ inline bf16 f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u); // round to nearest even (finite values)
+    return (bf16)(u >> 16);
+}
+
+__device__ inline bf16x8 load8(const bf16 *p)
+{
+    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return *reinterpret_cast<bf16x8 *>(&v);
+}
+
+__device__ inline bf16x8 zero8()
+{
+    uint4 v = {0u, 0u, 0u, 0u};
+    return *reinterpret_cast<bf16x8 *>(&v);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 (or 1x1) convolution, zero padding, as an implicit GEMM:
+//   out[p][co] = bias[co] + sum_{tap, ci} X[p + tap][ci] * Wt[co][tap][ci]
+// Workgroup: 4 waves = 64 output pixels x 32 output channels; wave w owns
+// pixels 16w..16w+15 and both 16-channel halves. Cin % 32 == 0, Cout % 16 == 0.
+// Epilogue: optional bias, optional multiply by (mask > 0) (ReLU derivative of
+// the layer input in the backward-data pass), optional ReLU, BF16 store into a
+// channel slice of Y (leading dimension ldy, offset yoff).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int P = a.N * a.H * a.W;
+    const int p0 = blockIdx.x * 64 + 16 * w;
+    const int c0 = blockIdx.y * 32;
+    const int p = p0 + r16;
+    const int n = p / (a.H * a.W), rem = p - n * a.H * a.W;
+    const int y = rem / a.W, x = rem - y * a.W;
+    const bool two = (c0 + 16 < a.Cout);
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int taps = a.ksize * a.ksize, half = a.ksize / 2;
+    for (int tap = 0; tap < taps; tap++) {
+        const int yy = y + tap / a.ksize - half, xx = x + tap % a.ksize - half;
+        const bool valid = (p < P) && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+        const bf16 *xrow = a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff;
+        const bf16 *w0 = a.Wt + ((size_t)(c0 + r16) * taps + tap) * a.Cin;
+        const bf16 *w1 = a.Wt + ((size_t)(c0 + 16 + r16) * taps + tap) * a.Cin;
+        for (int ci = 0; ci < a.Cin; ci += 32) {
+            const bf16x8 av = valid ? load8(xrow + ci + 8 * g) : zero8();
+            const bf16x8 b0 = load8(w0 + ci + 8 * g);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc0, 0, 0, 0);
+            if (two) {
+                const bf16x8 b1 = load8(w1 + ci + 8 * g);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc1, 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        if (t == 1 && !two) break;
+        const f32x4 acc = t ? acc1 : acc0;
+        const int co = c0 + 16 * t + r16;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int pix = p0 + 4 * g + r;
+            if (pix >= P) continue;
+            float v = acc[r];
+            if (a.bias) v += a.bias[co];
+            if (a.M && !(bf2f(a.M[(size_t)pix * a.ldm + a.moff + co]) > 0.f)) v = 0.f;
+            if (a.relu) v = fmaxf(v, 0.f);
+            if (a.Yf) a.Yf[(size_t)pix * a.ldyf + co] = v;
+            if (a.Y) a.Y[(size_t)pix * a.ldy + a.yoff + co] = f2bf(v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[co][tap][ci] += sum_p dY[p][co] * X[p + tap][ci] over one
+// pixel chunk per blockIdx.z, written as FP32 partials (reduced in order by
+// k_reduce_partials). Workgroup = one wave computing a 16 co x 16 ci tile of
+// one tap; K runs over pixels in steps of 32.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_conv_wgrad(WgradArgs a)
+{
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int taps = a.ksize * a.ksize, half = a.ksize / 2;
+    const int c0 = blockIdx.x * 16;
+    const int tap = blockIdx.y % taps;
+    const int i0 = (blockIdx.y / taps) * 16;
+    const int P = a.N * a.H * a.W;
+    const int chunk = (P + gridDim.z - 1) / gridDim.z;
+    const int pb = blockIdx.z * chunk, pe = min(P, pb + chunk);
+    const int dy = tap / a.ksize - half, dx = tap % a.ksize - half;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = pb; k0 < pe; k0 += 32) {
+        __bf16 av[8], bv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int p = k0 + 8 * g + e;
+            uint16_t ua = 0, ub = 0;
+            if (p < pe) {
+                ua = a.dY[(size_t)p * a.ldd + a.doff + c0 + r16];
+                const int n = p / (a.H * a.W), rem = p - n * a.H * a.W;
+                const int yy = rem / a.W + dy, xx = rem % a.W + dx;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                    ub = a.X[((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff + i0 + r16];
+            }
+            av[e] = __builtin_bit_cast(__bf16, ua);
+            bv[e] = __builtin_bit_cast(__bf16, ub);
+        }
+        bf16x8 A, B;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            A[e] = av[e];
+            B[e] = bv[e];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+    }
+    // acc[r] = D[co = c0 + 4g + r][ci = i0 + r16]
+    float *out = a.partial + (size_t)blockIdx.z * a.Cout * taps * a.Cin;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int co = c0 + 4 * g + r;
+        out[((size_t)co * taps + tap) * a.Cin + i0 + r16] = acc[r];
+    }
+}
+
+// out[i] (+)= sum_s partial[s][i], in split order (deterministic)
+__global__ void k_reduce_partials(const float *partial, int nsplit, size_t n, float *out, int accumulate)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < nsplit; k++) s += partial[(size_t)k * n + i];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+// db[c] = sum_p dY[p][c]: one workgroup per channel, fixed-order tree
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *dY, int ldd, int doff, int P, float *db)
+{
+    __shared__ float sh[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int p = threadIdx.x; p < P; p += 256) s += bf2f(dY[(size_t)p * ldd + doff + c]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// Backward-data weights: Wb[ci][tap][co] = W[co][taps-1-tap][ci] (BF16)
+__global__ void k_flip_weights(const float *W, int Cout, int taps, int Cin, bf16 *Wb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)Cout * taps * Cin;
+    if (i >= n) return;
+    const int ci = (int)(i % Cin);
+    const int tap = (int)((i / Cin) % taps);
+    const int co = (int)(i / ((size_t)Cin * taps));
+    Wb[((size_t)ci * taps + (taps - 1 - tap)) * Cout + co] = f2bf(W[i]);
+}
+
+__global__ void k_to_bf16(const float *src, size_t n, bf16 *dst)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = f2bf(src[i]);
+}
+
+// 2x2 max pool (NHWC); arg holds the winning child 0..3 per output element
+__global__ void k_maxpool(const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t n_out = (size_t)N * Ho * Wo * C;
+    if (i >= n_out) return;
+    const int c = (int)(i % C);
+    const size_t q = i / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
+    float best = -3.4e38f;
+    int bi = 0;
+    for (int k = 0; k < 4; k++) {
+        const int yy = 2 * yo + k / 2, xx = 2 * xo + k % 2;
+        const float v = bf2f(X[((size_t)(n * H + yy) * W + xx) * C + c]);
+        if (v > best) {
+            best = v;
+            bi = k;
+        }
+    }
+    Y[i] = f2bf(best);
+    arg[i] = (uint8_t)bi;
+}
+
+__global__ void k_maxpool_bwd(const bf16 *dY, const uint8_t *arg, int N, int H, int W, int C, bf16 *dX)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t n_out = (size_t)N * Ho * Wo * C;
+    if (i >= n_out) return;
+    const int c = (int)(i % C);
+    const size_t q = i / C;
+    const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
+    for (int k = 0; k < 4; k++) {
+        const int yy = 2 * yo + k / 2, xx = 2 * xo + k % 2;
+        dX[((size_t)(n * H + yy) * W + xx) * C + c] = (k == arg[i]) ? dY[i] : (bf16)0;
+    }
+}
+
+// nearest x2 upsample of X[N][H][W][C] into channels [off, off+C) of Y (ld)
+__global__ void k_upsample(const bf16 *X, int N, int H, int W, int C, bf16 *Y, int ld, int off)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n_out = (size_t)N * 2 * H * 2 * W * C;
+    if (i >= n_out) return;
+    const int c = (int)(i % C);
+    const size_t q = i / C;
+    const int x = (int)(q % (2 * W)), y = (int)((q / (2 * W)) % (2 * H)), n = (int)(q / ((size_t)4 * W * H));
+    Y[q * ld + off + c] = X[((size_t)(n * H + y / 2) * W + x / 2) * C + c];
+}
+
+// dX[n][y][x][c] = sum of the four children of channels [off, off+C) of dY
+__global__ void k_upsample_bwd(const bf16 *dY, int ld, int off, int N, int H, int W, int C, bf16 *dX)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n_in = (size_t)N * H * W * C;
+    if (i >= n_in) return;
+    const int c = (int)(i % C);
+    const size_t q = i / C;
+    const int x = (int)(q % W), y = (int)((q / W) % H), n = (int)(q / ((size_t)W * H));
+    float s = 0.f;
+    for (int k = 0; k < 4; k++) {
+        const size_t pc = (size_t)(n * 2 * H + 2 * y + k / 2) * (2 * W) + 2 * x + k % 2;
+        s += bf2f(dY[pc * ld + off + c]);
+    }
+    dX[i] = f2bf(s);
+}
+
+// channel-slice copy / accumulate: dst[p][doff+c] (+)= src[p][soff+c]
+__global__ void k_copy_ch(const bf16 *src, int lds, int soff, bf16 *dst, int ldd, int doff, int C, int P, int add)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)P * C) return;
+    const int c = (int)(i % C);
+    const size_t p = i / C;
+    float v = bf2f(src[p * lds + soff + c]);
+    if (add) v += bf2f(dst[p * ldd + doff + c]);
+    dst[p * ldd + doff + c] = f2bf(v);
+}
+
+// mean squared error over P pixels x C channels of an FP32 output; dY in BF16
+// padded to ldg channels (the head's MFMA width); loss partials per workgroup
+__global__ __launch_bounds__(256) void k_mse(const float *Y, int ldy, const float *T, int C, int P, bf16 *dY, int ldg,
+                                             float *partial)
+{
+    __shared__ float sh[256];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < (size_t)P * ldg) {
+        const int c = (int)(i % ldg);
+        const size_t p = i / ldg;
+        float d = 0.f;
+        if (c < C) {
+            d = Y[p * ldy + c] - T[p * C + c];
+            l = d * d;
+        }
+        dY[i] = f2bf(2.f * d / (float)((size_t)P * C));
+    }
+    sh[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master parameters
+__global__ void k_adam(float *w, const float *g, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 host-layout input [P][Cin_real] -> BF16 padded [P][Cpad]
+__global__ void k_pack_input(const float *src, int Creal, int P, bf16 *dst, int Cpad)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)P * Cpad) return;
+    const int c = (int)(i % Cpad);
+    const size_t p = i / Cpad;
+    dst[i] = c < Creal ? f2bf(src[p * Creal + c]) : (bf16)0;
+}
+
+// ---- launchers ----
+static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void conv(hipStream_t s, const ConvArgs &a)
+{
+    const int P = a.N * a.H * a.W;
+    hipLaunchKernelGGL(k_conv, dim3((P + 63) / 64, (a.Cout + 31) / 32), dim3(256), 0, s, a);
+}
+void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit)
+{
+    const int taps = a.ksize * a.ksize;
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(a.Cout / 16, taps * (a.Cin / 16), nsplit), dim3(64), 0, s, a);
+}
+void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int acc)
+{
+    hipLaunchKernelGGL(k_reduce_partials, g1(n), dim3(256), 0, s, partial, nsplit, n, out, acc);
+}
+void bias_grad(hipStream_t s, const bf16 *dY, int ldd, int doff, int P, int C, float *db)
+{
+    hipLaunchKernelGGL(k_bias_grad, dim3(C), dim3(256), 0, s, dY, ldd, doff, P, db);
+}
+void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb)
+{
+    hipLaunchKernelGGL(k_flip_weights, g1((size_t)Cout * taps * Cin), dim3(256), 0, s, W, Cout, taps, Cin, Wb);
+}
+void to_bf16(hipStream_t s, const float *src, size_t n, bf16 *dst)
+{
+    hipLaunchKernelGGL(k_to_bf16, g1(n), dim3(256), 0, s, src, n, dst);
+}
+void maxpool(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg)
+{
+    hipLaunchKernelGGL(k_maxpool, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, X, N, H, W, C, Y, arg);
+}
+void maxpool_bwd(hipStream_t s, const bf16 *dY, const uint8_t *arg, int N, int H, int W, int C, bf16 *dX)
+{
+    hipLaunchKernelGGL(k_maxpool_bwd, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, dY, arg, N, H, W, C,
+                       dX);
+}
+void upsample(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, int ld, int off)
+{
+    hipLaunchKernelGGL(k_upsample, g1((size_t)N * 4 * H * W * C), dim3(256), 0, s, X, N, H, W, C, Y, ld, off);
+}
+void upsample_bwd(hipStream_t s, const bf16 *dY, int ld, int off, int N, int H, int W, int C, bf16 *dX)
+{
+    hipLaunchKernelGGL(k_upsample_bwd, g1((size_t)N * H * W * C), dim3(256), 0, s, dY, ld, off, N, H, W, C, dX);
+}
+void copy_ch(hipStream_t s, const bf16 *src, int lds, int soff, bf16 *dst, int ldd, int doff, int C, int P, int add)
+{
+    hipLaunchKernelGGL(k_copy_ch, g1((size_t)P * C), dim3(256), 0, s, src, lds, soff, dst, ldd, doff, C, P, add);
+}
+int mse(hipStream_t s, const float *Y, int ldy, const float *T, int C, int P, bf16 *dY, int ldg, float *partial)
+{
+    const size_t n = (size_t)P * ldg;
+    hipLaunchKernelGGL(k_mse, g1(n), dim3(256), 0, s, Y, ldy, T, C, P, dY, ldg, partial);
+    return (int)((n + 255) / 256);
+}
+void adam(hipStream_t s, float *w, const float *g, float *m, float *v, size_t n, float lr, int t)
+{
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, g1(n), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, 1e-8f, c1, c2);
+}
+void pack_input(hipStream_t s, const float *src, int Creal, int P, bf16 *dst, int Cpad)
+{
+    hipLaunchKernelGGL(k_pack_input, g1((size_t)P * Cpad), dim3(256), 0, s, src, Creal, P, dst, Cpad);
+}
+
+} // namespace net
+} // namespace mbl

@@ -19,7 +19,7 @@ C_BOUNDARY = 9
 
 STORE_NATIVE, STORE_F32S, STORE_F16S, STORE_BF16S = 0, 1, 2, 3
 BC_REFERENCE, BC_CHANNEL, BC_PERIODIC = 0, 1, 2
-INIT_UNIFORM, INIT_REST, INIT_RANDOM = 0, 1, 2
+INIT_UNIFORM, INIT_REST, INIT_RANDOM, INIT_PROFILE = 0, 1, 2, 3
 
 E_NAMES = {0: "ok", -1: "invalid", -2: "hip", -3: "nomem", -4: "not finite",
            -5: "diverged", -6: "no handle", -7: "unsupported"}
@@ -121,5 +121,7 @@ EXPORTS = [
     "mbl_diagnostics_series", "mbl_diagnostics_clear", "mbl_step_block", "mbl_timer_begin",
     "mbl_timer_end", "mbl_set_tuning", "mbl_poiseuille_l2", "mbl_set_persistent",
     "mbl_batch_create", "mbl_batch_destroy", "mbl_batch_step", "mbl_batch_node_updates", "mbl_batch_forces",
-    "mbl_batch_autotune",
+    "mbl_batch_autotune", "mbl_batch_block", "mbl_batch_raster", "mbl_batch_init_from_raster",
+    "mbl_net_create", "mbl_net_destroy", "mbl_net_num_params", "mbl_net_params", "mbl_net_forward",
+    "mbl_net_loss_grad", "mbl_net_train",
 ]

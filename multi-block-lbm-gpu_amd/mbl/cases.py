@@ -51,7 +51,12 @@ def run(lib, geom, steps, storage=abi.STORE_NATIVE, chunk=1000, D=None, U=0.1, p
     out["Cl_final"] = float(cl[-1])
     if D is not None:
         out.update(postproc.force_statistics(cd, cl, D=D, U=U, transient=0.5))
-    out["t_converged_Cd"] = postproc.time_to_converged(cd, rel=0.005, window=None)
+    # steady flow: the instantaneous Cd; shedding: the Cd mean over one shedding period
+    window = None
+    if D is not None and out.get("St", 0.0) > 0.02 and out.get("Cl_amp", 0.0) > 1e-3:
+        window = int(round(D / (out["St"] * U)))
+        out["shedding_period_steps"] = window
+    out["t_converged_Cd"] = postproc.time_to_converged(cd, rel=0.005, window=window)
     sim.close()
     return out, {"t": drag["t"], "Cd": cd, "Cl": cl, "residual": l2["residual"]}
 

@@ -297,6 +297,14 @@ static void init_general(const mbl_config *c, orc_block *b)
                     flag = FLUID;
             }
             pp.solid[ixy] = flag;
+            if (c->init_mode == MBL_INIT_PROFILE) { /* EXT: developed inlet profile, rest in solids */
+                int solid = (flag == SOLID_SBB || flag == MBL_SOLID_WALL);
+                ux = solid ? 0.0 : inlet_u(c, Y);
+                pp.ux[ixy] = ux;
+                pp.uy[ixy] = 0.0;
+                b->upx[ixy] = ux;
+                b->upy[ixy] = 0.0;
+            }
             for (int k = 0; k < 9; k++) {
                 pp.F[k][ixy] = orc_feq(k, pp.rho[ixy], pp.ux[ixy], pp.uy[ixy]);
                 pp.f[k][ixy] = pp.F[k][ixy];

@@ -24,7 +24,7 @@ def test_batch_equals_individual_runs(lib64):
     cd_b, cl_b = b.forces(steps)
     b.close()
     for c, sp in enumerate(specs):
-        sim = mbl.MultiBlockSim(lib64, single_geometry(lib64, sp), U0=(sp.U, 0.0))
+        sim = mbl.MultiBlockSim(lib64, single_geometry(lib64, sp), U0=(sp.U, 0.0), init_mode=abi.INIT_PROFILE)
         sim.step(steps)
         _, drag = sim.series()
         sim.close()

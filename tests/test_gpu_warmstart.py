@@ -1,0 +1,213 @@
+"""Learned warm start on the GPU: the BF16 MFMA network against its CPU
+reference (tests/net_reference.py, BF16 rounding emulated at the places the
+GPU stores BF16), its training, and the raster exchange of batched cases
+(mbl_batch_raster / mbl_batch_init_from_raster) against numpy restatements.
+
+Tolerances: the GPU accumulates in FP32 in another order than the float64
+reference, so a value near a BF16 rounding boundary can round the other way
+(one BF16 ulp, 2^-8 relative) and carry that into later layers; gradients are
+also stored in BF16 between layers on the GPU only. Forward outputs agree to
+2e-2 relative L2 and gradients to 6e-2 per layer (cosine >= 0.995); a
+transposed, flipped or mis-offset operand gives O(1) errors.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mbl
+from mbl import abi
+from mbl.net import Net, unpack
+
+sys.path.insert(0, os.path.dirname(__file__))
+import net_reference as ref  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+H, W, IN, OUT, BASE = 16, 32, 5, 3, 32
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def cosine(a, b):
+    return float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+
+
+def sample(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, H, W, IN)).astype(np.float32)
+    t = rng.normal(size=(n, H, W, OUT)).astype(np.float32)
+    return x, t
+
+
+def test_net_forward_matches_reference(lib32):
+    net = Net(lib32, H, W, IN, OUT, base=BASE, max_batch=4, seed=11)
+    x, _ = sample(3, 1)
+    y = net.forward(x)
+    layers = unpack(net.get_params(), IN, OUT, BASE)
+    y_ref = ref.output(layers, x.astype(np.float64), bf16=True)
+    y_f64 = ref.output(layers, x.astype(np.float64), bf16=False)
+    e, e64 = rel(y, y_ref), rel(y, y_f64)
+    print(f"forward rel L2 vs bf16 reference {e:.3e}, vs float64 reference {e64:.3e}", flush=True)
+    assert np.isfinite(y).all()
+    assert e <= 2e-2
+    assert e64 <= 6e-2
+    net.close()
+
+
+def test_net_gradients_match_reference(lib32):
+    net = Net(lib32, H, W, IN, OUT, base=BASE, max_batch=4, seed=12)
+    x, t = sample(2, 2)
+    loss, g = net.loss_grad(x, t)
+    layers = unpack(net.get_params(), IN, OUT, BASE)
+    gl = unpack(g, IN, OUT, BASE)
+    loss_ref, g_ref = ref.loss_and_grads(layers, x.astype(np.float64), t.astype(np.float64), bf16=True)
+    print(f"loss gpu {loss:.6e} ref {loss_ref:.6e}", flush=True)
+    assert abs(loss - loss_ref) <= 2e-2 * loss_ref
+    worst = 0.0
+    for name in ref.ORDER:
+        for part in (0, 1):
+            a, b = gl[name][part], g_ref[name][part]
+            e, c = rel(a, b), cosine(a, b)
+            print(f"  {name} {'W' if part == 0 else 'b'}: rel {e:.3e} cos {c:.6f}", flush=True)
+            worst = max(worst, e)
+            assert e <= 6e-2 and c >= 0.995, (name, part, e, c)
+    # padding weights of the input and head layers get no gradient
+    table, _ = mbl.net.layer_table(IN, OUT, BASE)
+    name, cin, cr, cout, cor, k, woff, boff = table[0]
+    w0 = g[woff:woff + cout * k * k * cin].reshape(cout, k, k, cin)
+    assert np.all(w0[:, :, :, cr:] == 0.0)
+    name, cin, cr, cout, cor, k, woff, boff = table[-1]
+    assert np.all(g[woff + cor * cin:woff + cout * cin] == 0.0) and np.all(g[boff + cor:boff + cout] == 0.0)
+    net.close()
+
+
+def test_net_training_reduces_loss(lib32):
+    rng = np.random.default_rng(3)
+    n = 16
+    yy, xx = np.meshgrid(np.linspace(-1, 1, H), np.linspace(-1, 1, W), indexing="ij")
+    x = np.zeros((n, H, W, IN), np.float32)
+    t = np.zeros((n, H, W, OUT), np.float32)
+    for s in range(n):
+        cx, cy, r = rng.uniform(-0.5, 0.5), rng.uniform(-0.3, 0.3), rng.uniform(0.1, 0.3)
+        mask = ((xx - cx) ** 2 + (yy - cy) ** 2 < r * r).astype(np.float32)
+        x[s, :, :, 0] = mask
+        x[s, :, :, 1] = xx
+        x[s, :, :, 2] = yy
+        x[s, :, :, 3] = r
+        t[s, :, :, 0] = np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2 * r * r))
+        t[s, :, :, 1] = (xx - cx) * t[s, :, :, 0]
+        t[s, :, :, 2] = mask * 0.5
+    net = Net(lib32, H, W, IN, OUT, base=BASE, max_batch=8, seed=13)
+    hist = net.train(x, t, batch=8, epochs=80, lr=2e-3, seed=5)
+    print(f"epoch loss {hist[0]:.4e} -> {hist[-1]:.4e}", flush=True)
+    assert np.isfinite(hist).all()
+    assert hist[-1] < 0.2 * hist[0]
+    # the trained parameters predict what the loss says
+    y = net.forward(x)
+    assert abs(float(np.mean((y - t) ** 2)) - hist[-1]) < 0.5 * hist[-1] + 1e-4
+    net.close()
+
+
+# ---------------------------------------------------------------------------
+# raster exchange
+# ---------------------------------------------------------------------------
+SPECS = [mbl.case_spec(5, 0.2, 60.0), mbl.case_spec(8, 0.25, 100.0)]
+
+
+def test_raster_of_inlet_profile(lib64):
+    b = mbl.Batch(lib64, SPECS)
+    for Hr, Wr in [(32, 128), (17, 40)]:
+        r = b.raster(Hr, Wr)
+        assert r.shape == (len(SPECS), Hr, Wr, 3)
+        for c, sp in enumerate(SPECS):
+            LX, LY = sp.LX, sp.LY
+            Y = np.arange(Hr)[:, None] * (LY - 1.0) / (Hr - 1) + 0 * np.arange(Wr)[None, :]
+            X = np.arange(Wr)[None, :] * (LX - 1.0) / (Wr - 1) + 0 * Y
+            u = 4 * sp.U * (LY - 1.5 - Y) * (Y - 0.5) / (LY - 2.0) ** 2
+            far = ((X - sp.CX) ** 2 + (Y - sp.CY) ** 2 > (sp.R + 2) ** 2) & (Y >= 1.0) & (Y <= LY - 2.0)
+            # bilinear interpolation of a parabola between nodes one coarse cell apart
+            tol = sp.U * 1.0 / (LY - 2.0) ** 2 * 1.01 + 1e-12
+            assert np.max(np.abs(r[c][..., 1][far] - u[far])) <= tol
+            assert np.max(np.abs(r[c][..., 0][far] - 1.0)) <= 1e-6
+            assert np.max(np.abs(r[c][..., 2][far])) <= 1e-7
+    b.close()
+
+
+def restate_init(raster, LX, LY, area, flag, tau, nodes):
+    """F of nodes (i, j) of a block after mbl_batch_init_from_raster, in numpy."""
+    Hr, Wr = raster.shape[:2]
+    r = raster.astype(np.float64)
+
+    def at(gx, gy, q):
+        gx = min(max(gx, 0.0), Wr - 1.0)
+        gy = min(max(gy, 0.0), Hr - 1.0)
+        i0, j0 = min(int(gx), Wr - 2), min(int(gy), Hr - 2)
+        tx, ty = gx - i0, gy - j0
+        return ((1 - ty) * ((1 - tx) * r[j0, i0, q] + tx * r[j0, i0 + 1, q]) +
+                ty * ((1 - tx) * r[j0 + 1, i0, q] + tx * r[j0 + 1, i0 + 1, q]))
+
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+    ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    out = []
+    for i, j in nodes:
+        X, Y = area.base_x + i / area.m, area.base_y + j / area.m
+        if flag[j, i] in (abi.SOLID_SBB, abi.SOLID_WALL):
+            rho, ux, uy, g = 1.0, 0.0, 0.0, np.zeros((2, 2))
+        else:
+            sx, sy = (Wr - 1) / (LX - 1.0), (Hr - 1) / (LY - 1.0)
+            gx, gy = X * sx, Y * sy
+            rho, ux, uy = at(gx, gy, 0), at(gx, gy, 1), at(gx, gy, 2)
+            xp, xm = min(gx + 1, Wr - 1.0), max(gx - 1, 0.0)
+            yp, ym = min(gy + 1, Hr - 1.0), max(gy - 1, 0.0)
+            hx, hy = (xp - xm) / sx * area.m, (yp - ym) / sy * area.m
+            # g[a][b] = d u_b / d x_a in the block's lattice units
+            g = np.array([[(at(xp, gy, 1) - at(xm, gy, 1)) / hx, (at(xp, gy, 2) - at(xm, gy, 2)) / hx],
+                          [(at(gx, yp, 1) - at(gx, ym, 1)) / hy, (at(gx, yp, 2) - at(gx, ym, 2)) / hy]])
+        eu = ex * ux + ey * uy
+        feq = w * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (ux * ux + uy * uy))
+        qg = ex * ex * g[0, 0] + ex * ey * (g[0, 1] + g[1, 0]) + ey * ey * g[1, 1] - (g[0, 0] + g[1, 1]) / 3
+        out.append(feq - (tau - 1) * w * rho * 3 * qg)
+    return np.array(out)
+
+
+def test_init_from_raster_matches_numpy(lib64):
+    b = mbl.Batch(lib64, SPECS)
+    b.step(5)
+    Hr, Wr = 24, 64
+    rng = np.random.default_rng(9)
+    fields = b.raster(Hr, Wr)
+    fields[..., 1:] += rng.normal(0, 0.01, fields[..., 1:].shape).astype(np.float32)
+    b.init_from_raster(fields)
+    for c, sp in enumerate(SPECS):
+        for q in (0, 4):
+            area, _ = b.block(c, q)
+            flag = b.read(c, q, "solid")
+            nodes = [(int(rng.integers(0, area.nx)), int(rng.integers(0, area.ny))) for _ in range(40)]
+            nodes += [(0, 0), (area.nx - 1, area.ny - 1)]
+            want = restate_init(fields[c], sp.LX, sp.LY, area, flag, area.tau, nodes)
+            got = np.array([[b.read(c, q, f"F{k}")[j, i] for k in range(9)] for i, j in nodes])
+            assert np.max(np.abs(got - want)) <= 1e-12, (c, q)
+    b.close()
+
+
+def test_raster_roundtrip_on_coarse_nodes(lib64):
+    b = mbl.Batch(lib64, SPECS[:1])
+    b.step(20)
+    sp = SPECS[0]
+    Hr, Wr = sp.LY, sp.LX  # one pixel per coarse node
+    f = b.raster(Hr, Wr)
+    b.init_from_raster(f)
+    g = b.raster(Hr, Wr)
+    Y, X = np.meshgrid(np.arange(Hr, dtype=float), np.arange(Wr, dtype=float), indexing="ij")
+    keep = ((X - sp.CX) ** 2 + (Y - sp.CY) ** 2 > (sp.R + 1.5) ** 2) & (Y >= 1) & (Y <= Hr - 2)
+    assert np.max(np.abs(g[0][keep] - f[0][keep])) <= 1e-6
+    # the restarted case runs on and stays finite, with the force history restarted
+    b.step(50)
+    cd, cl = b.forces(100)
+    assert cd.shape[1] == 50 and np.isfinite(cd).all() and np.isfinite(cl).all()
+    b.close()
