@@ -24,8 +24,8 @@ sys.path.insert(0, os.path.join(REPO, "multi-block-lbm-gpu_amd"))
 
 METRIC = "MLUPS & %HBM roofline; time-to-converged Cd (±0.5%) vs plain FP32"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_NODE = {0: 72, 1: 72, 2: 36, 3: 36}
-STORAGE_NAMES = {0: "f32", 1: "f32s", 2: "f16s", 3: "bf16s"}
+BYTES_PER_NODE = {0: 72, 1: 72, 2: 36, 3: 36, 4: 36, 5: 36}
+STORAGE_NAMES = {0: "f32", 1: "f32s", 2: "f16s", 3: "bf16s", 4: "f16c", 5: "f16d"}
 
 
 def traffic_from_profile(storage):
@@ -42,6 +42,11 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
     import mbl
 
     blk = mbl.PeriodicBlock(lib, n, n, tau=0.6, storage=storage, tuning=tuning)
+    # bring the GPU clocks up first: untimed steps for at least half a second, then the W warmup steps
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.5:
+        blk.step(10)
+        blk.sync()
     blk.step(warmup)
     blk.sync()
     t0 = time.perf_counter()
@@ -93,7 +98,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--storage", type=int, default=0, help="0 f32, 1 f32 shifted, 2 f16 shifted, 3 bf16 shifted")
+    ap.add_argument("--storage", type=int, default=0, help="0 f32, 1 f32 shifted, 2 f16 shifted, 3 bf16 shifted, 4 1-4-11 half shifted, 5 1-3-12 half shifted")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", action="store_true", help="also time the 16-bit storage modes")
@@ -141,7 +146,7 @@ def main():
     }
     if args.variants:
         var = {}
-        for st in (1, 2, 3):
+        for st in (1, 2, 3, 4, 5):
             v = gpu_run(lib, args.n, max(50, args.steps // 4), args.warmup, st)
             b = BYTES_PER_NODE[st]
             var[STORAGE_NAMES[st]] = {"mlups": round(v["mlups"], 1), "per_launch_ms": round(v["per_launch_ms"], 5),

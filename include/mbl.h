@@ -158,7 +158,15 @@ enum {
     MBL_STORE_NATIVE = 0, /* Lbd planes holding f (FP32 build: float, FP64 build: double) */
     MBL_STORE_F32S = 1,   /* float planes holding f - w_k             (FP32 build only) */
     MBL_STORE_F16S = 2,   /* IEEE binary16 planes holding f - w_k      (FP32 build only) */
-    MBL_STORE_BF16S = 3   /* bfloat16 planes holding f - w_k           (FP32 build only) */
+    MBL_STORE_BF16S = 3,  /* bfloat16 planes holding f - w_k           (FP32 build only) */
+    MBL_STORE_F16C = 4,   /* 16-bit planes holding f - w_k in a 1-4-11 format: sign, 4
+                           * exponent bits (bias 15, no infinities), 11 fraction bits;
+                           * normal range 2^-14 .. 2 - 2^-10, subnormal step 2^-25,
+                           * round to nearest even, saturating; 0xFFFF is NaN.
+                           * (FP32 build only) */
+    MBL_STORE_F16D = 5    /* as MBL_STORE_F16C in a 1-3-12 format: 3 exponent bits (bias
+                           * 7), 12 fraction bits; normal range 2^-6 .. 2 - 2^-11,
+                           * subnormal step 2^-18. (FP32 build only) */
 };
 
 enum {

@@ -19,6 +19,10 @@ using namespace mbl;
 
 namespace {
 
+// Rows per stream-collide tile of a multi-block layout beyond the waves of a
+// workgroup (a loop over rows lowers occupancy more than it saves: measured).
+constexpr int MBL_TILE_ROWS = 1;
+
 constexpr int EXh[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 constexpr int EYh[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 
@@ -39,6 +43,9 @@ struct Layout {
     std::vector<BlockDesc> hb;
     BlockDesc *db = nullptr;
     int tiles[MBL_MAX_LEVEL] = {0};
+    int tile_off[MBL_MAX_LEVEL] = {0}; // first entry of each level in dtile
+    int *dtile = nullptr;              // per tile: descriptor index within its level
+    int rows = 0;                      // rows per stream-collide tile
     int vx[MBL_MAX_LEVEL] = {0};
     bool periodic = false;
     std::vector<LineOp> hops;
@@ -58,6 +65,8 @@ struct Layout {
         if (didx) hipFree(didx);
         if (dmask) hipFree(dmask);
         if (dM64) hipFree(dM64);
+        if (dtile) hipFree(dtile);
+        dtile = nullptr;
         db = nullptr;
         dops = nullptr;
         didx = nullptr;
@@ -152,7 +161,7 @@ static int storage_mode(const mbl_config &c)
 #ifdef MBL_FP64
     return c.storage == MBL_STORE_NATIVE ? ST_NATIVE : -1;
 #else
-    return (c.storage >= 0 && c.storage <= 3) ? c.storage : -1;
+    return (c.storage >= 0 && c.storage <= 5) ? c.storage : -1;
 #endif
 }
 
@@ -206,10 +215,11 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
     }
     h->stream = h->own_stream;
     // default launch shape of the fused kernel per plane storage, from launch-shape sweeps on
-    // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 1 node/thread for 2-byte
-    // planes, 128 threads per workgroup
+    // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 2 for IEEE half and
+    // bfloat16 planes, 1 for the custom 16-bit formats (whose conversions are ALU work), 128
+    // threads per workgroup
     const int st = storage_mode(h->cfg);
-    h->vx = (st == ST_F16S || st == ST_BF16S) ? 1 : 4;
+    h->vx = (st == ST_F16S || st == ST_BF16S) ? 2 : ((st == ST_F16C || st == ST_F16D) ? 1 : 4);
     h->tpb = 128;
     if (hipMalloc(&h->d_l2, sizeof(double) * 3 * h->cap) != hipSuccess ||
         hipMalloc(&h->d_drag, sizeof(double) * 2 * h->cap) != hipSuccess ||
@@ -697,8 +707,11 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     std::vector<int> level_of(nblocks, -1);
     for (int l = 0; l < cfg.LEVEL; l++)
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) level_of[a] = l;
-    const int rows = h->tpb / 64;
+    const int rows = std::max(h->tpb / 64, MBL_TILE_ROWS);
+    L->rows = rows;
+    std::vector<int> tile_blk;
     for (int l = 0; l < cfg.LEVEL; l++) {
+        L->tile_off[l] = (int)tile_blk.size();
         int vx = h->vx;
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
             while (vx > 1 && A.areas[a].nx % vx) vx >>= 1;
@@ -714,6 +727,7 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
             d.tiles_x = (ar.nx + 64 * vx - 1) / (64 * vx);
             d.tile_begin = tiles;
             tiles += d.tiles_x * ((ar.ny + rows - 1) / rows);
+            tile_blk.resize(L->tile_off[l] + tiles, a - L->ASP[l]);
             L->hb[a] = d;
             if (validate_flags(h, ar, A.pps[a].solid, L->periodic, nullptr)) {
                 L->release();
@@ -935,7 +949,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         hipMalloc(&L->dops, sizeof(LineOp) * std::max<size_t>(1, all.size())) != hipSuccess ||
         hipMalloc(&L->didx, sizeof(int) * std::max<size_t>(1, nidx)) != hipSuccess ||
         hipMalloc(&L->dmask, std::max<size_t>(1, nmask)) != hipSuccess ||
-        hipMalloc(&L->dM64, sizeof(double) * std::max<size_t>(1, nM)) != hipSuccess) {
+        hipMalloc(&L->dM64, sizeof(double) * std::max<size_t>(1, nM)) != hipSuccess ||
+        hipMalloc(&L->dtile, sizeof(int) * std::max<size_t>(1, tile_blk.size())) != hipSuccess) {
         L->release();
         delete L;
         return set_err(h, MBL_E_NOMEM);
@@ -960,6 +975,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         L->hops.push_back(op);
     }
     bool ok = hipMemcpy(L->db, L->hb.data(), sizeof(BlockDesc) * nblocks, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !tile_blk.empty())
+        ok = hipMemcpy(L->dtile, tile_blk.data(), sizeof(int) * tile_blk.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !L->hops.empty())
         ok = hipMemcpy(L->dops, L->hops.data(), sizeof(LineOp) * L->hops.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !hidx.empty())
@@ -1027,8 +1044,9 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
         }
         if (cs.launch && L->tiles[level] > 0) {
             int rc = launch_stream_collide(h->stream, mode, L->vx[level], h->tpb, L->periodic, true, true,
-                                           L->db + L->ASP[level], L->ASP[level + 1] - L->ASP[level], L->tiles[level],
-                                           h->tpb / 64, cs.pm, h->groups);
+                                           L->db + L->ASP[level], L->dtile + L->tile_off[level],
+                                           L->ASP[level + 1] - L->ASP[level], L->tiles[level], L->rows, cs.pm,
+                                           h->groups);
             if (rc) cs.rc = rc;
         }
         cs.pm ^= (1u << level);
@@ -1227,8 +1245,8 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     const int mode = storage_mode(h->cfg);
     const bool macros = !S.periodic || S.flags;
     for (int t = 0; t < nsteps; t++) {
-        int rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, 1, S.tiles,
-                                       S.rows, (uint32_t)par, h->groups);
+        int rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, nullptr, 1,
+                                       S.tiles, S.rows, (uint32_t)par, h->groups);
         if (rc) return set_err(h, rc);
         par ^= 1;
         for (int k = 0; k < 9; k++) std::swap(pp->F[k], pp->f[k]);

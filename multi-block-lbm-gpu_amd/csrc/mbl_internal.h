@@ -15,7 +15,7 @@ typedef float real_t;
 
 namespace mbl {
 
-enum StoreMode { ST_NATIVE = 0, ST_F32S = 1, ST_F16S = 2, ST_BF16S = 3 };
+enum StoreMode { ST_NATIVE = 0, ST_F32S = 1, ST_F16S = 2, ST_BF16S = 3, ST_F16C = 4, ST_F16D = 5 };
 
 // Element size in bytes of one f/F plane entry for a storage mode.
 inline size_t store_elem_bytes(int mode)
@@ -109,8 +109,10 @@ struct RasterCase {
 };
 
 // ---- launchers (mbl_kernels.hip), all asynchronous on `stream` ----
+// tile_rows rows per tile (at least tpb / 64; the waves take them in turn);
+// d_tile_blk[t] = descriptor of tile t, or null to search tile_begin.
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
-                          const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows,
+                          const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
                           uint32_t parity_mask, int groups);
 int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &area, Lbd *x, Lbd *y,
                         Lbd *forcex, Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed,

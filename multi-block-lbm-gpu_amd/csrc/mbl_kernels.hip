@@ -77,6 +77,62 @@ template <> struct Codec<ST_BF16S> {
         return (T)(u >> 16);
     }
 };
+
+// Custom 16-bit floats for shifted populations: sign, EB exponent bits with
+// bias 2^EB - 1 (every code finite, the top binade is [1, 2)), 15 - EB
+// fraction bits, subnormals below 2^(2 - 2^EB). |f - w_k| stays well inside
+// (2^-14, 2), so the exponent bits IEEE binary16 spends on range are better
+// spent on the significand. Code 0xFFFF is NaN so a diverging run stays
+// visible; overflow saturates. Decoding places the 15 magnitude bits under
+// the binary32 exponent field and rescales by a power of two (subnormal
+// codes come out as binary32 subnormals and scale exactly); encoding rounds
+// to nearest even on the dropped bits, and in the subnormal range rounds the
+// exactly scaled value to an integer with one addition. Both are branch-free.
+template <int EB> struct Half16 {
+    static constexpr int FB = 15 - EB;                       // fraction bits
+    static constexpr int BIAS = (1 << EB) - 1;
+    static constexpr uint32_t OFF = (uint32_t)(127 - BIAS);  // binary32 exponent offset
+    __device__ static inline float ld(uint16_t v)
+    {
+        const uint32_t b = v;
+        const float r = __uint_as_float(((b & 0x7FFFu) << (23 - FB)) | ((b & 0x8000u) << 16)) *
+                        __uint_as_float((OFF + 127u) << 23);
+        return b == 0xFFFFu ? __uint_as_float(0x7FC00000u) : r;
+    }
+    __device__ static inline uint16_t st(float v)
+    {
+        const uint32_t u = __float_as_uint(v);
+        const uint32_t s = (u >> 16) & 0x8000u, a = u & 0x7FFFFFFFu;
+        constexpr uint32_t SH = 23 - FB, HALF = (1u << (SH - 1)) - 1u;
+        uint32_t nrm = ((a + HALF + ((a >> SH) & 1u)) >> SH) - (OFF << FB);
+        nrm = min(nrm, 0x7FFFu); // saturate (and infinity)
+        // below 2^(1 - BIAS) the code is |v| 2^(BIAS - 1 + FB) (exact, < 2^FB + 1/2) rounded to the
+        // nearest even integer: adding 1.5 * 2^23 rounds it once, in the addition
+        const float scaled = __uint_as_float(a) * __uint_as_float((uint32_t)(127 + BIAS - 1 + FB) << 23);
+        const uint32_t sub = __float_as_uint(scaled + 12582912.0f) - 0x4B400000u;
+        uint32_t code = (a >= ((OFF + 1u) << 23) ? nrm : sub) | s;
+        code = code == 0xFFFFu ? 0xFFFEu : code;
+        return (uint16_t)(a > 0x7F800000u ? 0xFFFFu : code);
+    }
+};
+
+// 1-4-11: normal range 2^-14 .. 2 - 2^-10, subnormal step 2^-25.
+template <> struct Codec<ST_F16C> {
+    typedef uint16_t T;
+    typedef GLOBAL_AS uint16_t GT;
+    static constexpr bool SHIFT = true;
+    __device__ static inline float ld(T v) { return Half16<4>::ld(v); }
+    __device__ static inline T st(float v) { return Half16<4>::st(v); }
+};
+
+// 1-3-12: normal range 2^-6 .. 2 - 2^-11, subnormal step 2^-18.
+template <> struct Codec<ST_F16D> {
+    typedef uint16_t T;
+    typedef GLOBAL_AS uint16_t GT;
+    static constexpr bool SHIFT = true;
+    __device__ static inline float ld(T v) { return Half16<3>::ld(v); }
+    __device__ static inline T st(float v) { return Half16<3>::st(v); }
+};
 #endif
 
 // f value in double of a stored entry of direction k
@@ -304,27 +360,14 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
 // a workgroup covers tile_rows rows. Interior all-fluid runs use aligned
 // VX-wide loads of each source row plus one scalar for the x-shifted planes.
 // ---------------------------------------------------------------------------
+// One row segment of a tile: 64*VX consecutive nodes of row j starting at the
+// wave's first node.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ inline void sc_tile(const BlockDesc *__restrict__ descs, int ndesc, int tile_rows, uint32_t parity_mask, int t)
+__device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, int x0)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
-    int lo = 0, hi = ndesc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (descs[mid].tile_begin <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const BlockDesc &d = descs[lo];
     const Consts &c = d.bc;
-    const int lt = t - d.tile_begin;
-    const int tyi = lt / d.tiles_x;
-    const int txi = lt - tyi * d.tiles_x;
-    const int lane = threadIdx.x & 63;
-    const int j = tyi * tile_rows + (int)(threadIdx.x >> 6);
-    const int x0 = (txi * 64 + lane) * VX;
     const int nx = d.nx, ny = d.ny;
     if (j >= ny || x0 >= nx) return;
     const int par = (parity_mask >> d.level) & 1;
@@ -338,6 +381,49 @@ __device__ inline void sc_tile(const BlockDesc *__restrict__ descs, int ndesc, i
     }
     const real_t omega = (real_t)d.omega;
     const size_t r0 = (size_t)j * nx;
+
+    if constexpr (FLAGS && !PERIODIC && VX == 1) {
+        // One instruction stream for every flag a block edge or body can
+        // carry, so narrow blocks (whose waves all hold edge lanes) do not run
+        // a second, scalar path: each lane pulls its 9 sources (its own node
+        // for a ghost copy or a pull from outside the block), collides, and
+        // keeps the collided, the bounced-back or the copied values by select.
+        // Only inlet/outlet nodes and fluid nodes on the block edge (reference
+        // edge rule) branch to the general update.
+        const int i = x0;
+        const size_t ixy = r0 + i;
+        const int flag = d.flag[ixy];
+        const bool edge = (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
+        if (flag == BOUNDARY_INLET || flag == BOUNDARY_OUTLET || (flag == FLUID && edge)) {
+            general_node<MODE, PERIODIC, FLAGS, MACROS>(d, S, D, i, j, c, omega);
+            return;
+        }
+        const bool solid = (flag == SOLID_SBB || flag == MBL_SOLID_WALL);
+        const bool fluid = (flag == FLUID);
+        const bool copy = !solid && !fluid; // C_BOUNDARY and any other flag: F carried over
+        T raw[9];
+        real_t q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int id = i - EX[k], jd = j - EY[k];
+            const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
+            raw[k] = S[k][own ? ixy : (size_t)jd * nx + id];
+            q[k] = C::ld(raw[k]);
+        }
+        real_t Q[9], rho, ux, uy;
+        collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const T v = fluid ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
+            __builtin_nontemporal_store(v, D[k] + ixy);
+        }
+        if (MACROS && fluid) {
+            __builtin_nontemporal_store((Lbd)rho, (GLOBAL_AS Lbd *)d.rho + ixy);
+            __builtin_nontemporal_store((Lbd)ux, (GLOBAL_AS Lbd *)d.ux + ixy);
+            __builtin_nontemporal_store((Lbd)uy, (GLOBAL_AS Lbd *)d.uy + ixy);
+        }
+        return;
+    }
 
     bool fast;
     if (PERIODIC)
@@ -416,23 +502,56 @@ __device__ inline void sc_tile(const BlockDesc *__restrict__ descs, int ndesc, i
     }
 }
 
+// One tile: 64*VX columns x tile_rows rows of one block; the workgroup's
+// waves take the rows in turn. The block comes from the per-tile table when
+// there is one (one scalar load), else from a search of tile_begin.
+template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
+__device__ inline void sc_tile(const BlockDesc *__restrict__ descs, const int *__restrict__ tile_blk, int ndesc,
+                               int tile_rows, uint32_t parity_mask, int t)
+{
+    int lo = 0;
+    if (tile_blk) {
+        lo = tile_blk[t];
+    } else {
+        int hi = ndesc - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].tile_begin <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+    }
+    const BlockDesc &d = descs[lo];
+    const int lt = t - d.tile_begin;
+    const int tyi = lt / d.tiles_x;
+    const int txi = lt - tyi * d.tiles_x;
+    const int x0 = (txi * 64 + (int)(threadIdx.x & 63)) * VX;
+    const int waves = (int)(blockDim.x >> 6);
+    for (int r = (int)(threadIdx.x >> 6); r < tile_rows; r += waves)
+        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, tyi * tile_rows + r, x0);
+}
+
 // One tile per workgroup when the grid covers every tile; otherwise a
 // persistent grid in which workgroup b (after the XCD-aware reordering) walks
 // one contiguous chunk of tiles, so x-neighbouring tiles stay on one XCD.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs, int ndesc, int tile_rows,
+__global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs,
+                                                        const int *__restrict__ tile_blk, int ndesc, int tile_rows,
                                                         uint32_t parity_mask, int total_tiles)
 {
     const int G = (int)gridDim.x;
     const int b = xcd_tile((int)blockIdx.x, G);
     if (G >= total_tiles) {
-        if (b < total_tiles) sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, ndesc, tile_rows, parity_mask, b);
+        if (b < total_tiles)
+            sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, b);
         return;
     }
     const int per = (total_tiles + G - 1) / G;
     const int t0 = b * per;
     const int t1 = min(total_tiles, t0 + per);
-    for (int t = t0; t < t1; t++) sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, ndesc, tile_rows, parity_mask, t);
+    for (int t = t0; t < t1; t++)
+        sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -1079,41 +1198,41 @@ __global__ __launch_bounds__(256) void k_init_raster(const RasterBlock *__restri
     } while (0)
 
 template <int MODE, int VX>
-static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d, int nd,
-                       int tiles, int rows, uint32_t pm, int groups)
+static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d,
+                       const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups)
 {
     dim3 g(groups > 0 && groups < tiles ? groups : tiles), b(tpb);
     if (periodic) {
         if (flags) {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
         } else {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
         }
     } else {
         if (macros)
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, nd, rows, pm, tiles);
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
         else
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, nd, rows, pm, tiles);
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
     }
     MBL_LAUNCH_CHECK();
     return 0;
 }
 
 template <int MODE>
-static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d, int nd,
-                   int tiles, int rows, uint32_t pm, int groups)
+static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d,
+                   const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups)
 {
     switch (vx) {
-    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
-    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
-    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
-    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, nd, tiles, rows, pm, groups);
+    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
+    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
+    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
+    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
     default: return MBL_E_INVALID;
     }
 }
@@ -1147,19 +1266,27 @@ static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bo
         constexpr int M_ = ST_BF16S;                                                                                   \
         CALL;                                                                                                          \
     } break;                                                                                                           \
+    case ST_F16C: {                                                                                                    \
+        constexpr int M_ = ST_F16C;                                                                                    \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
+    case ST_F16D: {                                                                                                    \
+        constexpr int M_ = ST_F16D;                                                                                    \
+        CALL;                                                                                                          \
+    } break;                                                                                                           \
     default: return MBL_E_UNSUPPORTED;                                                                                 \
     }
 #endif
 
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
-                          const BlockDesc *d_descs, int ndesc, int total_tiles, int tile_rows, uint32_t parity_mask,
-                          int groups)
+                          const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
+                          uint32_t parity_mask, int groups)
 {
     if (total_tiles <= 0) return 0;
-    if (tpb != 64 * tile_rows || tpb > 256) return MBL_E_INVALID;
+    if (tpb % 64 || tpb < 64 || tpb > 256 || tile_rows < tpb / 64) return MBL_E_INVALID;
     hipStream_t s = (hipStream_t)stream;
-    MBL_MODE_SWITCH(mode, return sc_mode<M_>(s, vx, tpb, periodic, flags, macros, d_descs, ndesc, total_tiles,
-                                             tile_rows, parity_mask, groups));
+    MBL_MODE_SWITCH(mode, return sc_mode<M_>(s, vx, tpb, periodic, flags, macros, d_descs, d_tile_blk, ndesc,
+                                             total_tiles, tile_rows, parity_mask, groups));
     return 0;
 }
 

@@ -17,7 +17,7 @@ BOUNDARY_INLET = 5
 BOUNDARY_OUTLET = 7
 C_BOUNDARY = 9
 
-STORE_NATIVE, STORE_F32S, STORE_F16S, STORE_BF16S = 0, 1, 2, 3
+STORE_NATIVE, STORE_F32S, STORE_F16S, STORE_BF16S, STORE_F16C, STORE_F16D = 0, 1, 2, 3, 4, 5
 BC_REFERENCE, BC_CHANNEL, BC_PERIODIC = 0, 1, 2
 INIT_UNIFORM, INIT_REST, INIT_RANDOM, INIT_PROFILE = 0, 1, 2, 3
 

@@ -1,6 +1,6 @@
 """Cylinder cases of the multi-block solver, run through the C ABI.
 
-    python -m mbl.cases c1  [--steps 20000] [--storage f32|f32s|f16s|bf16s] [--fp64]
+    python -m mbl.cases c1  [--steps 20000] [--storage f32|f32s|f16s|f16c|f16d|bf16s] [--fp64]
     python -m mbl.cases c2b [--steps 50000] ...
 
 c1:  two-level channel of 300 x 80 coarse cells, one ratio-2 block of 80 x 40
@@ -19,18 +19,21 @@ import time
 
 import numpy as np
 
-from . import MultiBlockSim, c1_geometry, c2b_geometry, load_library, postproc
+from . import MultiBlockSim, c1_geometry, c2b_geometry, check, load_library, postproc
 from . import abi
 
-STORAGE = {"f32": abi.STORE_NATIVE, "f32s": abi.STORE_F32S, "f16s": abi.STORE_F16S, "bf16s": abi.STORE_BF16S}
+STORAGE = {"f32": abi.STORE_NATIVE, "f32s": abi.STORE_F32S, "f16s": abi.STORE_F16S, "f16c": abi.STORE_F16C, "f16d": abi.STORE_F16D,
+           "bf16s": abi.STORE_BF16S}
 
 
 def node_updates_per_step(sim):
     return sum(sim.areas[a].nx * sim.areas[a].ny * sim.areas[a].m for a in range(sim.na))
 
 
-def run(lib, geom, steps, storage=abi.STORE_NATIVE, chunk=1000, D=None, U=0.1, progress=False, **kw):
+def run(lib, geom, steps, storage=abi.STORE_NATIVE, chunk=1000, D=None, U=0.1, progress=False, tuning=None, **kw):
     sim = MultiBlockSim(lib, geom, storage=storage, flush_interval=min(chunk, 4096), **kw)
+    if tuning:
+        check(lib, lib.mbl_set_tuning(sim.h, tuning[0], tuning[1]), "tuning")
     t0 = time.perf_counter()
     done = 0
     while done < steps:
@@ -69,6 +72,8 @@ def main(argv=None):
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--save", default=None, help="write the Cd/Cl/residual series to this .npz")
     ap.add_argument("--progress", action="store_true")
+    ap.add_argument("--vx", type=int, default=0, help="nodes per thread of the fused kernel (0: library default)")
+    ap.add_argument("--tpb", type=int, default=0, help="threads per workgroup (0: library default)")
     args = ap.parse_args(argv)
     lib = load_library(args.fp64)
     if args.case == "c1":
@@ -76,7 +81,7 @@ def main(argv=None):
     else:
         geom, steps, D = c2b_geometry(lib), args.steps or 50000, 16.0
     out, series = run(lib, geom, steps, storage=STORAGE[args.storage] if not args.fp64 else abi.STORE_NATIVE,
-                      D=D, progress=args.progress)
+                      D=D, progress=args.progress, tuning=(args.vx, args.tpb) if (args.vx or args.tpb) else None)
     out.update({"case": args.case, "precision": "f64" if args.fp64 else args.storage})
     if args.save:
         np.savez(args.save, **series)

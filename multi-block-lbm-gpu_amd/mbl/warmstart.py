@@ -1,6 +1,6 @@
 """Learned warm start of batched cylinder cases.
 
-    python -m mbl.warmstart [--train 128] [--test 64] [--spin 40000] [--steps 60000] [--epochs 30] ...
+    python -m mbl.warmstart [--train 256] [--test 64] [--spin 40000] [--steps 60000] [--epochs 60] ...
 
 1. Training cases (two-level channels, D in {16, 24} finest cells, blockage
    D/H ~ U[0.1, 0.3], Re ~ U[80, 200], the cylinder 0.25 coarse cells above
@@ -8,8 +8,10 @@
    `spin` steps, then `nsnap` snapshots of (rho, ux, uy) are rasterised onto an
    H x W grid of each channel, every `every` steps.
 2. Inputs per pixel: solid mask, Re/200, D/H, sin and cos of the shedding
-   phase (from the analytic signal of Cl), x and y; targets:
-   ((rho - 1)/U^2, ux/U, uy/U). The encoder-decoder trains on the GPU.
+   phase (linear in time between the upward zero crossings of Cl that
+   bracket the snapshot), x and y; targets: (0.25 (rho - 1)/U^2, ux/U, uy/U).
+   The encoder-decoder trains on the GPU; 1/8 of the training cases are held
+   out for a validation loss.
 3. Held-out cases run twice from step 0: cold (profile at rest) and warm
    (the network's field at phase 0, with the Chapman-Enskog non-equilibrium
    part). For each, the steps until the Cl amplitude (half the peak-to-peak
@@ -27,7 +29,8 @@ import time
 
 import numpy as np
 from scipy.ndimage import maximum_filter1d, minimum_filter1d
-from scipy.signal import hilbert
+from scipy.sparse import coo_matrix
+from scipy.sparse.linalg import splu
 
 from . import Batch, abi, case_spec, load_library
 from .net import Net
@@ -47,7 +50,7 @@ def c5_specs(n, seed, offset=0.25):
 
 
 def raster_inputs(specs, H, W, phase):
-    """[n, H, W, 7]: mask, Re/200, D/H, sin phase, cos phase, x, y."""
+    """[n, H, W, 7]: mask, Re/200, D/H, sin and cos of the phase, x, y."""
     n = len(specs)
     x = np.zeros((n, H, W, IN_CH), np.float32)
     phase = np.broadcast_to(np.asarray(phase, np.float64), (n,))
@@ -64,10 +67,13 @@ def raster_inputs(specs, H, W, phase):
     return x
 
 
+RHO_SCALE = 0.25  # scale of the density target: its weight in the loss (measured best of 0.25 and 1)
+
+
 def to_target(fields, specs):
     U = np.array([sp.U for sp in specs])[:, None, None]
     t = np.empty_like(fields)
-    t[..., 0] = (fields[..., 0] - 1.0) / U ** 2
+    t[..., 0] = RHO_SCALE * (fields[..., 0] - 1.0) / U ** 2
     t[..., 1] = fields[..., 1] / U
     t[..., 2] = fields[..., 2] / U
     return t
@@ -76,10 +82,88 @@ def to_target(fields, specs):
 def from_target(t, specs):
     U = np.array([sp.U for sp in specs])[:, None, None]
     f = np.empty_like(t)
-    f[..., 0] = 1.0 + t[..., 0] * U ** 2
+    f[..., 0] = 1.0 + t[..., 0] / RHO_SCALE * U ** 2
     f[..., 1] = t[..., 1] * U
     f[..., 2] = t[..., 2] * U
     return f
+
+
+def poisson_density(fields, specs):
+    """Density consistent with the rasters' velocity: rho = 1 + 3 p, with the
+    kinematic pressure p of the incompressible momentum balance,
+        lap p = -d_i d_j (u_i u_j)
+    on the raster (central differences, pixel spacing in coarse cells, solid
+    pixels taken as fluid at rest), p = 0 at the outlet column (the outlet
+    holds rho = 1), dp/dx = nu u''(y) = -8 nu Umax / (LY - 2)^2 at the inlet
+    (the parabolic profile) and dp/dy = 0 on the walls. A density that does
+    not balance the velocity launches pressure waves through the channel that
+    viscosity damps only slowly. Returns a copy of fields with channel 0
+    replaced."""
+    out = np.array(fields, np.float64)
+    n, H, W, _ = out.shape
+    for c, sp in enumerate(specs):
+        dx = (sp.LX - 1.0) / (W - 1)
+        dy = (sp.LY - 1.0) / (H - 1)
+        Y = np.arange(H)[:, None] * dy
+        X = np.arange(W)[None, :] * dx
+        solid = (X - sp.CX) ** 2 + (Y - sp.CY) ** 2 < sp.R ** 2
+        u = np.where(solid, 0.0, out[c, :, :, 1])
+        v = np.where(solid, 0.0, out[c, :, :, 2])
+        uu, uv, vv = u * u, u * v, v * v
+        src = -(np.gradient(np.gradient(uu, dx, axis=1), dx, axis=1)
+                + 2 * np.gradient(np.gradient(uv, dx, axis=1), dy, axis=0)
+                + np.gradient(np.gradient(vv, dy, axis=0), dy, axis=0))
+        nu = sp.U * 2 * sp.R / sp.Re
+        g_in = -8.0 * nu * sp.U / (sp.LY - 2.0) ** 2
+        # unknowns: every pixel left of the outlet column
+        Wn = W - 1
+        idx = np.arange(H * Wn).reshape(H, Wn)
+        rows, cols, vals = [], [], []
+        rhs = src[:, :Wn].copy()
+        ax, ay = 1.0 / dx ** 2, 1.0 / dy ** 2
+        for j in range(H):
+            for i in range(Wn):
+                r = idx[j, i]
+                rows.append(r)
+                cols.append(r)
+                vals.append(-2 * ax - 2 * ay)
+                # x neighbours: ghost p[-1] = p[1] - 2 dx g_in at the inlet; p = 0 at the outlet column
+                for di in (-1, 1):
+                    ii = i + di
+                    if ii < 0:
+                        rows.append(r)
+                        cols.append(idx[j, 1])
+                        vals.append(ax)
+                        rhs[j, i] += 2 * dx * g_in * ax
+                    elif ii < Wn:
+                        rows.append(r)
+                        cols.append(idx[j, ii])
+                        vals.append(ax)
+                # y neighbours: mirror at the walls
+                for dj in (-1, 1):
+                    jj = j + dj
+                    if jj < 0 or jj >= H:
+                        jj = j - dj
+                    rows.append(r)
+                    cols.append(idx[jj, i])
+                    vals.append(ay)
+        A = coo_matrix((vals, (rows, cols)), shape=(H * Wn, H * Wn)).tocsc()
+        pr = splu(A).solve(rhs.ravel()).reshape(H, Wn)
+        out[c, :, :Wn, 0] = 1.0 + 3.0 * pr
+        out[c, :, Wn, 0] = 1.0
+    return out.astype(np.float32)
+
+
+def crossing_phase(cl, idx):
+    """Shedding phase in [0, 2 pi) at sample idx: linear in time between the
+    upward zero crossings of cl - mean(cl) that bracket idx; nan without both."""
+    x = np.asarray(cl, np.float64) - np.mean(cl)
+    s = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+    tc = s + (-x[s]) / (x[s + 1] - x[s])
+    k = np.searchsorted(tc, idx, side="right") - 1
+    if k < 0 or k + 1 >= tc.size:
+        return float("nan")
+    return float(2 * np.pi * (idx - tc[k]) / (tc[k + 1] - tc[k]))
 
 
 def shedding_period(cl):
@@ -149,69 +233,10 @@ def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None):
     return cd, cl, wall, (shots, shot_t), nu
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=128)
-    ap.add_argument("--test", type=int, default=64)
-    ap.add_argument("--spin", type=int, default=40000)
-    ap.add_argument("--nsnap", type=int, default=32)
-    ap.add_argument("--every", type=int, default=50)
-    ap.add_argument("--steps", type=int, default=60000)
-    ap.add_argument("--H", type=int, default=32)
-    ap.add_argument("--W", type=int, default=128)
-    ap.add_argument("--base", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16s"])
-    ap.add_argument("--save", default=None, help="write the series and the trained parameters to this .npz")
-    args = ap.parse_args(argv)
-    lib = load_library(False)
-    storage = abi.STORE_NATIVE if args.storage == "f32" else abi.STORE_F16S
-    H, W = args.H, args.W
-    out = {"train_cases": args.train, "test_cases": args.test, "raster": [H, W], "storage": args.storage}
-
-    # 1. snapshots of the training cases
-    tr = c5_specs(args.train, seed=101)
-    steps_tr = args.spin + args.nsnap * args.every
-    _, cl_tr, wall_tr, (shots, shot_t), nu_tr = run(lib, tr, steps_tr, storage,
-                                                    snap=(args.spin + args.every, args.every, args.nsnap, H, W),
-                                                    log="train cases")
-    out["snapshot_run_s"] = round(wall_tr, 2)
-    out["snapshot_run_mlups"] = round(nu_tr * steps_tr / wall_tr / 1e6, 1)
-    xs, ts = [], []
-    for c, sp in enumerate(tr):
-        x = cl_tr[c, args.spin // 2:]
-        ph = np.angle(hilbert(x - x.mean()))
-        for k, t in enumerate(shot_t):
-            phase = ph[t - 1 - args.spin // 2]
-            xs.append(raster_inputs([sp], H, W, phase)[0])
-            ts.append(to_target(shots[k][c:c + 1], [sp])[0])
-    X = np.stack(xs)
-    T = np.stack(ts).astype(np.float32)
-    ok = np.isfinite(T).all(axis=(1, 2, 3))
-    X, T = X[ok], T[ok]
-    out["samples"] = int(X.shape[0])
-
-    # 2. training
-    net = Net(lib, H, W, IN_CH, OUT_CH, base=args.base, max_batch=args.batch, seed=2024)
-    t0 = time.perf_counter()
-    hist = net.train(X, T, batch=args.batch, epochs=args.epochs, lr=args.lr, seed=99)
-    out["train_s"] = round(time.perf_counter() - t0, 2)
-    out["epoch_loss_first_last"] = [float(hist[0]), float(hist[-1])]
-    print(f"# trained: loss {hist[0]:.4e} -> {hist[-1]:.4e} in {out['train_s']} s", file=sys.stderr, flush=True)
-
-    # 3. held-out cases, cold and warm
-    te = c5_specs(args.test, seed=202)
-    cd_c, cl_c, wall_c, _, nu = run(lib, te, args.steps, storage, log="cold")
-    t0 = time.perf_counter()
-    pred = net.forward(raster_inputs(te, H, W, 0.0))
-    init = from_target(pred, te).astype(np.float32)
-    t_inf = time.perf_counter() - t0
-    cd_w, cl_w, wall_w, _, _ = run(lib, te, args.steps, storage, init=init, log="warm")
-    ms_step = 0.5 * (wall_c + wall_w) / args.steps * 1e3
+def convergence(cl_c, cl_w, cd_w):
+    """Per case: steps to a converged Cl amplitude, cold and warm, and the fallback count."""
     tc, tw, fallback = [], [], 0
-    for c in range(len(te)):
+    for c in range(cl_c.shape[0]):
         T_c = shedding_period(cl_c[c])
         if T_c < 8:
             tc.append(-1)
@@ -227,11 +252,14 @@ def main(argv=None):
             t_warm = time_to_amplitude(cl_w[c], T_c, a_ref)
         tc.append(t_cold)
         tw.append(t_warm)
-    tc, tw = np.array(tc), np.array(tw)
+    return np.array(tc), np.array(tw), fallback
+
+
+def summary(out, tc, tw, fallback, ms_step, t_inf):
     both = (tc > 0) & (tw > 0)
     out.update({
-        "steps_run": args.steps, "ms_per_step": round(ms_step, 4), "mlups": round(nu * args.steps / wall_c / 1e6, 1),
-        "converged_cold": int((tc > 0).sum()), "converged_warm": int((tw > 0).sum()), "fallbacks": fallback,
+        "ms_per_step": round(ms_step, 4), "converged_cold": int((tc > 0).sum()), "converged_warm": int((tw > 0).sum()),
+        "fallbacks": fallback,
         "median_steps_cold": float(np.median(tc[both])) if both.any() else None,
         "median_steps_warm": float(np.median(tw[both])) if both.any() else None,
         "inference_s": round(t_inf, 3),
@@ -244,9 +272,139 @@ def main(argv=None):
         out["time_to_shedding_cold_s"] = round(wall_cold, 3)
         out["time_to_shedding_warm_s"] = round(wall_warm, 3)
         out["speedup_wall"] = round(wall_cold / wall_warm, 3)
+    return out
+
+
+def self_init(lib, args, storage, out):
+    te = c5_specs(args.test, seed=202)
+    cd_c, cl_c, wall_c, (shots, _), _ = run(lib, te, args.steps, storage,
+                                             snap=(args.steps, 1, 1, args.H, args.W), log="cold")
+    if args.dump:
+        np.savez(args.dump, fields=shots[0], specs=np.array([[sp.LX, sp.LY, sp.R, sp.CX, sp.CY, sp.Re, sp.U]
+                                                             for sp in te]),
+                 cl=cl_c[:, -4000:].astype(np.float32), cd=cd_c[:, -4000:].astype(np.float32))
+    init = shots[0].copy()
+    if args.self_init_rho == "one":
+        init[..., 0] = 1.0
+    elif args.self_init_rho == "poisson":
+        init = poisson_density(init, te)
+    out["self_init_rho"] = args.self_init_rho
+    cd_w, cl_w, wall_w, _, _ = run(lib, te, args.steps, storage, init=init, log="self-init")
+    tc, tw, fb = convergence(cl_c, cl_w, cd_w)
+    out["mode"] = "self-init"
+    out["steps_run"] = args.steps
+    summary(out, tc, tw, fb, 0.5 * (wall_c + wall_w) / args.steps * 1e3, 0.0)
     if args.save:
-        np.savez(args.save, tc=tc, tw=tw, cl_cold=cl_c.astype(np.float32), cl_warm=cl_w.astype(np.float32),
-                 loss=hist, params=net.get_params())
+        np.savez(args.save, tc=tc, tw=tw, cl_cold=cl_c[:, ::10].astype(np.float32),
+                 cl_warm=cl_w[:, ::10].astype(np.float32))
+    print(json.dumps(out), flush=True)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=256)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--spin", type=int, default=40000)
+    ap.add_argument("--nsnap", type=int, default=32)
+    ap.add_argument("--every", type=int, default=50)
+    ap.add_argument("--tail", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=60000)
+    ap.add_argument("--H", type=int, default=48)
+    ap.add_argument("--W", type=int, default=192)
+    ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16s"])
+    ap.add_argument("--save", default=None, help="write the series and the trained parameters to this .npz")
+    ap.add_argument("--self-init", action="store_true",
+                    help="no network: each held-out case restarts from the raster of its own final cold state "
+                         "(the best any network can do at this raster)")
+    ap.add_argument("--self-init-rho", default="raster", choices=["raster", "one", "poisson"],
+                    help="with --self-init: keep the raster's density, start from rho = 1 or solve for it")
+    ap.add_argument("--dump", default=None, help="with --self-init: save the final rasters to this .npz")
+    ap.add_argument("--rho", default="net", choices=["net", "poisson"],
+                    help="warm-start density: the network's or the pressure Poisson solution of its velocity")
+    args = ap.parse_args(argv)
+    lib = load_library(False)
+    storage = abi.STORE_NATIVE if args.storage == "f32" else abi.STORE_F16S
+    H, W = args.H, args.W
+    out = {"train_cases": args.train, "test_cases": args.test, "raster": [H, W], "storage": args.storage}
+    if args.self_init:
+        return self_init(lib, args, storage, out)
+
+    # 1. snapshots of the training cases
+    tr = c5_specs(args.train, seed=101)
+    # the run goes on `tail` steps past the last snapshot so that a later zero crossing brackets it
+    steps_tr = args.spin + args.nsnap * args.every + args.tail
+    _, cl_tr, wall_tr, (shots, shot_t), nu_tr = run(lib, tr, steps_tr, storage,
+                                                    snap=(args.spin + args.every, args.every, args.nsnap, H, W),
+                                                    log="train cases")
+    out["snapshot_run_s"] = round(wall_tr, 2)
+    out["snapshot_run_mlups"] = round(nu_tr * steps_tr / wall_tr / 1e6, 1)
+    xs, ts = [], []
+    for c, sp in enumerate(tr):
+        x = cl_tr[c, args.spin:]
+        for k, t in enumerate(shot_t):
+            phase = crossing_phase(x, t - 1 - args.spin)
+            if not np.isfinite(phase):
+                continue
+            xs.append(raster_inputs([sp], H, W, phase)[0])
+            ts.append(to_target(shots[k][c:c + 1], [sp])[0])
+    X = np.stack(xs)
+    T = np.stack(ts).astype(np.float32)
+    ok = np.isfinite(T).all(axis=(1, 2, 3))
+    X, T = X[ok], T[ok]
+    out["samples"] = int(X.shape[0])
+
+    # 2. training: the last `val` cases' snapshots are held out for a validation loss;
+    # the learning rate steps down (lr, lr/3, lr/10 over 50%, 30%, 20% of the epochs)
+    nval = max(1, args.train // 8) * args.nsnap
+    Xt, Tt, Xv, Tv = X[:-nval], T[:-nval], X[-nval:], T[-nval:]
+    net = Net(lib, H, W, IN_CH, OUT_CH, base=args.base, max_batch=args.batch, seed=2024)
+    t0 = time.perf_counter()
+    hist = []
+    e1 = max(1, int(0.5 * args.epochs))
+    e2 = max(1, int(0.3 * args.epochs))
+    for ep, lr in ((e1, args.lr), (e2, args.lr / 3), (max(1, args.epochs - e1 - e2), args.lr / 10)):
+        for _ in range(ep):  # one call per epoch: Adam state carries over, progress is printed
+            hist.extend(net.train(Xt, Tt, batch=args.batch, epochs=1, lr=lr, seed=99 + len(hist)))
+            if len(hist) % 5 == 0:
+                print(f"# epoch {len(hist)} loss {hist[-1]:.4e} {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
+    hist = np.array(hist)
+    out["train_s"] = round(time.perf_counter() - t0, 2)
+    out["epochs"] = int(hist.size)
+    out["epoch_loss_first_last"] = [float(hist[0]), float(hist[-1])]
+    out["val_loss"] = float(np.mean((net.forward(Xv) - Tv) ** 2))
+    out["val_loss_per_channel"] = [float(v) for v in np.mean((net.forward(Xv) - Tv) ** 2, axis=(0, 1, 2))]
+    out["target_var_per_channel"] = [float(v) for v in np.var(Tv, axis=(0, 1, 2))]
+    # validation error along the channel (8 bins in x), per channel, relative to the target variance
+    ev = np.mean((net.forward(Xv) - Tv) ** 2, axis=(0, 1)).reshape(8, -1, OUT_CH).mean(axis=1)
+    out["val_rel_err_along_x"] = np.round(ev / np.var(Tv, axis=(0, 1, 2)), 4).T.tolist()
+    print(f"# trained: loss {hist[0]:.4e} -> {hist[-1]:.4e} in {out['train_s']} s, val {out['val_loss']:.4e}",
+          file=sys.stderr, flush=True)
+
+    # 3. held-out cases, cold and warm
+    te = c5_specs(args.test, seed=202)
+    cd_c, cl_c, wall_c, _, nu = run(lib, te, args.steps, storage, log="cold")
+    t0 = time.perf_counter()
+    pred = net.forward(raster_inputs(te, H, W, 0.0))
+    init = from_target(pred, te).astype(np.float32)
+    if args.rho == "poisson":
+        init = poisson_density(init, te)
+    out["rho"] = args.rho
+    t_inf = time.perf_counter() - t0
+    cd_w, cl_w, wall_w, _, _ = run(lib, te, args.steps, storage, init=init, log="warm")
+    ms_step = 0.5 * (wall_c + wall_w) / args.steps * 1e3
+    tc, tw, fallback = convergence(cl_c, cl_w, cd_w)
+    out["steps_run"] = args.steps
+    out["mlups"] = round(nu * args.steps / wall_c / 1e6, 1)
+    summary(out, tc, tw, fallback, ms_step, t_inf)
+    if args.save:
+        np.savez(args.save, tc=tc, tw=tw, cl_cold=cl_c[:, ::10].astype(np.float32),
+                 cl_warm=cl_w[:, ::10].astype(np.float32), loss=hist, params=net.get_params())
     net.close()
     print(json.dumps(out), flush=True)
 

@@ -21,6 +21,22 @@ def bf16_round(t):
     return t + (r - t).detach()
 
 
+class _GradRound(torch.autograd.Function):
+    """Identity forward; the incoming gradient rounded to BF16 backward."""
+
+    @staticmethod
+    def forward(ctx, t):
+        return t.view_as(t)
+
+    @staticmethod
+    def backward(ctx, g):
+        return g.to(torch.float32).to(torch.bfloat16).to(g.dtype)
+
+
+def grad_round(t):
+    return _GradRound.apply(t)
+
+
 def make_params(layers, requires_grad=True):
     """{name: (W, b)} numpy -> {name: (W, b)} float64 torch leaves."""
     out = {}
@@ -30,36 +46,43 @@ def make_params(layers, requires_grad=True):
     return out
 
 
-def forward(params, x, bf16=False):
-    """x: [N, H, W, in_ch] -> [N, H, W, out_ch] (NHWC in and out)."""
+def forward(params, x, bf16=False, grad_bf16=False):
+    """x: [N, H, W, in_ch] -> [N, H, W, out_ch] (NHWC in and out).
+
+    grad_bf16 also rounds gradients to BF16 where the GPU stores them: the
+    gradient of the network output, of every convolution's input (the GPU
+    applies the ReLU mask in the same pass), of each upsampled tensor after
+    the sum over its four children, and of each encoder output after the skip
+    and pooling parts are added."""
     rnd = bf16_round if bf16 else (lambda t: t)
+    grnd = grad_round if grad_bf16 else (lambda t: t)
     h = x.permute(0, 3, 1, 2)
     h = rnd(h)
 
     def conv(h, name, relu=True):
         w, b = params[name]
-        y = F.conv2d(h, rnd(w), b, padding=w.shape[-1] // 2)
+        y = F.conv2d(grnd(h), rnd(w), b, padding=w.shape[-1] // 2)
         return rnd(F.relu(y)) if relu else y
 
     skips = []
     for lv in range(4):
         h = conv(h, f"e{lv}a")
-        h = conv(h, f"e{lv}b")
+        h = grnd(conv(h, f"e{lv}b"))
         if lv < 3:
             skips.append(h)
             h = F.max_pool2d(h, 2)
     for lv in (2, 1, 0):
-        h = torch.cat([F.interpolate(h, scale_factor=2, mode="nearest"), skips[lv]], dim=1)
+        h = torch.cat([F.interpolate(grnd(h), scale_factor=2, mode="nearest"), skips[lv]], dim=1)
         h = conv(h, f"d{lv}a")
         h = conv(h, f"d{lv}b")
-    y = conv(h, "head", relu=False)
+    y = grnd(conv(h, "head", relu=False))
     return y.permute(0, 2, 3, 1)
 
 
-def loss_and_grads(layers, x, t, bf16=False):
+def loss_and_grads(layers, x, t, bf16=False, grad_bf16=False):
     """Mean squared error and {name: (dW, db)} as numpy float64."""
     p = make_params(layers)
-    y = forward(p, torch.tensor(x, dtype=torch.float64), bf16=bf16)
+    y = forward(p, torch.tensor(x, dtype=torch.float64), bf16=bf16, grad_bf16=grad_bf16)
     loss = ((y - torch.tensor(t, dtype=torch.float64)) ** 2).mean()
     loss.backward()
     return float(loss.detach()), {k: (w.grad.numpy(), b.grad.numpy()) for k, (w, b) in p.items()}

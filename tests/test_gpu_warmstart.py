@@ -6,9 +6,10 @@ GPU stores BF16), its training, and the raster exchange of batched cases
 Tolerances: the GPU accumulates in FP32 in another order than the float64
 reference, so a value near a BF16 rounding boundary can round the other way
 (one BF16 ulp, 2^-8 relative) and carry that into later layers; gradients are
-also stored in BF16 between layers on the GPU only. Forward outputs agree to
-2e-2 relative L2 and gradients to 6e-2 per layer (cosine >= 0.995); a
-transposed, flipped or mis-offset operand gives O(1) errors.
+stored in BF16 between layers (the reference rounds them at the same places).
+Forward outputs agree to 2e-2 relative L2; gradients are compared within the
+reference's own sensitivity; a transposed, flipped or mis-offset operand
+gives O(1) errors.
 """
 import os
 import sys
@@ -59,22 +60,43 @@ def test_net_forward_matches_reference(lib32):
 
 
 def test_net_gradients_match_reference(lib32):
+    """GPU gradients against the reference. The gradient of a BF16 network is
+    discontinuous: 2x2 max-pool windows hold near-ties that one BF16 ulp
+    flips, moving the gradient of that window to another pixel. The
+    tolerance per layer is therefore 1.5 times the reference's own change
+    under a 1e-4 relative perturbation of the input (and at least 2e-2), with
+    a cosine of at least 0.995. (A finite-difference slope of the GPU loss
+    cannot be tight here: below a BF16 ulp of the weights the step is
+    quantised away, above it ReLU kinks and max-pool switches dominate. The
+    reference's own gradients are checked against finite differences in
+    float64 on the CPU.)"""
     net = Net(lib32, H, W, IN, OUT, base=BASE, max_batch=4, seed=12)
     x, t = sample(2, 2)
     loss, g = net.loss_grad(x, t)
     layers = unpack(net.get_params(), IN, OUT, BASE)
     gl = unpack(g, IN, OUT, BASE)
-    loss_ref, g_ref = ref.loss_and_grads(layers, x.astype(np.float64), t.astype(np.float64), bf16=True)
+    x64, t64 = x.astype(np.float64), t.astype(np.float64)
+    loss_ref, g_ref = ref.loss_and_grads(layers, x64, t64, bf16=True, grad_bf16=True)
+    rng = np.random.default_rng(21)
+    spread = {k: 0.0 for k in ref.ORDER}
+    for _ in range(2):
+        _, gp = ref.loss_and_grads(layers, x64 * (1 + 1e-4 * rng.normal(size=x.shape)), t64, bf16=True,
+                                   grad_bf16=True)
+        for k in ref.ORDER:
+            spread[k] = max(spread[k], rel(gp[k][0], g_ref[k][0]))
     print(f"loss gpu {loss:.6e} ref {loss_ref:.6e}", flush=True)
-    assert abs(loss - loss_ref) <= 2e-2 * loss_ref
-    worst = 0.0
+    fails = []
     for name in ref.ORDER:
         for part in (0, 1):
             a, b = gl[name][part], g_ref[name][part]
             e, c = rel(a, b), cosine(a, b)
-            print(f"  {name} {'W' if part == 0 else 'b'}: rel {e:.3e} cos {c:.6f}", flush=True)
-            worst = max(worst, e)
-            assert e <= 6e-2 and c >= 0.995, (name, part, e, c)
+            tol = max(1.5 * spread[name], 2e-2)
+            print(f"  {name} {'W' if part == 0 else 'b'}: rel {e:.3e} cos {c:.6f} (reference spread "
+                  f"{spread[name]:.3e}, tol {tol:.3e})", flush=True)
+            if not (e <= tol and c >= 0.995):
+                fails.append((name, part, e, c))
+    assert abs(loss - loss_ref) <= 2e-2 * loss_ref
+    assert not fails, fails
     # padding weights of the input and head layers get no gradient
     table, _ = mbl.net.layer_table(IN, OUT, BASE)
     name, cin, cr, cout, cor, k, woff, boff = table[0]

@@ -31,6 +31,17 @@ def test_time_to_amplitude_known_envelope():
     assert ws.time_to_amplitude(np.sin(2 * np.pi * t / T), T, 1.0) == T
 
 
+def test_crossing_phase_of_a_sinusoid():
+    T = 97.0
+    t = np.arange(3000)
+    cl = 0.2 + np.sin(2 * np.pi * t / T + 0.3)
+    for idx in (400, 1234, 2500):
+        want = (2 * np.pi * idx / T + 0.3) % (2 * np.pi)
+        got = ws.crossing_phase(cl, idx)
+        assert abs(np.angle(np.exp(1j * (got - want)))) < 0.02
+    assert np.isnan(ws.crossing_phase(cl, 10)) and np.isnan(ws.crossing_phase(cl, 2990))
+
+
 def test_shedding_period():
     t = np.arange(10000)
     assert ws.shedding_period(np.sin(2 * np.pi * t / 173.0)) in (172, 173, 174)
@@ -54,3 +65,64 @@ def test_inputs_and_targets():
     f = np.stack([1 + rng.normal(0, 1e-3, (32, 128)), rng.normal(0, 0.05, (32, 128)),
                   rng.normal(0, 0.05, (32, 128))], -1)[None].repeat(4, 0)
     assert np.allclose(ws.from_target(ws.to_target(f, specs), specs), f, rtol=1e-12, atol=1e-14)
+
+
+def test_chapman_enskog_state_of_a_relaxed_shear_wave():
+    """The non-equilibrium part that mbl_batch_init_from_raster writes,
+    -(tau - 1) w_k rho / cs^2 (e_k e_k - cs^2 I) : grad u, is what a BGK run
+    with pull streaming and post-collision storage settles into. A periodic
+    shear wave ux = A sin(k y) runs from F = feq for 300 steps (the initial
+    layer decays within a few tau) in plain numpy; the stored F minus feq of
+    its own moments then matches the formula for the diagonal directions to
+    O((k)^2). A factor tau instead of tau - 1, or the opposite sign, fails."""
+    N, tau, A = 64, 0.8, 0.01
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+    ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    kw = 2 * np.pi / N
+    y = np.arange(N)[:, None] * np.ones((1, N))
+
+    def feq(rho, ux, uy):
+        eu = ex[:, None, None] * ux + ey[:, None, None] * uy
+        return w[:, None, None] * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (ux ** 2 + uy ** 2))
+
+    F = feq(np.ones((N, N)), A * np.sin(kw * y), np.zeros((N, N)))
+    for _ in range(300):
+        f = np.stack([np.roll(np.roll(F[k], ey[k], axis=0), ex[k], axis=1) for k in range(9)])  # pull
+        rho = f.sum(0)
+        ux = (f * ex[:, None, None]).sum(0) / rho
+        uy = (f * ey[:, None, None]).sum(0) / rho
+        F = f - (f - feq(rho, ux, uy)) / tau
+    rho = F.sum(0)
+    ux = (F * ex[:, None, None]).sum(0) / rho
+    uy = (F * ey[:, None, None]).sum(0) / rho
+    neq = F - feq(rho, ux, uy)
+    # d ux / dy from the wave's current amplitude and phase (spectral)
+    c = np.fft.rfft(ux[:, 0])[1] * 2 / N
+    dux_dy = np.real(1j * kw * c * np.exp(1j * kw * y))
+    for k in (5, 6, 7, 8):
+        qg = ex[k] * ey[k] * dux_dy  # e_a e_b d_a u_b - cs^2 div u, div u = 0
+        want = -(tau - 1) * w[k] * rho * 3 * qg
+        ratio = np.sum(neq[k] * want) / np.sum(want * want)
+        assert abs(ratio - 1) < 0.02, (k, ratio)
+
+
+def test_poisson_density_of_plane_poiseuille_flow():
+    """A parabolic channel profile without a body: lap p = 0 inside, the
+    inlet gradient nu u'' = -8 nu Umax / (LY - 2)^2, p = 0 at the outlet, so the
+    density falls linearly from 1 + 24 nu Umax (LX - 1) / (LY - 2)^2 to 1."""
+    specs = ws.c5_specs(2, seed=1)
+    H, W = 24, 96
+    f = np.zeros((2, H, W, 3), np.float32)
+    for c, sp in enumerate(specs):
+        Y = np.arange(H)[:, None] * (sp.LY - 1.0) / (H - 1)
+        f[c, :, :, 1] = np.clip(4 * sp.U * (sp.LY - 1.5 - Y) * (Y - 0.5) / (sp.LY - 2.0) ** 2, 0, None)
+        sp.CX = 1e6  # the body out of the channel
+    g = ws.poisson_density(f, specs)
+    for c, sp in enumerate(specs):
+        nu = sp.U * 2 * sp.R / sp.Re
+        drop = 24 * nu * sp.U / (sp.LY - 2.0) ** 2 * (sp.LX - 1)
+        X = np.arange(W) * (sp.LX - 1.0) / (W - 1)
+        want = 1 + drop * (1 - X / (sp.LX - 1.0))
+        assert np.allclose(g[c, :, :, 0], want[None, :], rtol=0, atol=1e-6 * drop + 1e-7)
+        assert np.array_equal(g[c, :, :, 1:], f[c, :, :, 1:])
