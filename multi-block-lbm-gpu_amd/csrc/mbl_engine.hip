@@ -466,6 +466,7 @@ static BlockDesc make_desc(const mbl_config &cfg, const AREA &a, const D2Q9 &pp,
 {
     BlockDesc d;
     memset(&d, 0, sizeof(d));
+    d.tw = 64;
     d.bc = make_consts(cfg);
     for (int k = 0; k < 9; k++) {
         d.P[0][k] = (void *)pp.F[k];
@@ -724,9 +725,16 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
                 if (!A.pps[a].F[k] || !A.pps[a].f[k] || A.pps[a].F[k] == A.pps[a].f[k]) return fail("F/f planes");
             BlockDesc d = make_desc(cfg, ar, A.pps[a], l, nullptr, nullptr);
             if (A.bc) d.bc = A.bc[a];
-            d.tiles_x = (ar.nx + 64 * vx - 1) / (64 * vx);
+            // narrow blocks at one node per thread: 32-lane row segments when they waste fewer lanes
+            d.tw = 64;
+            if (vx == 1) {
+                const int w64 = (ar.nx + 63) / 64 * 64 - ar.nx, w32 = (ar.nx + 31) / 32 * 32 - ar.nx;
+                if (w32 < w64) d.tw = 32;
+            }
+            d.trows = rows * (64 / d.tw);
+            d.tiles_x = (ar.nx + d.tw * vx - 1) / (d.tw * vx);
             d.tile_begin = tiles;
-            tiles += d.tiles_x * ((ar.ny + rows - 1) / rows);
+            tiles += d.tiles_x * ((ar.ny + d.trows - 1) / d.trows);
             tile_blk.resize(L->tile_off[l] + tiles, a - L->ASP[l]);
             L->hb[a] = d;
             if (validate_flags(h, ar, A.pps[a].solid, L->periodic, nullptr)) {
@@ -1232,6 +1240,8 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
         while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
         S.tpb = h->tpb;
         S.rows = S.tpb / 64;
+        S.hd.tw = 64;
+        S.hd.trows = 0;
         S.hd.tiles_x = (area->nx + 64 * S.vx - 1) / (64 * S.vx);
         S.hd.tile_begin = 0;
         S.tiles = S.hd.tiles_x * ((area->ny + S.rows - 1) / S.rows);

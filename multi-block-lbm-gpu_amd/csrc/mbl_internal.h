@@ -49,6 +49,8 @@ struct BlockDesc {
     int level;
     int m;
     int tiles_x;          // stream-collide tiles along x
+    int tw;               // tile width in lanes (64, or 32: a wave then covers two rows)
+    int trows;            // rows per tile (0: the launch's tile_rows)
     int tile_begin;       // first stream-collide tile of the block inside its launch
     int rtile_begin;      // first residual tile of the block inside the residual launch
     int rtiles_x;

@@ -502,8 +502,8 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
     }
 }
 
-// One tile: 64*VX columns x tile_rows rows of one block; the workgroup's
-// waves take the rows in turn. The block comes from the per-tile table when
+// One tile: tw*VX columns x rows rows of one block; each wave takes 64/tw
+// rows at a time, the workgroup's waves the next ones in turn. The block comes from the per-tile table when
 // there is one (one scalar load), else from a search of tile_begin.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
 __device__ inline void sc_tile(const BlockDesc *__restrict__ descs, const int *__restrict__ tile_blk, int ndesc,
@@ -526,10 +526,14 @@ __device__ inline void sc_tile(const BlockDesc *__restrict__ descs, const int *_
     const int lt = t - d.tile_begin;
     const int tyi = lt / d.tiles_x;
     const int txi = lt - tyi * d.tiles_x;
-    const int x0 = (txi * 64 + (int)(threadIdx.x & 63)) * VX;
-    const int waves = (int)(blockDim.x >> 6);
-    for (int r = (int)(threadIdx.x >> 6); r < tile_rows; r += waves)
-        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, tyi * tile_rows + r, x0);
+    // a wave covers 64 / tw row segments of tw consecutive lanes
+    const int tw = d.tw, lane = (int)(threadIdx.x & 63);
+    const int per_wave = 64 / tw, sub = lane / tw, col = lane - sub * tw;
+    const int x0 = (txi * tw + col) * VX;
+    const int rows = d.trows > 0 ? d.trows : tile_rows;
+    const int step = (int)(blockDim.x >> 6) * per_wave;
+    for (int r = (int)(threadIdx.x >> 6) * per_wave + sub; r < rows; r += step)
+        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, tyi * rows + r, x0);
 }
 
 // One tile per workgroup when the grid covers every tile; otherwise a

@@ -5,8 +5,9 @@
 1. Training cases (two-level channels, D in {16, 24} finest cells, blockage
    D/H ~ U[0.1, 0.3], Re ~ U[80, 200], the cylinder 0.25 coarse cells above
    the channel centre line) start at rest with the parabolic profile and run
-   `spin` steps, then `nsnap` snapshots of (rho, ux, uy) are rasterised onto an
-   H x W grid of each channel, every `every` steps.
+   `spin` steps with shifted FP16 planes, then `nsnap` snapshots of
+   (rho, ux, uy) are rasterised onto an H x W grid of each channel, every
+   `every` steps.
 2. Inputs per pixel: solid mask, Re/200, D/H, sin and cos of the shedding
    phase (linear in time between the upward zero crossings of Cl that
    bracket the snapshot), x and y; targets: (0.25 (rho - 1)/U^2, ux/U, uy/U).
@@ -316,7 +317,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16s"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16s"], help="planes of the held-out runs")
+    ap.add_argument("--snap-storage", default="f16s", choices=["f32", "f16s"], help="planes of the snapshot runs")
     ap.add_argument("--save", default=None, help="write the series and the trained parameters to this .npz")
     ap.add_argument("--self-init", action="store_true",
                     help="no network: each held-out case restarts from the raster of its own final cold state "
@@ -329,8 +331,10 @@ def main(argv=None):
     args = ap.parse_args(argv)
     lib = load_library(False)
     storage = abi.STORE_NATIVE if args.storage == "f32" else abi.STORE_F16S
+    snap_storage = abi.STORE_NATIVE if args.snap_storage == "f32" else abi.STORE_F16S
     H, W = args.H, args.W
-    out = {"train_cases": args.train, "test_cases": args.test, "raster": [H, W], "storage": args.storage}
+    out = {"train_cases": args.train, "test_cases": args.test, "raster": [H, W], "storage": args.storage,
+           "snap_storage": args.snap_storage}
     if args.self_init:
         return self_init(lib, args, storage, out)
 
@@ -338,7 +342,7 @@ def main(argv=None):
     tr = c5_specs(args.train, seed=101)
     # the run goes on `tail` steps past the last snapshot so that a later zero crossing brackets it
     steps_tr = args.spin + args.nsnap * args.every + args.tail
-    _, cl_tr, wall_tr, (shots, shot_t), nu_tr = run(lib, tr, steps_tr, storage,
+    _, cl_tr, wall_tr, (shots, shot_t), nu_tr = run(lib, tr, steps_tr, snap_storage,
                                                     snap=(args.spin + args.every, args.every, args.nsnap, H, W),
                                                     log="train cases")
     out["snapshot_run_s"] = round(wall_tr, 2)
