@@ -204,6 +204,8 @@ typedef struct mbl_config {
     int flush_interval; /* records buffered on the device before a file flush (default 1000) */
     int write_files;    /* 1: append error_L2.dat / fdrag_t.dat in out_dir (default 1) */
     char out_dir[256];  /* directory of the .dat files ("" = current directory) */
+    /* block count read as a global by output/output.h:3 (0: the count of the last evolution()) */
+    int number_area;
 } mbl_config;
 
 /* Fills a config with the reference defaults (parameters.h, _GEO_2_5 levels). */
@@ -413,6 +415,33 @@ int mbl_batch_forces(mbl_batch_t b, int nt, double *cd, double *cl);
  * persistent groups) candidate and keeps the fastest; the trial steps advance
  * the cases. Returns 0 and the choice and its time per step. */
 int mbl_batch_autotune(mbl_batch_t b, int trial_steps, int *vx, int *tpb, int *groups, double *ms_per_step);
+
+/* ---- field output and checkpoints ---- */
+
+/* output/output.h:2-38: writes steady_<n>.dat in out_dir, a Tecplot point file
+ * with one zone per block ("zone_level<a>", I = nx, J = ny) of lines
+ * "x\ty\tsolid\trho\tU\tV" in the reference's "%g" format. Like the
+ * reference it also copies the device arrays into pps_h when pps_h and its
+ * pointers are non-NULL (solid, x, y, rho, ux, uy, forcex, forcey, and the f/F
+ * planes decoded to Lbd). The block count is cfg.number_area of the current
+ * handle, else the count of its last evolution() layout. */
+void output(Lbi n, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d);
+/* The same with an explicit handle and block count; returns 0 or an error code. */
+int mbl_output(mbl_handle_t h, Lbi n, Lbi number_area, const AREA *areas, D2Q9 *pps_h, const D2Q9 *pps_d);
+
+/* Checkpoint of the whole time-stepping state of a multi-block layout (no
+ * reference counterpart): per block the 9 F and 9 f planes in their storage
+ * encoding, rho/ux/uy and (when given) the previous velocities of error_L2;
+ * per coarse/fine line the three ring slots and the markers; and the caller's
+ * step counter. Load writes into the caller's existing allocations after
+ * checking block sizes, storage mode and element sizes, and restores the
+ * markers into line_t3; a run resumed from a checkpoint reproduces the
+ * uninterrupted run bit for bit. upsx/upsy may be NULL. */
+int mbl_checkpoint_save(mbl_handle_t h, const char *path, long long step, Lbi number_area, const AREA *areas,
+                        const D2Q9 *pps_d, Lbd *const *upsx, Lbd *const *upsy, Lbi number_line,
+                        const Ff_Store *line_t3);
+int mbl_checkpoint_load(mbl_handle_t h, const char *path, long long *step, Lbi number_area, const AREA *areas,
+                        D2Q9 *pps_d, Lbd **upsx, Lbd **upsy, Lbi number_line, Ff_Store *line_t3);
 
 /* ---- learned warm start (no reference counterpart) ----
  * A convolutional encoder-decoder over an H x W raster (H, W multiples of 8):

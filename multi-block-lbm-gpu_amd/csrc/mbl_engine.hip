@@ -1643,6 +1643,220 @@ extern "C" void mbl_diagnostics_clear(mbl_handle_t h)
 }
 
 // ---------------------------------------------------------------------------
+// field output (output/output.h:2-38) and checkpoints
+// ---------------------------------------------------------------------------
+static int d2h(mbl_context *h, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return MBL_E_HIP;
+    return 0;
+}
+
+static int h2d(mbl_context *h, void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return 0;
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return MBL_E_HIP;
+    return 0;
+}
+
+extern "C" int mbl_output(mbl_handle_t h, Lbi n, Lbi number_area, const AREA *areas, D2Q9 *pps_h, const D2Q9 *pps_d)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (number_area <= 0 || !areas || !pps_d) return set_err(h, MBL_E_INVALID);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    FILE *fo = fopen(out_path(h, ("steady_" + std::to_string(n) + ".dat").c_str()).c_str(), "w");
+    if (!fo) return set_err(h, MBL_E_INVALID);
+    fprintf(fo, "Title= \"LBM 2D\"\n");
+    fprintf(fo, "VARIABLES= \"X\",\"Y\",\"solid\",\"rho\",\"U\",\"V\",\n");
+    int rc = 0;
+    for (int a = 0; a < number_area && !rc; a++) {
+        const size_t N = (size_t)areas[a].nx * areas[a].ny;
+        std::vector<int> solid(N);
+        std::vector<Lbd> v[5];
+        const Lbd *src[5] = {pps_d[a].x, pps_d[a].y, pps_d[a].rho, pps_d[a].ux, pps_d[a].uy};
+        rc = d2h(h, solid.data(), pps_d[a].solid, N * sizeof(int));
+        for (int q = 0; q < 5 && !rc; q++) {
+            v[q].resize(N);
+            rc = d2h(h, v[q].data(), src[q], N * sizeof(Lbd));
+        }
+        if (rc) break;
+        // the reference's side effect: the host mirror of every array
+        if (pps_h) {
+            D2Q9 &ph = pps_h[a];
+            if (ph.solid) memcpy(ph.solid, solid.data(), N * sizeof(int));
+            Lbd *dst[5] = {ph.x, ph.y, ph.rho, ph.ux, ph.uy};
+            for (int q = 0; q < 5; q++)
+                if (dst[q]) memcpy(dst[q], v[q].data(), N * sizeof(Lbd));
+            if (ph.forcex && !rc) rc = d2h(h, ph.forcex, pps_d[a].forcex, N * sizeof(Lbd));
+            if (ph.forcey && !rc) rc = d2h(h, ph.forcey, pps_d[a].forcey, N * sizeof(Lbd));
+            std::vector<double> tmp(N);
+            for (int which = 0; which < 2 && !rc; which++)
+                for (int k = 0; k < 9 && !rc; k++) {
+                    Lbd *dp = which ? ph.f[k] : ph.F[k];
+                    if (!dp) continue;
+                    rc = plane_io(h, &pps_d[a], which, k, N, tmp.data(), true);
+                    for (size_t i = 0; i < N && !rc; i++) dp[i] = (Lbd)tmp[i];
+                }
+            if (rc) break;
+        }
+        fprintf(fo, "ZONE T= \"zone_level%d\",I= %d,J= %d,F=POINT\n", a, areas[a].nx, areas[a].ny);
+        for (size_t i = 0; i < N; i++)
+            fprintf(fo, "%g\t%g\t%d\t%g\t%g\t%g\n", (double)v[0][i], (double)v[1][i], solid[i], (double)v[2][i],
+                    (double)v[3][i], (double)v[4][i]);
+    }
+    if (fclose(fo) != 0 && !rc) rc = MBL_E_INVALID;
+    return set_err(h, rc);
+}
+
+extern "C" void output(Lbi n, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d)
+{
+    mbl_context *h = g_current;
+    if (!h) return;
+    int na = h->cfg.number_area;
+    if (na <= 0 && h->layout) na = h->layout->nblocks;
+    if (na <= 0) {
+        set_err(h, MBL_E_INVALID);
+        return;
+    }
+    mbl_output(h, n, na, areas, pps_h, pps_d);
+}
+
+// Checkpoint file: header, then per block F[9], f[9] (storage encoding), rho, ux, uy and
+// optionally upx, upy (Lbd), then per line the markers and 27 ring arrays (Lbd).
+struct CkptHeader {
+    char magic[8];     // "MBLCKPT1"
+    int32_t storage, elem_bytes, lbd_bytes, number_area, number_line, has_up;
+    int64_t step;
+};
+
+static int ckpt_write(FILE *f, const void *p, size_t bytes) { return fwrite(p, 1, bytes, f) == bytes ? 0 : MBL_E_INVALID; }
+static int ckpt_read(FILE *f, void *p, size_t bytes) { return fread(p, 1, bytes, f) == bytes ? 0 : MBL_E_INVALID; }
+
+extern "C" int mbl_checkpoint_save(mbl_handle_t h, const char *path, long long step, Lbi number_area,
+                                   const AREA *areas, const D2Q9 *pps_d, Lbd *const *upsx, Lbd *const *upsy,
+                                   Lbi number_line, const Ff_Store *line_t3)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!path || number_area <= 0 || !areas || !pps_d || number_line < 0 || (number_line > 0 && !line_t3))
+        return set_err(h, MBL_E_INVALID);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    FILE *f = fopen(path, "wb");
+    if (!f) return set_err(h, MBL_E_INVALID);
+    CkptHeader hd;
+    memset(&hd, 0, sizeof(hd));
+    memcpy(hd.magic, "MBLCKPT1", 8);
+    hd.storage = h->cfg.storage;
+    hd.elem_bytes = (int32_t)store_elem_bytes(storage_mode(h->cfg));
+    hd.lbd_bytes = (int32_t)sizeof(Lbd);
+    hd.number_area = number_area;
+    hd.number_line = number_line;
+    hd.has_up = (upsx && upsy) ? 1 : 0;
+    hd.step = step;
+    int rc = ckpt_write(f, &hd, sizeof(hd));
+    std::vector<char> buf;
+    for (int a = 0; a < number_area && !rc; a++) {
+        const size_t N = (size_t)areas[a].nx * areas[a].ny;
+        const int32_t dims[3] = {areas[a].m, areas[a].nx, areas[a].ny};
+        rc = ckpt_write(f, dims, sizeof(dims));
+        buf.resize(N * std::max<size_t>(hd.elem_bytes, sizeof(Lbd)));
+        for (int which = 0; which < 2 && !rc; which++)
+            for (int k = 0; k < 9 && !rc; k++) {
+                const void *src = which ? (const void *)pps_d[a].f[k] : (const void *)pps_d[a].F[k];
+                rc = d2h(h, buf.data(), src, N * hd.elem_bytes);
+                if (!rc) rc = ckpt_write(f, buf.data(), N * hd.elem_bytes);
+            }
+        const Lbd *mac[5] = {pps_d[a].rho, pps_d[a].ux, pps_d[a].uy, hd.has_up ? upsx[a] : nullptr,
+                             hd.has_up ? upsy[a] : nullptr};
+        for (int q = 0; q < (hd.has_up ? 5 : 3) && !rc; q++) {
+            rc = mac[q] ? d2h(h, buf.data(), mac[q], N * sizeof(Lbd)) : MBL_E_INVALID;
+            if (!rc) rc = ckpt_write(f, buf.data(), N * sizeof(Lbd));
+        }
+    }
+    for (int b = 0; b < number_line && !rc; b++) {
+        const Ff_Store &t = line_t3[b];
+        const int32_t hdr[4] = {t.N, t.marker[0], t.marker[1], t.marker[2]};
+        rc = ckpt_write(f, hdr, sizeof(hdr));
+        buf.resize((size_t)t.N * sizeof(Lbd));
+        const F_k3 *ft = &t.f_t1;
+        for (int sl = 0; sl < 3 && !rc; sl++)
+            for (int k = 0; k < 9 && !rc; k++) {
+                rc = d2h(h, buf.data(), ft[sl].F[k], (size_t)t.N * sizeof(Lbd));
+                if (!rc) rc = ckpt_write(f, buf.data(), (size_t)t.N * sizeof(Lbd));
+            }
+    }
+    if (!rc) rc = ckpt_write(f, "MBLEND\0\0", 8);
+    if (fclose(f) != 0 && !rc) rc = MBL_E_INVALID;
+    return set_err(h, rc);
+}
+
+extern "C" int mbl_checkpoint_load(mbl_handle_t h, const char *path, long long *step, Lbi number_area,
+                                   const AREA *areas, D2Q9 *pps_d, Lbd **upsx, Lbd **upsy, Lbi number_line,
+                                   Ff_Store *line_t3)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!path || number_area <= 0 || !areas || !pps_d || number_line < 0 || (number_line > 0 && !line_t3))
+        return set_err(h, MBL_E_INVALID);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    FILE *f = fopen(path, "rb");
+    if (!f) return set_err(h, MBL_E_INVALID);
+    CkptHeader hd;
+    int rc = ckpt_read(f, &hd, sizeof(hd));
+    if (!rc && (memcmp(hd.magic, "MBLCKPT1", 8) != 0 || hd.storage != h->cfg.storage ||
+                hd.elem_bytes != (int32_t)store_elem_bytes(storage_mode(h->cfg)) ||
+                hd.lbd_bytes != (int32_t)sizeof(Lbd) || hd.number_area != number_area ||
+                hd.number_line != number_line || (hd.has_up && !(upsx && upsy))))
+        rc = MBL_E_INVALID;
+    std::vector<char> buf;
+    for (int a = 0; a < number_area && !rc; a++) {
+        const size_t N = (size_t)areas[a].nx * areas[a].ny;
+        int32_t dims[3];
+        rc = ckpt_read(f, dims, sizeof(dims));
+        if (!rc && (dims[0] != areas[a].m || dims[1] != areas[a].nx || dims[2] != areas[a].ny)) rc = MBL_E_INVALID;
+        buf.resize(N * std::max<size_t>(hd.elem_bytes, sizeof(Lbd)));
+        for (int which = 0; which < 2 && !rc; which++)
+            for (int k = 0; k < 9 && !rc; k++) {
+                void *dst = which ? (void *)pps_d[a].f[k] : (void *)pps_d[a].F[k];
+                rc = ckpt_read(f, buf.data(), N * hd.elem_bytes);
+                if (!rc) rc = h2d(h, dst, buf.data(), N * hd.elem_bytes);
+            }
+        Lbd *mac[5] = {pps_d[a].rho, pps_d[a].ux, pps_d[a].uy, hd.has_up ? upsx[a] : nullptr,
+                       hd.has_up ? upsy[a] : nullptr};
+        for (int q = 0; q < (hd.has_up ? 5 : 3) && !rc; q++) {
+            rc = ckpt_read(f, buf.data(), N * sizeof(Lbd));
+            if (!rc) rc = mac[q] ? h2d(h, mac[q], buf.data(), N * sizeof(Lbd)) : MBL_E_INVALID;
+        }
+    }
+    for (int b = 0; b < number_line && !rc; b++) {
+        Ff_Store &t = line_t3[b];
+        int32_t hdr[4];
+        rc = ckpt_read(f, hdr, sizeof(hdr));
+        if (!rc && hdr[0] != t.N) rc = MBL_E_INVALID;
+        if (rc) break;
+        buf.resize((size_t)t.N * sizeof(Lbd));
+        F_k3 *ft = &t.f_t1;
+        for (int sl = 0; sl < 3 && !rc; sl++)
+            for (int k = 0; k < 9 && !rc; k++) {
+                rc = ckpt_read(f, buf.data(), (size_t)t.N * sizeof(Lbd));
+                if (!rc) rc = h2d(h, ft[sl].F[k], buf.data(), (size_t)t.N * sizeof(Lbd));
+            }
+        if (!rc)
+            for (int q = 0; q < 3; q++) t.marker[q] = hdr[1 + q];
+    }
+    char tail[8];
+    if (!rc && (ckpt_read(f, tail, 8) || memcmp(tail, "MBLEND\0\0", 8) != 0)) rc = MBL_E_INVALID;
+    fclose(f);
+    if (!rc && step) *step = hd.step;
+    return set_err(h, rc);
+}
+
+// ---------------------------------------------------------------------------
 // batched independent cases
 // ---------------------------------------------------------------------------
 struct mbl_batch {

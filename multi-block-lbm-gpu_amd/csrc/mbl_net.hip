@@ -308,7 +308,7 @@ static void conv_bwd(mbl_net *n, int li, int N, const bf16 *dZ, int ldd, const b
     wa.partial = n->partial;
     conv_wgrad(n->s, wa, nsplit);
     reduce_partials(n->s, n->partial, nsplit, l.nw(), n->g + l.woff, 0);
-    bias_grad(n->s, dZ, ldd, 0, P, l.cout, n->g + l.boff);
+    bias_grad(n->s, dZ, ldd, 0, P, l.cout, n->g + l.boff, n->partial);
     if (!dX) return;
     ConvArgs a;
     memset(&a, 0, sizeof(a));

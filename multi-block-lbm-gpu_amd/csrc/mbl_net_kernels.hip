@@ -95,6 +95,15 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
     }
 }
 
+// floor(v / d) for 0 <= v < 2^24 through a float reciprocal, corrected to the exact quotient
+__device__ inline int fdiv(int v, int d, float inv)
+{
+    int q = (int)((float)v * inv);
+    q -= (q * d > v);
+    q += ((q + 1) * d <= v);
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // Weight gradient: dW[co][tap][ci] += sum_p dY[p][co] * X[p + tap][ci] over one
 // pixel chunk per blockIdx.z, written as FP32 partials (reduced in order by
@@ -113,6 +122,8 @@ __global__ __launch_bounds__(64) void k_conv_wgrad(WgradArgs a)
     const int chunk = (P + gridDim.z - 1) / gridDim.z;
     const int pb = blockIdx.z * chunk, pe = min(P, pb + chunk);
     const int dy = tap / a.ksize - half, dx = tap % a.ksize - half;
+    const int HW = a.H * a.W;
+    const float invHW = 1.0f / (float)HW, invW = 1.0f / (float)a.W;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = pb; k0 < pe; k0 += 32) {
         __bf16 av[8], bv[8];
@@ -122,8 +133,9 @@ __global__ __launch_bounds__(64) void k_conv_wgrad(WgradArgs a)
             uint16_t ua = 0, ub = 0;
             if (p < pe) {
                 ua = a.dY[(size_t)p * a.ldd + a.doff + c0 + r16];
-                const int n = p / (a.H * a.W), rem = p - n * a.H * a.W;
-                const int yy = rem / a.W + dy, xx = rem % a.W + dx;
+                const int n = fdiv(p, HW, invHW), rem = p - n * HW;
+                const int yq = fdiv(rem, a.W, invW);
+                const int yy = yq + dy, xx = rem - yq * a.W + dx;
                 if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
                     ub = a.X[((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff + i0 + r16];
             }
@@ -157,20 +169,23 @@ __global__ void k_reduce_partials(const float *partial, int nsplit, size_t n, fl
     out[i] = accumulate ? out[i] + s : s;
 }
 
-// db[c] = sum_p dY[p][c]: one workgroup per channel, fixed-order tree
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *dY, int ldd, int doff, int P, float *db)
+// Partial bias gradients over one pixel chunk per blockIdx.y:
+// partial[chunk][c] = sum_{p in chunk} dY[p][c]. A workgroup covers 64 channels
+// (contiguous across lanes, so each pixel row is one coalesced read) times 4
+// pixel lanes; the chunk partials are then reduced in order (deterministic).
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *dY, int ldd, int doff, int P, int C, int chunk,
+                                                   float *partial)
 {
-    __shared__ float sh[256];
-    const int c = blockIdx.x;
+    __shared__ float sh[4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lane;
+    const int p0 = blockIdx.y * chunk, p1 = min(P, p0 + chunk);
     float s = 0.f;
-    for (int p = threadIdx.x; p < P; p += 256) s += bf2f(dY[(size_t)p * ldd + doff + c]);
-    sh[threadIdx.x] = s;
+    if (c < C)
+        for (int p = p0 + q; p < p1; p += 4) s += bf2f(dY[(size_t)p * ldd + doff + c]);
+    sh[q][lane] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[c] = sh[0];
+    if (q == 0 && c < C) partial[(size_t)blockIdx.y * C + c] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
 }
 
 // Backward-data weights: Wb[ci][tap][co] = W[co][taps-1-tap][ci] (BF16)
@@ -339,9 +354,11 @@ void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, 
 {
     hipLaunchKernelGGL(k_reduce_partials, g1(n), dim3(256), 0, s, partial, nsplit, n, out, acc);
 }
-void bias_grad(hipStream_t s, const bf16 *dY, int ldd, int doff, int P, int C, float *db)
+void bias_grad(hipStream_t s, const bf16 *dY, int ldd, int doff, int P, int C, float *db, float *partial)
 {
-    hipLaunchKernelGGL(k_bias_grad, dim3(C), dim3(256), 0, s, dY, ldd, doff, P, db);
+    const int chunk = 2048, nchunk = (P + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_bias_grad, dim3((C + 63) / 64, nchunk), dim3(256), 0, s, dY, ldd, doff, P, C, chunk, partial);
+    reduce_partials(s, partial, nchunk, (size_t)C, db, 0);
 }
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb)
 {

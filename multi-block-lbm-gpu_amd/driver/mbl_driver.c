@@ -6,9 +6,11 @@
  * evolution -> error_L2 -> getDrag_block with the per-chunk timing line of
  * time_t.dat. It includes only include/mbl.h and links libmbl_hip.
  *
- *   mbl_driver [layout] [steps] [T]
+ *   mbl_driver [layout] [steps] [T] [output]
  *     layout: GEO_1_1 | GEO_2_2 | GEO_2_5 (default) | GEO_4_9 | C1
  *     steps : total coarse steps (default 1000), T: steps per timing chunk
+ *     output: 1 writes steady_<n>.dat at the start and after every chunk, as
+ *             the reference does (Multi-block-LBM.cu:120, 170); default 0
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
@@ -42,6 +44,7 @@ int main(int argc, char **argv)
     int steps = argc > 2 ? atoi(argv[2]) : 1000;
     int T = argc > 3 ? atoi(argv[3]) : 1000;
     if (T <= 0) T = 1000;
+    const int write_fields = argc > 4 ? atoi(argv[4]) : 0;
 
     mbl_geometry g;
     mbl_config cfg;
@@ -77,6 +80,7 @@ int main(int argc, char **argv)
     cfg.LEVEL = g.LEVEL;
     for (int l = 0; l < g.LEVEL; l++) cfg.mi[l] = g.mi[l];
     cfg.number_line2 = g.number_line2;
+    cfg.number_area = g.number_area;
     cfg.flush_interval = T;
 
     const int na = g.number_area, nl = g.number_line, nl2 = g.number_line2;
@@ -143,6 +147,7 @@ int main(int argc, char **argv)
         return 1;
     }
     printf("%s: %d blocks, %d coarse/fine lines, %d same-level lines, %d drag links\n", layout, na, nl, nl2, N_drag);
+    if (write_fields) output(0, Areas, Vh, Vd); /* Multi-block-LBM.cu:120 */
 
     int asp[MBL_MAX_LEVEL + 1], ls3[MBL_MAX_LEVEL + 1], ltc[MBL_MAX_LEVEL + 1], lcc[MBL_MAX_LEVEL + 1];
     for (int l = 0; l <= g.LEVEL; l++) {
@@ -178,6 +183,7 @@ int main(int argc, char **argv)
             fclose(fp);
         }
         printf("MLUPS %.1f\n", updates * chunk / dt / 1e6);
+        if (write_fields) output(done, Areas, Vh, Vd); /* Multi-block-LBM.cu:170 */
     }
     double diag[5];
     mbl_last_diagnostics(h, diag);

@@ -98,6 +98,12 @@ def load_library(fp64=False):
         "mbl_batch_block": (ip, [vp, ip, ip, C.POINTER(AREA), C.POINTER(D2Q9)]),
         "mbl_batch_raster": (ip, [vp, ip, ip, fp]),
         "mbl_batch_init_from_raster": (ip, [vp, ip, ip, fp]),
+        "output": (None, [ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(D2Q9)]),
+        "mbl_output": (ip, [vp, ip, ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(D2Q9)]),
+        "mbl_checkpoint_save": (ip, [vp, C.c_char_p, C.c_longlong, ip, C.POINTER(AREA), C.POINTER(D2Q9),
+                                     C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(Ff_Store)]),
+        "mbl_checkpoint_load": (ip, [vp, C.c_char_p, C.POINTER(C.c_longlong), ip, C.POINTER(AREA), C.POINTER(D2Q9),
+                                     C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(Ff_Store)]),
         "mbl_net_create": (ip, [vp, ip, ip, ip, ip, ip, ip, C.c_ulonglong, C.POINTER(vp)]),
         "mbl_net_destroy": (None, [vp]),
         "mbl_net_num_params": (C.c_longlong, [vp]),
@@ -221,7 +227,7 @@ class MultiBlockSim:
                              bc_mode=geom.bc_mode if bc_mode is None else bc_mode,
                              init_mode=geom.init_mode if init_mode is None else init_mode,
                              storage=storage, use_graph=use_graph, sync_diag=sync_diag, write_files=write_files,
-                             out_dir=out_dir, flush_interval=flush_interval)
+                             out_dir=out_dir, flush_interval=flush_interval, number_area=g.number_area)
         for i in range(g.LEVEL):
             cfg.mi[i] = g.mi[i]
         self.cfg = cfg
@@ -316,6 +322,25 @@ class MultiBlockSim:
 
     def flush(self):
         check(self.lib, self.lib.mbl_flush_diagnostics(self.h), "flush")
+
+    def output(self, n=None):
+        """steady_<n>.dat (Tecplot point file of x, y, solid, rho, U, V per block) in out_dir."""
+        check(self.lib, self.lib.mbl_output(self.h, self.t if n is None else n, self.na, self.areas, None,
+                                            self.pps_d), "output")
+
+    def checkpoint(self, path):
+        """Saves the whole time-stepping state (planes, macros, previous velocities, rings, markers, step)."""
+        check(self.lib, self.lib.mbl_checkpoint_save(self.h, str(path).encode(), self.t, self.na, self.areas,
+                                                     self.pps_d, self.upx, self.upy, self.nl, self.t3),
+              "checkpoint save")
+
+    def restore(self, path):
+        """Loads a checkpoint of the same layout and storage written by checkpoint()."""
+        step = C.c_longlong()
+        check(self.lib, self.lib.mbl_checkpoint_load(self.h, str(path).encode(), C.byref(step), self.na, self.areas,
+                                                     self.pps_d, self.upx, self.upy, self.nl, self.t3),
+              "checkpoint load")
+        self.t = step.value
 
     def poiseuille_l2(self):
         """Velocity L2 distance from the analytic channel profile (FLUID nodes, 1/m weights)."""

@@ -86,7 +86,7 @@ class mbl_config(C.Structure):
                 ("bc_mode", C.c_int), ("init_mode", C.c_int), ("storage", C.c_int),
                 ("seed", C.c_ulonglong), ("sigma", C.c_double),
                 ("use_graph", C.c_int), ("sync_diag", C.c_int), ("flush_interval", C.c_int),
-                ("write_files", C.c_int), ("out_dir", C.c_char * 256)]
+                ("write_files", C.c_int), ("out_dir", C.c_char * 256), ("number_area", C.c_int)]
 
 
 class mbl_geometry(C.Structure):
@@ -123,5 +123,5 @@ EXPORTS = [
     "mbl_batch_create", "mbl_batch_destroy", "mbl_batch_step", "mbl_batch_node_updates", "mbl_batch_forces",
     "mbl_batch_autotune", "mbl_batch_block", "mbl_batch_raster", "mbl_batch_init_from_raster",
     "mbl_net_create", "mbl_net_destroy", "mbl_net_num_params", "mbl_net_params", "mbl_net_forward",
-    "mbl_net_loss_grad", "mbl_net_train",
+    "mbl_net_loss_grad", "mbl_net_train", "output", "mbl_output", "mbl_checkpoint_save", "mbl_checkpoint_load",
 ]

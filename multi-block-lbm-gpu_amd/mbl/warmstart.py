@@ -5,12 +5,15 @@
 1. Training cases (two-level channels, D in {16, 24} finest cells, blockage
    D/H ~ U[0.1, 0.3], Re ~ U[80, 200], the cylinder 0.25 coarse cells above
    the channel centre line) start at rest with the parabolic profile and run
-   `spin` steps with shifted FP16 planes, then `nsnap` snapshots of
+   `spin` steps with 16-bit planes (the 1-3-12 format by default), then `nsnap` snapshots of
    (rho, ux, uy) are rasterised onto an H x W grid of each channel, every
    `every` steps.
-2. Inputs per pixel: solid mask, Re/200, D/H, sin and cos of the shedding
-   phase (linear in time between the upward zero crossings of Cl that
-   bracket the snapshot), x and y; targets: (0.25 (rho - 1)/U^2, ux/U, uy/U).
+2. Phase model (default): inputs per pixel are the solid mask, Re/200, D/H,
+   sin and cos of the shedding phase (linear in time between the upward zero
+   crossings of Cl that bracket the snapshot), x and y; targets
+   (0.25 (rho - 1)/U^2, ux/U, uy/U). Harmonic model: the inputs without the
+   phase, and as targets each case's least-squares mean and two shedding
+   harmonics per pixel (15 channels).
    The encoder-decoder trains on the GPU; 1/8 of the training cases are held
    out for a validation loss.
 3. Held-out cases run twice from step 0: cold (profile at rest) and warm
@@ -69,6 +72,42 @@ def raster_inputs(specs, H, W, phase):
 
 
 RHO_SCALE = 0.25  # scale of the density target: its weight in the loss (measured best of 0.25 and 1)
+
+
+STATIC_CH = 5  # harmonic model inputs: mask, Re/200, D/H, x, y
+NHARM = 5      # mean, cos and sin of the phase and of twice the phase
+
+
+def static_inputs(specs, H, W):
+    """[n, H, W, 5]: the phase-free inputs (mask, Re/200, D/H, x, y)."""
+    x = raster_inputs(specs, H, W, 0.0)
+    return np.ascontiguousarray(x[..., [0, 1, 2, 5, 6]])
+
+
+def harmonic_basis(phase):
+    phase = np.asarray(phase, np.float64)
+    return np.stack([np.ones_like(phase), np.cos(phase), np.sin(phase), np.cos(2 * phase), np.sin(2 * phase)], -1)
+
+
+def harmonic_fit(targets, phases):
+    """Least-squares mean + two shedding harmonics of a case's snapshots:
+    targets [k, H, W, 3], phases [k] -> coefficients [H, W, 5 * 3] (harmonic-major)."""
+    A = harmonic_basis(phases)
+    k, H, W, C = targets.shape
+    coef, *_ = np.linalg.lstsq(A, targets.reshape(k, -1).astype(np.float64), rcond=None)
+    return coef.reshape(NHARM, H, W, C).transpose(1, 2, 0, 3).reshape(H, W, NHARM * C).astype(np.float32)
+
+
+def harmonic_eval(coef, phase):
+    """Targets from coefficients [..., H, W, 15]: at a scalar phase -> [..., H, W, 3]; one field
+    [H, W, 15] at phases [k] -> [k, H, W, 3]; fields [n, H, W, 15] at phases [n] -> [n, H, W, 3]."""
+    c = coef.reshape(coef.shape[:-1] + (NHARM, 3))
+    b = harmonic_basis(phase)
+    if np.ndim(phase) == 0:
+        return np.einsum("...hc,h->...c", c, b)
+    if c.ndim == 4:
+        return np.einsum("yxhc,kh->kyxc", c, b)
+    return np.einsum("nyxhc,nh->nyxc", c, b)
 
 
 def to_target(fields, specs):
@@ -234,6 +273,15 @@ def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None):
     return cd, cl, wall, (shots, shot_t), nu
 
 
+def predict(net, model, specs, H, W, phase):
+    """Targets [n, H, W, 3] of the cases at phase (scalar or per case)."""
+    if model == "phase":
+        return net.forward(raster_inputs(specs, H, W, phase))
+    coef = net.forward(static_inputs(specs, H, W))
+    ph = np.broadcast_to(np.asarray(phase, np.float64), (len(specs),))
+    return np.stack([harmonic_eval(coef[i], ph[i]) for i in range(len(specs))])
+
+
 def convergence(cl_c, cl_w, cd_w):
     """Per case: steps to a converged Cl amplitude, cold and warm, and the fallback count."""
     tc, tw, fallback = [], [], 0
@@ -314,11 +362,15 @@ def main(argv=None):
     ap.add_argument("--H", type=int, default=48)
     ap.add_argument("--W", type=int, default=192)
     ap.add_argument("--base", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=None, help="default 60 (phase model), 400 (harmonic model)")
+    ap.add_argument("--model", default="phase", choices=["harmonic", "phase"],
+                    help="harmonic: the network predicts each pixel's mean and two shedding harmonics from the "
+                         "geometry and Re; phase: it predicts the field at a given phase")
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16s"], help="planes of the held-out runs")
-    ap.add_argument("--snap-storage", default="f16s", choices=["f32", "f16s"], help="planes of the snapshot runs")
+    ap.add_argument("--snap-storage", default="f16d", choices=["f32", "f16s", "f16d"],
+                    help="planes of the snapshot runs (f16d: the 1-3-12 16-bit format)")
     ap.add_argument("--save", default=None, help="write the series and the trained parameters to this .npz")
     ap.add_argument("--self-init", action="store_true",
                     help="no network: each held-out case restarts from the raster of its own final cold state "
@@ -326,12 +378,17 @@ def main(argv=None):
     ap.add_argument("--self-init-rho", default="raster", choices=["raster", "one", "poisson"],
                     help="with --self-init: keep the raster's density, start from rho = 1 or solve for it")
     ap.add_argument("--dump", default=None, help="with --self-init: save the final rasters to this .npz")
+    ap.add_argument("--diagnose", action="store_true",
+                    help="also restart from hybrids of the network's field and each case's own final cold state "
+                         "at the same phase (exact density + network velocity, and the reverse)")
     ap.add_argument("--rho", default="net", choices=["net", "poisson"],
                     help="warm-start density: the network's or the pressure Poisson solution of its velocity")
     args = ap.parse_args(argv)
     lib = load_library(False)
     storage = abi.STORE_NATIVE if args.storage == "f32" else abi.STORE_F16S
-    snap_storage = abi.STORE_NATIVE if args.snap_storage == "f32" else abi.STORE_F16S
+    snap_storage = {"f32": abi.STORE_NATIVE, "f16s": abi.STORE_F16S, "f16d": abi.STORE_F16D}[args.snap_storage]
+    if args.epochs is None:
+        args.epochs = 400 if args.model == "harmonic" else 60
     H, W = args.H, args.W
     out = {"train_cases": args.train, "test_cases": args.test, "raster": [H, W], "storage": args.storage,
            "snap_storage": args.snap_storage}
@@ -347,54 +404,75 @@ def main(argv=None):
                                                     log="train cases")
     out["snapshot_run_s"] = round(wall_tr, 2)
     out["snapshot_run_mlups"] = round(nu_tr * steps_tr / wall_tr / 1e6, 1)
-    xs, ts = [], []
+    # per case: the snapshots' targets and phases
+    case_t, case_ph = [], []
     for c, sp in enumerate(tr):
         x = cl_tr[c, args.spin:]
-        for k, t in enumerate(shot_t):
-            phase = crossing_phase(x, t - 1 - args.spin)
-            if not np.isfinite(phase):
-                continue
-            xs.append(raster_inputs([sp], H, W, phase)[0])
-            ts.append(to_target(shots[k][c:c + 1], [sp])[0])
-    X = np.stack(xs)
-    T = np.stack(ts).astype(np.float32)
-    ok = np.isfinite(T).all(axis=(1, 2, 3))
-    X, T = X[ok], T[ok]
-    out["samples"] = int(X.shape[0])
+        ph = np.array([crossing_phase(x, t - 1 - args.spin) for t in shot_t])
+        tg = np.stack([to_target(shots[k][c:c + 1], [sp])[0] for k in range(len(shot_t))])
+        ok = np.isfinite(ph) & np.isfinite(tg).all(axis=(1, 2, 3))
+        case_t.append(tg[ok])
+        case_ph.append(ph[ok])
+    nval = max(1, args.train // 8)  # the last cases are held out for validation
+    out["model"] = args.model
+    if args.model == "phase":
+        in_ch, out_ch = IN_CH, OUT_CH
+        def samples(cases):
+            X = np.concatenate([raster_inputs([tr[c]] * len(case_ph[c]), H, W, case_ph[c]) for c in cases])
+            return X, np.concatenate([case_t[c] for c in cases]).astype(np.float32)
+    else:
+        in_ch, out_ch = STATIC_CH, NHARM * OUT_CH
+        def samples(cases):
+            use = [c for c in cases if len(case_ph[c]) >= 2 * NHARM]
+            X = np.concatenate([static_inputs([tr[c]], H, W) for c in use])
+            return X, np.stack([harmonic_fit(case_t[c], case_ph[c]) for c in use])
+    Xt, Tt = samples(range(len(tr) - nval))
+    Xv, Tv = samples(range(len(tr) - nval, len(tr)))
+    out["samples"] = int(Xt.shape[0] + Xv.shape[0])
 
-    # 2. training: the last `val` cases' snapshots are held out for a validation loss;
-    # the learning rate steps down (lr, lr/3, lr/10 over 50%, 30%, 20% of the epochs)
-    nval = max(1, args.train // 8) * args.nsnap
-    Xt, Tt, Xv, Tv = X[:-nval], T[:-nval], X[-nval:], T[-nval:]
-    net = Net(lib, H, W, IN_CH, OUT_CH, base=args.base, max_batch=args.batch, seed=2024)
+    # 2. training; the learning rate steps down (lr, lr/3, lr/10 over 50%, 30%, 20% of the epochs)
+    batch = min(args.batch, Xt.shape[0])
+    net = Net(lib, H, W, in_ch, out_ch, base=args.base, max_batch=max(batch, args.batch), seed=2024)
     t0 = time.perf_counter()
     hist = []
     e1 = max(1, int(0.5 * args.epochs))
     e2 = max(1, int(0.3 * args.epochs))
     for ep, lr in ((e1, args.lr), (e2, args.lr / 3), (max(1, args.epochs - e1 - e2), args.lr / 10)):
-        for _ in range(ep):  # one call per epoch: Adam state carries over, progress is printed
-            hist.extend(net.train(Xt, Tt, batch=args.batch, epochs=1, lr=lr, seed=99 + len(hist)))
-            if len(hist) % 5 == 0:
-                print(f"# epoch {len(hist)} loss {hist[-1]:.4e} {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                      flush=True)
+        done_ep = 0
+        while done_ep < ep:  # calls of a few epochs: Adam state carries over, progress is printed
+            k = min(ep - done_ep, 5 if args.model == "phase" else 25)
+            hist.extend(net.train(Xt, Tt, batch=batch, epochs=k, lr=lr, seed=99 + len(hist)))
+            done_ep += k
+            print(f"# epoch {len(hist)} loss {hist[-1]:.4e} {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
     hist = np.array(hist)
     out["train_s"] = round(time.perf_counter() - t0, 2)
     out["epochs"] = int(hist.size)
     out["epoch_loss_first_last"] = [float(hist[0]), float(hist[-1])]
-    out["val_loss"] = float(np.mean((net.forward(Xv) - Tv) ** 2))
-    out["val_loss_per_channel"] = [float(v) for v in np.mean((net.forward(Xv) - Tv) ** 2, axis=(0, 1, 2))]
-    out["target_var_per_channel"] = [float(v) for v in np.var(Tv, axis=(0, 1, 2))]
+    # validation on the held-out cases' snapshots, in the target units of the phase model
+    vcases = range(len(tr) - nval, len(tr))
+    Tsnap = np.concatenate([case_t[c] for c in vcases])
+    if args.model == "phase":
+        Psnap = net.forward(Xv)
+    else:
+        pv = net.forward(np.concatenate([static_inputs([tr[c]], H, W) for c in vcases]))
+        Psnap = np.concatenate([harmonic_eval(pv[i], case_ph[c]) for i, c in enumerate(vcases)])
+    err = (Psnap - Tsnap) ** 2
+    out["val_loss"] = float(np.mean(err))
+    out["val_loss_per_channel"] = [float(v) for v in np.mean(err, axis=(0, 1, 2))]
+    out["target_var_per_channel"] = [float(v) for v in np.var(Tsnap, axis=(0, 1, 2))]
     # validation error along the channel (8 bins in x), per channel, relative to the target variance
-    ev = np.mean((net.forward(Xv) - Tv) ** 2, axis=(0, 1)).reshape(8, -1, OUT_CH).mean(axis=1)
-    out["val_rel_err_along_x"] = np.round(ev / np.var(Tv, axis=(0, 1, 2)), 4).T.tolist()
+    ev = np.mean(err, axis=(0, 1)).reshape(8, -1, OUT_CH).mean(axis=1)
+    out["val_rel_err_along_x"] = np.round(ev / np.var(Tsnap, axis=(0, 1, 2)), 4).T.tolist()
     print(f"# trained: loss {hist[0]:.4e} -> {hist[-1]:.4e} in {out['train_s']} s, val {out['val_loss']:.4e}",
           file=sys.stderr, flush=True)
 
     # 3. held-out cases, cold and warm
     te = c5_specs(args.test, seed=202)
-    cd_c, cl_c, wall_c, _, nu = run(lib, te, args.steps, storage, log="cold")
+    cd_c, cl_c, wall_c, (fin, _), nu = run(lib, te, args.steps, storage, log="cold",
+                                           snap=(args.steps, 1, 1, H, W) if args.diagnose else None)
     t0 = time.perf_counter()
-    pred = net.forward(raster_inputs(te, H, W, 0.0))
+    pred = predict(net, args.model, te, H, W, 0.0)
     init = from_target(pred, te).astype(np.float32)
     if args.rho == "poisson":
         init = poisson_density(init, te)
@@ -406,6 +484,29 @@ def main(argv=None):
     out["steps_run"] = args.steps
     out["mlups"] = round(nu * args.steps / wall_c / 1e6, 1)
     summary(out, tc, tw, fallback, ms_step, t_inf)
+    if args.diagnose:
+        # phase of each case's final cold state: linear from its last upward zero crossing over one period
+        ph = np.zeros(len(te))
+        for c in range(len(te)):
+            T_c = max(shedding_period(cl_c[c]), 1)
+            x = cl_c[c, -4 * T_c:] - np.mean(cl_c[c, -4 * T_c:])
+            s_up = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+            t_last = s_up[-1] + (-x[s_up[-1]]) / (x[s_up[-1] + 1] - x[s_up[-1]]) if s_up.size else 0.0
+            ph[c] = 2 * np.pi * ((x.size - 1 - t_last) / T_c)
+        exact = fin[0]
+        netf = from_target(predict(net, args.model, te, H, W, ph), te).astype(np.float32)
+        variants = {"net_matched_phase": netf,
+                    "exact_rho_net_u": np.concatenate([exact[..., :1], netf[..., 1:]], -1),
+                    "net_rho_exact_u": np.concatenate([netf[..., :1], exact[..., 1:]], -1)}
+        diag = {}
+        for name, init_v in variants.items():
+            cd_v, cl_v, _, _, _ = run(lib, te, args.steps, storage, init=init_v, log=name)
+            tcv, twv, fbv = convergence(cl_c, cl_v, cd_v)
+            both_v = (tcv > 0) & (twv > 0)
+            diag[name] = {"median_steps": float(np.median(twv[both_v])) if both_v.any() else None,
+                          "speedup_median_steps": round(float(np.median(tcv[both_v]) / np.median(twv[both_v])), 3)
+                          if both_v.any() else None, "fallbacks": fbv}
+        out["diagnose"] = diag
     if args.save:
         np.savez(args.save, tc=tc, tw=tw, cl_cold=cl_c[:, ::10].astype(np.float32),
                  cl_warm=cl_w[:, ::10].astype(np.float32), loss=hist, params=net.get_params())

@@ -126,3 +126,21 @@ def test_poisson_density_of_plane_poiseuille_flow():
         want = 1 + drop * (1 - X / (sp.LX - 1.0))
         assert np.allclose(g[c, :, :, 0], want[None, :], rtol=0, atol=1e-6 * drop + 1e-7)
         assert np.array_equal(g[c, :, :, 1:], f[c, :, :, 1:])
+
+
+def test_harmonic_fit_recovers_mean_and_two_harmonics():
+    rng = np.random.default_rng(4)
+    coef = rng.normal(size=(6, 10, 3 * ws.NHARM)).astype(np.float32)
+    ph = rng.uniform(0, 2 * np.pi, 16)
+    snaps = ws.harmonic_eval(coef, ph)
+    assert snaps.shape == (16, 6, 10, 3)
+    # the snapshot at phase p is mean + a1 cos p + b1 sin p + a2 cos 2p + b2 sin 2p
+    c = coef.reshape(6, 10, ws.NHARM, 3)
+    p = ph[3]
+    want = c[:, :, 0] + c[:, :, 1] * np.cos(p) + c[:, :, 2] * np.sin(p) + c[:, :, 3] * np.cos(2 * p) \
+        + c[:, :, 4] * np.sin(2 * p)
+    assert np.allclose(snaps[3], want, atol=1e-5)
+    assert np.allclose(ws.harmonic_fit(snaps, ph), coef, atol=1e-4)
+    noisy = snaps + rng.normal(0, 1e-3, snaps.shape)
+    assert np.abs(ws.harmonic_fit(noisy, ph) - coef).max() < 5e-3
+    assert ws.static_inputs(ws.c5_specs(2, seed=1), 16, 32).shape == (2, 16, 32, ws.STATIC_CH)
