@@ -16,7 +16,7 @@ struct ConvArgs {
     int N, H, W, Cin;   // Cin % 32 == 0
     const bf16 *Wt;     // [Cout][ksize*ksize][Cin]
     const float *bias;  // [Cout] or null
-    int Cout;           // % 16 == 0
+    int Cout;           // % 32 == 0
     int ksize;          // 3 or 1
     bf16 *Y;            // BF16 output or null
     int ldy, yoff;
@@ -50,6 +50,7 @@ void copy_ch(hipStream_t s, const bf16 *src, int lds, int soff, bf16 *dst, int l
 int mse(hipStream_t s, const float *Y, int ldy, const float *T, int C, int P, bf16 *dY, int ldg, float *partial);
 void adam(hipStream_t s, float *w, const float *g, float *m, float *v, size_t n, float lr, int t);
 void pack_input(hipStream_t s, const float *src, int Creal, int P, bf16 *dst, int Cpad);
+void take_channels(hipStream_t s, const float *src, int lds, int C, int P, float *dst);
 
 } // namespace net
 } // namespace mbl

@@ -55,6 +55,7 @@ struct mbl_net {
     size_t partial_n = 0;
     float *lossp = nullptr;
     float *din = nullptr, *dtarget = nullptr; // FP32 staging of one batch
+    float *dout = nullptr;                    // the real output channels of one batch
     std::vector<void *> allocs;
 };
 
@@ -182,6 +183,7 @@ extern "C" int mbl_net_create(mbl_handle_t h, int H, int W, int in_ch, int out_c
     n->lossp = (float *)dev(n, sizeof(float) * ((size_t)P0 * 32 / 256 + 1));
     n->din = (float *)dev(n, sizeof(float) * (size_t)P0 * in_ch);
     n->dtarget = (float *)dev(n, sizeof(float) * (size_t)P0 * out_ch);
+    n->dout = (float *)dev(n, sizeof(float) * (size_t)P0 * out_ch);
     // split-K partial buffer: the largest layer x its split count
     for (size_t i = 0; i < n->L.size(); i++) {
         const Layer &l = n->L[i];
@@ -388,12 +390,11 @@ extern "C" int mbl_net_forward(mbl_net_t n, const float *in, int N, float *out)
     if (stage(n, in, nullptr, N, false)) return MBL_E_HIP;
     forward(n, N);
     const size_t P0 = (size_t)P_of(n, N, 0);
-    std::vector<float> o(P0 * 32);
-    if (hipMemcpyAsync(o.data(), n->out, sizeof(float) * P0 * 32, hipMemcpyDeviceToHost, n->s) != hipSuccess ||
+    // only the real output channels of the 32-wide head cross to the host
+    take_channels(n->s, n->out, 32, n->out_ch, (int)P0, n->dout);
+    if (hipMemcpyAsync(out, n->dout, sizeof(float) * P0 * n->out_ch, hipMemcpyDeviceToHost, n->s) != hipSuccess ||
         hipStreamSynchronize(n->s) != hipSuccess)
         return MBL_E_HIP;
-    for (size_t p = 0; p < P0; p++)
-        for (int j = 0; j < n->out_ch; j++) out[p * n->out_ch + j] = o[p * 32 + j];
     return 0;
 }
 

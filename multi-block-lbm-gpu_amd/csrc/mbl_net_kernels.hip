@@ -41,58 +41,92 @@ __device__ inline bf16x8 zero8()
 // ---------------------------------------------------------------------------
 // 3x3 (or 1x1) convolution, zero padding, as an implicit GEMM:
 //   out[p][co] = bias[co] + sum_{tap, ci} X[p + tap][ci] * Wt[co][tap][ci]
-// Workgroup: 4 waves = 64 output pixels x 32 output channels; wave w owns
-// pixels 16w..16w+15 and both 16-channel halves. Cin % 32 == 0, Cout % 16 == 0.
+// Workgroup: 4 waves = 128 output pixels x COT output channels (COT = 64, or
+// 32 when Cout is not a multiple of 64); wave w owns pixels 32w..32w+31 as two
+// 16-pixel MFMA rows and all COT/16 channel tiles, so each activation load
+// feeds COT/16 MFMAs. Per k-step (one tap, 32 input channels) the COT x 32
+// weight tile is loaded once per workgroup into LDS (one 16-byte load per
+// thread) and read from there by the four waves. Cin % 32 == 0, Cout % 32 == 0.
 // Epilogue: optional bias, optional multiply by (mask > 0) (ReLU derivative of
 // the layer input in the backward-data pass), optional ReLU, BF16 store into a
-// channel slice of Y (leading dimension ldy, offset yoff).
+// channel slice of Y (leading dimension ldy, offset yoff) and/or FP32 store.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a)
+constexpr int CONV_PX = 128;  // output pixels per workgroup
+constexpr int CONV_WLD = 40;  // LDS row of the weight tile (bf16): 32 + 8, rows stay 16-byte aligned
+
+template <int COT> __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int NC = COT / 16;
+    __shared__ __attribute__((aligned(16))) bf16 sW[COT * CONV_WLD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int g = lane >> 4, r16 = lane & 15;
-    const int P = a.N * a.H * a.W;
-    const int p0 = blockIdx.x * 64 + 16 * w;
-    const int c0 = blockIdx.y * 32;
-    const int p = p0 + r16;
-    const int n = p / (a.H * a.W), rem = p - n * a.H * a.W;
-    const int y = rem / a.W, x = rem - y * a.W;
-    const bool two = (c0 + 16 < a.Cout);
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    const int HW = a.H * a.W, P = a.N * HW;
+    const int p0 = blockIdx.x * CONV_PX + 32 * w;
+    const int c0 = blockIdx.y * COT;
     const int taps = a.ksize * a.ksize, half = a.ksize / 2;
+    // the two pixels this lane supplies (row r16 of each 16-pixel MFMA row)
+    int pn[2], py[2], px[2];
+    bool pin[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int p = p0 + 16 * h + r16;
+        pin[h] = p < P;
+        pn[h] = p / HW;
+        const int rem = p - pn[h] * HW;
+        py[h] = rem / a.W;
+        px[h] = rem - py[h] * a.W;
+    }
+    f32x4 acc[2][NC];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[h][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // weight staging role: row t >> 2 of the tile, input channels (t & 3) * 8 .. + 7
+    const int wrow = t >> 2, wcc = (t & 3) * 8;
     for (int tap = 0; tap < taps; tap++) {
-        const int yy = y + tap / a.ksize - half, xx = x + tap % a.ksize - half;
-        const bool valid = (p < P) && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-        const bf16 *xrow = a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff;
-        const bf16 *w0 = a.Wt + ((size_t)(c0 + r16) * taps + tap) * a.Cin;
-        const bf16 *w1 = a.Wt + ((size_t)(c0 + 16 + r16) * taps + tap) * a.Cin;
+        const int ty = tap / a.ksize - half, tx = tap % a.ksize - half;
+        const bf16 *xrow[2];
+        bool valid[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int yy = py[h] + ty, xx = px[h] + tx;
+            valid[h] = pin[h] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            xrow[h] = valid[h] ? a.X + ((size_t)(pn[h] * a.H + yy) * a.W + xx) * a.ldx + a.xoff : a.X;
+        }
+        const bf16 *wsrc = a.Wt + ((size_t)(c0 + wrow) * taps + tap) * a.Cin + wcc;
         for (int ci = 0; ci < a.Cin; ci += 32) {
-            const bf16x8 av = valid ? load8(xrow + ci + 8 * g) : zero8();
-            const bf16x8 b0 = load8(w0 + ci + 8 * g);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b0, acc0, 0, 0, 0);
-            if (two) {
-                const bf16x8 b1 = load8(w1 + ci + 8 * g);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b1, acc1, 0, 0, 0);
+            __syncthreads(); // the previous k-step's reads of sW are done
+            if (wrow < COT) *reinterpret_cast<uint4 *>(sW + wrow * CONV_WLD + wcc) = *reinterpret_cast<const uint4 *>(wsrc + ci);
+            bf16x8 av[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) av[h] = valid[h] ? load8(xrow[h] + ci + 8 * g) : zero8();
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const bf16x8 bv = load8(sW + (16 * c + r16) * CONV_WLD + 8 * g);
+#pragma unroll
+                for (int h = 0; h < 2; h++) acc[h][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[h], bv, acc[h][c], 0, 0, 0);
             }
         }
     }
+    // acc[h][c][r] = out[pixel p0 + 16h + 4g + r][channel c0 + 16c + r16]
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-        if (t == 1 && !two) break;
-        const f32x4 acc = t ? acc1 : acc0;
-        const int co = c0 + 16 * t + r16;
+    for (int h = 0; h < 2; h++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int pix = p0 + 4 * g + r;
-            if (pix >= P) continue;
-            float v = acc[r];
-            if (a.bias) v += a.bias[co];
-            if (a.M && !(bf2f(a.M[(size_t)pix * a.ldm + a.moff + co]) > 0.f)) v = 0.f;
-            if (a.relu) v = fmaxf(v, 0.f);
-            if (a.Yf) a.Yf[(size_t)pix * a.ldyf + co] = v;
-            if (a.Y) a.Y[(size_t)pix * a.ldy + a.yoff + co] = f2bf(v);
+        for (int c = 0; c < NC; c++) {
+            const int co = c0 + 16 * c + r16;
+            const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int pix = p0 + 16 * h + 4 * g + r;
+                if (pix >= P) continue;
+                float v = acc[h][c][r] + bias;
+                if (a.M && !(bf2f(a.M[(size_t)pix * a.ldm + a.moff + co]) > 0.f)) v = 0.f;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.Yf) a.Yf[(size_t)pix * a.ldyf + co] = v;
+                if (a.Y) a.Y[(size_t)pix * a.ldy + a.yoff + co] = f2bf(v);
+            }
         }
-    }
 }
 
 // floor(v / d) for 0 <= v < 2^24 through a float reciprocal, corrected to the exact quotient
@@ -105,57 +139,73 @@ __device__ inline int fdiv(int v, int d, float inv)
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[co][tap][ci] += sum_p dY[p][co] * X[p + tap][ci] over one
+// Weight gradient: dW[co][tap][ci] = sum_p dY[p][co] * X[p + tap][ci] over one
 // pixel chunk per blockIdx.z, written as FP32 partials (reduced in order by
-// k_reduce_partials). Workgroup = one wave computing a 16 co x 16 ci tile of
-// one tap; K runs over pixels in steps of 32.
+// k_reduce_partials). A workgroup (4 waves) computes a 32 co x 32 ci tile of
+// one tap. Per stage it loads 64 pixels of dY[.][co0..co0+31] and of the
+// tap-shifted X[.][ci0..ci0+31] with one 16-byte load per thread and operand,
+// stores both transposed in LDS ([channel][pixel], so each lane's 8 K values
+// are one 16-byte LDS read), and each wave runs two MFMA k-steps (K = 32
+// pixels) on its 16 x 16 sub-tile.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_conv_wgrad(WgradArgs a)
+constexpr int WG_PX = 64;      // pixels per stage
+constexpr int WG_LD = WG_PX + 8; // LDS row length (bf16): rows stay 16-byte aligned, banks spread
+
+__global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
 {
-    const int lane = threadIdx.x;
+    __shared__ __attribute__((aligned(16))) bf16 sA[32 * WG_LD]; // dY^T
+    __shared__ __attribute__((aligned(16))) bf16 sB[32 * WG_LD]; // X^T (tap-shifted)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int taps = a.ksize * a.ksize, half = a.ksize / 2;
-    const int c0 = blockIdx.x * 16;
+    const int co0 = blockIdx.x * 32;
     const int tap = blockIdx.y % taps;
-    const int i0 = (blockIdx.y / taps) * 16;
+    const int ci0 = (blockIdx.y / taps) * 32;
     const int P = a.N * a.H * a.W;
     const int chunk = (P + gridDim.z - 1) / gridDim.z;
     const int pb = blockIdx.z * chunk, pe = min(P, pb + chunk);
     const int dy = tap / a.ksize - half, dx = tap % a.ksize - half;
     const int HW = a.H * a.W;
     const float invHW = 1.0f / (float)HW, invW = 1.0f / (float)a.W;
+    // loader role: pixel lp of the stage, channels lc..lc+7 of both operands
+    const int lp = t >> 2, lc = (t & 3) * 8;
+    // compute role: wave w owns co rows 16*(w&1).. and ci columns 16*(w>>1)..
+    const int wco = (w & 1) * 16, wci = (w >> 1) * 16;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = pb; k0 < pe; k0 += 32) {
-        __bf16 av[8], bv[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int p = k0 + 8 * g + e;
-            uint16_t ua = 0, ub = 0;
-            if (p < pe) {
-                ua = a.dY[(size_t)p * a.ldd + a.doff + c0 + r16];
-                const int n = fdiv(p, HW, invHW), rem = p - n * HW;
-                const int yq = fdiv(rem, a.W, invW);
-                const int yy = yq + dy, xx = rem - yq * a.W + dx;
-                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                    ub = a.X[((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff + i0 + r16];
-            }
-            av[e] = __builtin_bit_cast(__bf16, ua);
-            bv[e] = __builtin_bit_cast(__bf16, ub);
+    for (int s0 = pb; s0 < pe; s0 += WG_PX) {
+        const int p = s0 + lp;
+        uint4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
+        if (p < pe) {
+            va = *reinterpret_cast<const uint4 *>(a.dY + (size_t)p * a.ldd + a.doff + co0 + lc);
+            const int n = fdiv(p, HW, invHW), rem = p - n * HW;
+            const int yq = fdiv(rem, a.W, invW);
+            const int yy = yq + dy, xx = rem - yq * a.W + dx;
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                vb = *reinterpret_cast<const uint4 *>(a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff +
+                                                      ci0 + lc);
         }
-        bf16x8 A, B;
+        const bf16 *ea = reinterpret_cast<const bf16 *>(&va);
+        const bf16 *eb = reinterpret_cast<const bf16 *>(&vb);
+        __syncthreads(); // the previous stage's reads are done
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            A[e] = av[e];
-            B[e] = bv[e];
+        for (int i = 0; i < 8; i++) {
+            sA[(lc + i) * WG_LD + lp] = ea[i];
+            sB[(lc + i) * WG_LD + lp] = eb[i];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < WG_PX / 32; ks++) {
+            const bf16x8 A = load8(sA + (wco + r16) * WG_LD + ks * 32 + 8 * g);
+            const bf16x8 B = load8(sB + (wci + r16) * WG_LD + ks * 32 + 8 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+        }
     }
-    // acc[r] = D[co = c0 + 4g + r][ci = i0 + r16]
+    // acc[r] = D[co = co0 + wco + 4g + r][ci = ci0 + wci + r16]
     float *out = a.partial + (size_t)blockIdx.z * a.Cout * taps * a.Cin;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int co = c0 + 4 * g + r;
-        out[((size_t)co * taps + tap) * a.Cin + i0 + r16] = acc[r];
+        const int co = co0 + wco + 4 * g + r;
+        out[((size_t)co * taps + tap) * a.Cin + ci0 + wci + r16] = acc[r];
     }
 }
 
@@ -337,18 +387,30 @@ __global__ void k_pack_input(const float *src, int Creal, int P, bf16 *dst, int 
     dst[i] = c < Creal ? f2bf(src[p * Creal + c]) : (bf16)0;
 }
 
+// dst[p][c] = src[p][c] for the first C of lds channels
+__global__ void k_take_channels(const float *src, int lds, int C, int P, float *dst)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)P * C) return;
+    const size_t p = i / C;
+    dst[i] = src[p * lds + (i - p * C)];
+}
+
 // ---- launchers ----
 static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void conv(hipStream_t s, const ConvArgs &a)
 {
     const int P = a.N * a.H * a.W;
-    hipLaunchKernelGGL(k_conv, dim3((P + 63) / 64, (a.Cout + 31) / 32), dim3(256), 0, s, a);
+    if (a.Cout % 64 == 0)
+        hipLaunchKernelGGL((k_conv<64>), dim3((P + CONV_PX - 1) / CONV_PX, a.Cout / 64), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_conv<32>), dim3((P + CONV_PX - 1) / CONV_PX, a.Cout / 32), dim3(256), 0, s, a);
 }
 void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit)
 {
     const int taps = a.ksize * a.ksize;
-    hipLaunchKernelGGL(k_conv_wgrad, dim3(a.Cout / 16, taps * (a.Cin / 16), nsplit), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(a.Cout / 32, taps * (a.Cin / 32), nsplit), dim3(256), 0, s, a);
 }
 void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int acc)
 {
@@ -400,6 +462,10 @@ void adam(hipStream_t s, float *w, const float *g, float *m, float *v, size_t n,
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     hipLaunchKernelGGL(k_adam, g1(n), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2, 1e-8f, c1, c2);
+}
+void take_channels(hipStream_t s, const float *src, int lds, int C, int P, float *dst)
+{
+    hipLaunchKernelGGL(k_take_channels, g1((size_t)P * C), dim3(256), 0, s, src, lds, C, P, dst);
 }
 void pack_input(hipStream_t s, const float *src, int Creal, int P, bf16 *dst, int Cpad)
 {
