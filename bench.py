@@ -58,6 +58,24 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
             "mlups": nodes * steps / wall / 1e6, "nodes": nodes}
 
 
+def stream_copy_gbs(lib, mib=4096, reps=10):
+    """Streaming-copy rate (bytes read + written per second) of the library's copy kernel over MIB MiB:
+    the measured ceiling the fused kernel is compared with besides the 8 TB/s datasheet peak."""
+    import ctypes
+
+    import mbl
+
+    cfg = mbl.default_config(lib, LEVEL=1, write_files=0)
+    cfg.mi[0] = 1
+    h = ctypes.c_void_p()
+    if lib.mbl_handle_create(ctypes.byref(h), ctypes.byref(cfg)):
+        return None
+    g = ctypes.c_double()
+    rc = lib.mbl_bandwidth_probe(h, mib << 20, reps, ctypes.byref(g))
+    lib.mbl_handle_destroy(h)
+    return g.value if rc == 0 else None
+
+
 def copy_calibration(mib):
     """Five device-to-device copies of a known size: what FETCH_SIZE / WRITE_SIZE
     read for them calibrates the counters against bytes actually moved."""
@@ -144,6 +162,10 @@ def main():
                      "kernel": "k_stream_collide (periodic)", "per_launch_ms": round(r["per_launch_ms"], 5),
                      "bytes_per_node": bpn},
     }
+    copy = stream_copy_gbs(lib)
+    if copy:
+        out["roofline"]["copy_gbs"] = round(copy, 1)
+        out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
     if args.variants:
         var = {}
         for st in (1, 2, 3, 4, 5):

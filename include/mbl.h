@@ -416,6 +416,12 @@ int mbl_batch_forces(mbl_batch_t b, int nt, double *cd, double *cl);
  * the cases. Returns 0 and the choice and its time per step. */
 int mbl_batch_autotune(mbl_batch_t b, int trial_steps, int *vx, int *tpb, int *groups, double *ms_per_step);
 
+/* Streaming-copy bandwidth (bytes read + written per second, GB/s) of a
+ * hand-written copy kernel over two fresh buffers of `bytes`, timed with HIP
+ * events over `reps` launches: the measured HBM ceiling. No reference
+ * counterpart. */
+int mbl_bandwidth_probe(mbl_handle_t h, size_t bytes, int reps, double *gbs);
+
 /* ---- field output and checkpoints ---- */
 
 /* output/output.h:2-38: writes steady_<n>.dat in out_dir, a Tecplot point file

@@ -1264,6 +1264,33 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     return 0;
 }
 
+extern "C" int mbl_bandwidth_probe(mbl_handle_t h, size_t bytes, int reps, double *gbs)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!gbs || reps <= 0 || bytes < 16) return set_err(h, MBL_E_INVALID);
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return set_err(h, MBL_E_NOMEM);
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        hipFree(a);
+        return set_err(h, MBL_E_NOMEM);
+    }
+    int rc = hipMemsetAsync(a, 1, bytes, h->stream) == hipSuccess ? 0 : MBL_E_HIP;
+    if (!rc) rc = launch_stream_copy(h->stream, a, b, bytes); // warm-up
+    if (!rc && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = MBL_E_HIP;
+    for (int r = 0; r < reps && !rc; r++) rc = launch_stream_copy(h->stream, a, b, bytes);
+    if (!rc && hipEventRecord(h->ev1, h->stream) != hipSuccess) rc = MBL_E_HIP;
+    float ms = 0.f;
+    if (!rc && (hipEventSynchronize(h->ev1) != hipSuccess || hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess))
+        rc = MBL_E_HIP;
+    hipStreamSynchronize(h->stream);
+    hipFree(a);
+    hipFree(b);
+    if (!rc) *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return set_err(h, rc);
+}
+
 extern "C" int mbl_timer_begin(mbl_handle_t h)
 {
     if (!h) return MBL_E_NO_HANDLE;

@@ -130,6 +130,7 @@ int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total
                     int mode, double umax, double LY, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
+int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes);
 int launch_raster(void *stream, const RasterBlock *d_blocks, const RasterCase *d_cases, int ncases, int H, int W,
                   float *d_out);
 int launch_init_raster(void *stream, int mode, const RasterBlock *d_blocks, int nblocks, size_t max_nodes,

@@ -1193,6 +1193,28 @@ __global__ __launch_bounds__(256) void k_init_raster(const RasterBlock *__restri
     }
 }
 
+// Streaming copy (the measured bandwidth ceiling the fused kernel is compared
+// with): 16-byte loads, non-temporal 16-byte stores, 4 per thread, XCD-aware
+// workgroup order as in the fused kernel.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+    typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+    const int G = (int)gridDim.x;
+    const size_t b = (size_t)xcd_tile((int)blockIdx.x, G);
+    const size_t base = b * 1024 + threadIdx.x;
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t i = base + 256 * u;
+        if (i < n16) v[u] = ((const GLOBAL_AS u32x4 *)src)[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t i = base + 256 * u;
+        if (i < n16) __builtin_nontemporal_store(v[u], (GLOBAL_AS u32x4 *)dst + i);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1434,6 +1456,16 @@ int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const do
     hipStream_t s = (hipStream_t)stream;
     dim3 g((unsigned)((n + 255) / 256)), b(256);
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_encode<M_>), g, b, 0, s, plane, k, n, in));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes)
+{
+    const size_t n16 = bytes / 16;
+    if (n16 == 0 || (n16 + 1023) / 1024 > 0x7FFFFFFF) return MBL_E_INVALID;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n16 + 1023) / 1024)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4 *)src, (uint4 *)dst, n16);
     MBL_LAUNCH_CHECK();
     return 0;
 }
