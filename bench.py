@@ -169,7 +169,7 @@ def main():
     if args.variants:
         var = {}
         for st in (1, 2, 3, 4, 5):
-            v = gpu_run(lib, args.n, max(50, args.steps // 4), args.warmup, st)
+            v = gpu_run(lib, args.n, max(200, args.steps // 4), args.warmup, st)
             b = BYTES_PER_NODE[st]
             var[STORAGE_NAMES[st]] = {"mlups": round(v["mlups"], 1), "per_launch_ms": round(v["per_launch_ms"], 5),
                                       "achieved_gbs": round(b * v["nodes"] / (v["per_launch_ms"] * 1e-3) / 1e9, 1),

@@ -162,10 +162,12 @@ enum {
     MBL_STORE_F16C = 4,   /* 16-bit planes holding f - w_k in a 1-4-11 format: sign, 4
                            * exponent bits (bias 15, no infinities), 11 fraction bits;
                            * normal range 2^-14 .. 2 - 2^-10, subnormal step 2^-25,
-                           * round to nearest even, saturating; 0xFFFF is NaN.
+                           * round to nearest even, saturating at the magnitude code
+                           * 0x7FFE (2 - 2^-10); 0xFFFF is NaN.
                            * (FP32 build only) */
     MBL_STORE_F16D = 5    /* as MBL_STORE_F16C in a 1-3-12 format: 3 exponent bits (bias
-                           * 7), 12 fraction bits; normal range 2^-6 .. 2 - 2^-11,
+                           * 7), 12 fraction bits; normal range 2^-6 .. 2 - 2^-11
+                           * (saturating at 2 - 2^-11),
                            * subnormal step 2^-18. (FP32 build only) */
 };
 
@@ -245,6 +247,8 @@ size_t mbl_plane_bytes(mbl_handle_t h, int nx, int ny);
 /* Allocates every plane of a block (solid, 25 state planes in the handle's
  * storage) and the two previous-velocity planes (may be NULL). */
 int mbl_block_alloc(mbl_handle_t h, const AREA *area, D2Q9 *pp, Lbd **upx, Lbd **upy);
+/* (A block's planes may hold at most 4 GiB each: nx*ny*max(element, Lbd) bytes;
+ * larger blocks are rejected when stepped, MBL_E_INVALID / MBL_E_UNSUPPORTED.) */
 void mbl_block_free(D2Q9 *pp, Lbd *upx, Lbd *upy);
 /* Reads plane k of F (which=0) or f (which=1) of a block as f values in double,
  * decoding the storage mode. */

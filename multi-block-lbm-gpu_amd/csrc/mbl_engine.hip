@@ -215,11 +215,10 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
     }
     h->stream = h->own_stream;
     // default launch shape of the fused kernel per plane storage, from launch-shape sweeps on
-    // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 2 for IEEE half and
-    // bfloat16 planes, 1 for the custom 16-bit formats (whose conversions are ALU work), 128
-    // threads per workgroup
+    // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 1 for 2-byte planes,
+    // 128 threads per workgroup
     const int st = storage_mode(h->cfg);
-    h->vx = (st == ST_F16S || st == ST_BF16S) ? 2 : ((st == ST_F16C || st == ST_F16D) ? 1 : 4);
+    h->vx = store_elem_bytes(st) == 2 ? 1 : 4;
     h->tpb = 128;
     if (hipMalloc(&h->d_l2, sizeof(double) * 3 * h->cap) != hipSuccess ||
         hipMalloc(&h->d_drag, sizeof(double) * 2 * h->cap) != hipSuccess ||
@@ -457,6 +456,14 @@ extern "C" int mbl_read_distribution(mbl_handle_t h, const D2Q9 *pp, int which, 
 extern "C" int mbl_write_distribution(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, const double *in)
 {
     return plane_io(h, pp, which, k, n, (double *)in, false);
+}
+
+// The fused kernel addresses planes with 32-bit byte offsets: every plane of a
+// block (distributions and macros) must fit in 4 GiB.
+static bool plane_fits(const mbl_context *h, int nx, int ny)
+{
+    const size_t eb = std::max(store_elem_bytes(storage_mode(h->cfg)), sizeof(Lbd));
+    return (size_t)nx * (size_t)ny * eb <= ((size_t)1 << 32);
 }
 
 // ---------------------------------------------------------------------------
@@ -721,6 +728,7 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) {
             const AREA &ar = A.areas[a];
             if (ar.nx < 2 || ar.ny < 2) return fail("block smaller than 2x2");
+            if (!plane_fits(h, ar.nx, ar.ny)) return fail("block plane larger than 4 GiB");
             for (int k = 0; k < 9; k++)
                 if (!A.pps[a].F[k] || !A.pps[a].f[k] || A.pps[a].F[k] == A.pps[a].f[k]) return fail("F/f planes");
             BlockDesc d = make_desc(cfg, ar, A.pps[a], l, nullptr, nullptr);
@@ -1233,6 +1241,7 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     if (S.sig != sig || !S.dd) {
         release_single(h);
         const bool periodic = (h->cfg.bc_mode == MBL_BC_PERIODIC);
+        if (!plane_fits(h, area->nx, area->ny)) return set_err(h, MBL_E_UNSUPPORTED);
         bool any_solid = true;
         if (validate_flags(h, *area, pp->solid, periodic, &any_solid)) return MBL_E_INVALID;
         S.hd = make_desc(h->cfg, *area, *pp, 0, nullptr, nullptr);

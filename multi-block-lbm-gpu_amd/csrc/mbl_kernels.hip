@@ -39,6 +39,19 @@ template <int MODE> struct Codec;
 // global_load/global_store (SGPR base + VGPR offset) rather than flat accesses.
 #define GLOBAL_AS __attribute__((address_space(1)))
 
+// Element i of a plane through a 32-bit byte offset from the plane's (uniform)
+// base, so loads and stores take the scalar-base + vector-offset form instead
+// of 64-bit per-lane address arithmetic. Planes are at most 4 GiB (checked on
+// the host when a layout is built).
+template <class T> __device__ inline const GLOBAL_AS T *off32(const GLOBAL_AS T *b, uint32_t i)
+{
+    return (const GLOBAL_AS T *)((const GLOBAL_AS char *)b + i * (uint32_t)sizeof(T));
+}
+template <class T> __device__ inline GLOBAL_AS T *off32(GLOBAL_AS T *b, uint32_t i)
+{
+    return (GLOBAL_AS T *)((GLOBAL_AS char *)b + i * (uint32_t)sizeof(T));
+}
+
 template <> struct Codec<ST_NATIVE> {
     typedef Lbd T;
     typedef GLOBAL_AS Lbd GT;
@@ -83,7 +96,7 @@ template <> struct Codec<ST_BF16S> {
 // fraction bits, subnormals below 2^(2 - 2^EB). |f - w_k| stays well inside
 // (2^-14, 2), so the exponent bits IEEE binary16 spends on range are better
 // spent on the significand. Code 0xFFFF is NaN so a diverging run stays
-// visible; overflow saturates. Decoding places the 15 magnitude bits under
+// visible; overflow saturates at magnitude code 0x7FFE (2 - 2^(2-FB)). Decoding places the 15 magnitude bits under
 // the binary32 exponent field and rescales by a power of two (subnormal
 // codes come out as binary32 subnormals and scale exactly); encoding rounds
 // to nearest even on the dropped bits, and in the subnormal range rounds the
@@ -105,13 +118,12 @@ template <int EB> struct Half16 {
         const uint32_t s = (u >> 16) & 0x8000u, a = u & 0x7FFFFFFFu;
         constexpr uint32_t SH = 23 - FB, HALF = (1u << (SH - 1)) - 1u;
         uint32_t nrm = ((a + HALF + ((a >> SH) & 1u)) >> SH) - (OFF << FB);
-        nrm = min(nrm, 0x7FFFu); // saturate (and infinity)
+        nrm = min(nrm, 0x7FFEu); // saturate (and infinity) one code below 0x7FFF, so no sign gives 0xFFFF
         // below 2^(1 - BIAS) the code is |v| 2^(BIAS - 1 + FB) (exact, < 2^FB + 1/2) rounded to the
         // nearest even integer: adding 1.5 * 2^23 rounds it once, in the addition
         const float scaled = __uint_as_float(a) * __uint_as_float((uint32_t)(127 + BIAS - 1 + FB) << 23);
         const uint32_t sub = __float_as_uint(scaled + 12582912.0f) - 0x4B400000u;
-        uint32_t code = (a >= ((OFF + 1u) << 23) ? nrm : sub) | s;
-        code = code == 0xFFFFu ? 0xFFFEu : code;
+        const uint32_t code = (a >= ((OFF + 1u) << 23) ? nrm : sub) | s;
         return (uint16_t)(a > 0x7F800000u ? 0xFFFFu : code);
     }
 };
@@ -403,11 +415,12 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
         const bool copy = !solid && !fluid; // C_BOUNDARY and any other flag: F carried over
         T raw[9];
         real_t q[9];
+        const uint32_t o = (uint32_t)ixy;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             const int id = i - EX[k], jd = j - EY[k];
             const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
-            raw[k] = S[k][own ? ixy : (size_t)jd * nx + id];
+            raw[k] = *off32(S[k], own ? o : (uint32_t)jd * (uint32_t)nx + (uint32_t)id);
             q[k] = C::ld(raw[k]);
         }
         real_t Q[9], rho, ux, uy;
@@ -415,12 +428,12 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             const T v = fluid ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
-            __builtin_nontemporal_store(v, D[k] + ixy);
+            __builtin_nontemporal_store(v, off32(D[k], o));
         }
         if (MACROS && fluid) {
-            __builtin_nontemporal_store((Lbd)rho, (GLOBAL_AS Lbd *)d.rho + ixy);
-            __builtin_nontemporal_store((Lbd)ux, (GLOBAL_AS Lbd *)d.ux + ixy);
-            __builtin_nontemporal_store((Lbd)uy, (GLOBAL_AS Lbd *)d.uy + ixy);
+            __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o));
+            __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o));
+            __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o));
         }
         return;
     }
@@ -449,31 +462,30 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
         xm = (x0 == 0) ? nx - 1 : xm;
         xp = (xp >= nx) ? 0 : xp;
     }
-    const size_t rm = (size_t)jm * nx, rp = (size_t)jp * nx;
+    const uint32_t rm = (uint32_t)jm * (uint32_t)nx, rp = (uint32_t)jp * (uint32_t)nx, rc = (uint32_t)r0;
     real_t q[9][VX];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         // pull: q_k(x) = F_k(x - e_k); EY = +1 reads row j-1, EY = -1 row j+1
-        const size_t rb = (EY[k] == 1) ? rm : ((EY[k] == -1) ? rp : r0);
-        const GT *base = S[k] + rb;
+        const uint32_t rb = (EY[k] == 1) ? rm : ((EY[k] == -1) ? rp : rc);
         if (EX[k] == 0) {
             T tv[VX];
-            vload<T, VX>(base + x0, tv);
+            vload<T, VX>(off32(S[k], rb + x0), tv);
 #pragma unroll
             for (int v = 0; v < VX; v++) q[k][v] = C::ld(tv[v]);
         } else if (EX[k] == 1) {
-            q[k][0] = C::ld(base[xm]);
+            q[k][0] = C::ld(*off32(S[k], rb + xm));
             if constexpr (VX > 1) {
                 T tv[VX];
-                vload<T, VX>(base + x0, tv);
+                vload<T, VX>(off32(S[k], rb + x0), tv);
 #pragma unroll
                 for (int v = 1; v < VX; v++) q[k][v] = C::ld(tv[v - 1]);
             }
         } else {
-            q[k][VX - 1] = C::ld(base[xp]);
+            q[k][VX - 1] = C::ld(*off32(S[k], rb + xp));
             if constexpr (VX > 1) {
                 T tv[VX];
-                vload<T, VX>(base + x0, tv);
+                vload<T, VX>(off32(S[k], rb + x0), tv);
 #pragma unroll
                 for (int v = 0; v < VX - 1; v++) q[k][v] = C::ld(tv[v + 1]);
             }
@@ -494,11 +506,11 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
         my[v] = (Lbd)uy;
     }
 #pragma unroll
-    for (int k = 0; k < 9; k++) vstore<T, VX>(D[k] + r0 + x0, out[k]);
+    for (int k = 0; k < 9; k++) vstore<T, VX>(off32(D[k], rc + x0), out[k]);
     if (MACROS) {
-        vstore<Lbd, VX>((GLOBAL_AS Lbd *)d.rho + r0 + x0, mr);
-        vstore<Lbd, VX>((GLOBAL_AS Lbd *)d.ux + r0 + x0, mx);
-        vstore<Lbd, VX>((GLOBAL_AS Lbd *)d.uy + r0 + x0, my);
+        vstore<Lbd, VX>(off32((GLOBAL_AS Lbd *)d.rho, rc + x0), mr);
+        vstore<Lbd, VX>(off32((GLOBAL_AS Lbd *)d.ux, rc + x0), mx);
+        vstore<Lbd, VX>(off32((GLOBAL_AS Lbd *)d.uy, rc + x0), my);
     }
 }
 

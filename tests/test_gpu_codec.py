@@ -34,14 +34,15 @@ def definition(codes, eb):
 
 def nearest_code(x, eb):
     """Brute force: the code whose value is nearest to x, ties to the even code;
-    beyond the largest value the largest code of that sign."""
-    vals = definition(np.arange(0x7FFF + 1), eb)  # increasing for the positive codes
+    beyond the largest value the largest code of that sign (magnitude code
+    0x7FFE: 0x7FFF is never produced, so no sign gives the NaN code 0xFFFF)."""
+    vals = definition(np.arange(0x7FFE + 1), eb)  # increasing for the positive codes
     out = np.empty(x.size, np.int64)
     for i, xv in enumerate(x):
         a = abs(float(xv))
         j = int(np.searchsorted(vals, a))
         if j >= vals.size:
-            c = 0x7FFF
+            c = 0x7FFE
         elif j == 0:
             c = 0
         else:
@@ -54,7 +55,7 @@ def nearest_code(x, eb):
                 c = j - 1 if (j - 1) % 2 == 0 else j
         if np.signbit(xv):
             c |= 0x8000
-        out[i] = 0xFFFE if c == 0xFFFF else c
+        out[i] = c
     return out
 
 
