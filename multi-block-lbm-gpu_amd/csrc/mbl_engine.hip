@@ -469,6 +469,17 @@ static bool plane_fits(const mbl_context *h, int nx, int ny)
 // ---------------------------------------------------------------------------
 // block descriptors
 // ---------------------------------------------------------------------------
+// Row segments of tw lanes x vx nodes start at multiples of vx of the plane
+// index, so a row whose length is not a multiple of vx is shifted by up to
+// vx - 1 nodes inside its first segment: tiles along x cover nx + vx - 1.
+static int seg_tiles(int nx, int vx, int tw)
+{
+    const int span = tw * vx, ext = (nx % vx) ? vx - 1 : 0;
+    return (nx + ext + span - 1) / span;
+}
+
+static int seg_waste(int nx, int vx, int tw) { return seg_tiles(nx, vx, tw) * tw * vx - nx; }
+
 static BlockDesc make_desc(const mbl_config &cfg, const AREA &a, const D2Q9 &pp, int level, Lbd *upx, Lbd *upy)
 {
     BlockDesc d;
@@ -721,8 +732,9 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     for (int l = 0; l < cfg.LEVEL; l++) {
         L->tile_off[l] = (int)tile_blk.size();
         int vx = h->vx;
-        for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
-            while (vx > 1 && A.areas[a].nx % vx) vx >>= 1;
+        if (L->periodic) // periodic rows wrap, so they hold whole segments
+            for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
+                while (vx > 1 && A.areas[a].nx % vx) vx >>= 1;
         L->vx[l] = vx;
         int tiles = 0;
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) {
@@ -733,14 +745,11 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
                 if (!A.pps[a].F[k] || !A.pps[a].f[k] || A.pps[a].F[k] == A.pps[a].f[k]) return fail("F/f planes");
             BlockDesc d = make_desc(cfg, ar, A.pps[a], l, nullptr, nullptr);
             if (A.bc) d.bc = A.bc[a];
-            // narrow blocks at one node per thread: 32-lane row segments when they waste fewer lanes
+            // 32-lane row segments when they waste fewer lanes than 64-lane ones
             d.tw = 64;
-            if (vx == 1) {
-                const int w64 = (ar.nx + 63) / 64 * 64 - ar.nx, w32 = (ar.nx + 31) / 32 * 32 - ar.nx;
-                if (w32 < w64) d.tw = 32;
-            }
+            if (seg_waste(ar.nx, vx, 32) < seg_waste(ar.nx, vx, 64)) d.tw = 32;
             d.trows = rows * (64 / d.tw);
-            d.tiles_x = (ar.nx + d.tw * vx - 1) / (d.tw * vx);
+            d.tiles_x = seg_tiles(ar.nx, vx, d.tw);
             d.tile_begin = tiles;
             tiles += d.tiles_x * ((ar.ny + d.trows - 1) / d.trows);
             tile_blk.resize(L->tile_off[l] + tiles, a - L->ASP[l]);
@@ -1246,12 +1255,13 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
         if (validate_flags(h, *area, pp->solid, periodic, &any_solid)) return MBL_E_INVALID;
         S.hd = make_desc(h->cfg, *area, *pp, 0, nullptr, nullptr);
         S.vx = h->vx;
-        while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
+        if (periodic)
+            while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
         S.tpb = h->tpb;
         S.rows = S.tpb / 64;
         S.hd.tw = 64;
         S.hd.trows = 0;
-        S.hd.tiles_x = (area->nx + 64 * S.vx - 1) / (64 * S.vx);
+        S.hd.tiles_x = seg_tiles(area->nx, S.vx, 64);
         S.hd.tile_begin = 0;
         S.tiles = S.hd.tiles_x * ((area->ny + S.rows - 1) / S.rows);
         S.periodic = periodic;

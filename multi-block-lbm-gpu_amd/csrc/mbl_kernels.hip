@@ -167,22 +167,36 @@ __device__ inline double feq64(int k, double rh, double ux, double uy)
     return W64[k] * rh * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
 }
 
+// Fused multiply-add in compute precision. The collision below spells out
+// every fused operation and turns the compiler's own contraction off, so its
+// rounding is fixed by the source: the vector and the per-node paths, and
+// every launch shape, produce the same bits.
+__device__ inline float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Equilibrium in the stored domain, compute precision. drho = rho - 1 is
 // passed separately so the shifted form keeps its small magnitude.
+// feq = w rho (1 + 3 eu + 4.5 eu^2 - 1.5 u.u)  (lbm/assit_func.h:2-9)
 template <bool SHIFT>
 __device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t uy, real_t uu)
 {
+#pragma clang fp contract(off)
     const real_t w = (real_t)W64[k];
-    real_t eu = (real_t)EX[k] * ux + (real_t)EY[k] * uy;
-    real_t poly = (real_t)3.0 * eu + (real_t)4.5 * eu * eu - (real_t)1.5 * uu;
-    if (SHIFT) return w * (drho + rho * poly);
-    return w * rho * ((real_t)1.0 + poly);
+    real_t eu = (real_t)0.0; // e_k . u with e_k in {-1, 0, 1}^2: one rounding at most
+    if (EX[k] == 1) eu = ux;
+    if (EX[k] == -1) eu = -ux;
+    if (EY[k] == 1) eu = (EX[k] == 0) ? uy : eu + uy;
+    if (EY[k] == -1) eu = (EX[k] == 0) ? -uy : eu - uy;
+    const real_t poly = fmar(eu, fmar((real_t)4.5, eu, (real_t)3.0), (real_t)-1.5 * uu);
+    if (SHIFT) return w * fmar(rho, poly, drho);
+    return w * fmar(rho, poly, rho);
 }
 
 // BGK on one node: q[9] in, Q[9] out, macros out (evolution_block.h:60-70, 100-111).
 template <bool SHIFT>
 __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
 {
+#pragma clang fp contract(off)
     real_t s = q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7] + q[8];
     real_t jx = q[1] - q[3] + q[5] - q[6] - q[7] + q[8];
     real_t jy = q[2] - q[4] + q[5] + q[6] - q[7] - q[8];
@@ -191,14 +205,16 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
     real_t inv = (real_t)1.0 / rho;
     ux = jx * inv;
     uy = jy * inv;
-    real_t uu = ux * ux + uy * uy;
+    const real_t uu = fmar(ux, ux, uy * uy);
+    // Q = q - omega (q - feq)
 #pragma unroll
-    for (int k = 0; k < 9; k++) Q[k] = q[k] - (q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu)) * omega;
+    for (int k = 0; k < 9; k++) Q[k] = fmar(-omega, q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu), q[k]);
 }
 
 template <bool SHIFT> __device__ inline void equilibrium(real_t rho, real_t ux, real_t uy, real_t (&Q)[9])
 {
-    real_t uu = ux * ux + uy * uy;
+#pragma clang fp contract(off)
+    const real_t uu = fmar(ux, ux, uy * uy);
 #pragma unroll
     for (int k = 0; k < 9; k++) Q[k] = qeq<SHIFT>(k, rho, rho - (real_t)1.0, ux, uy, uu);
 }
@@ -224,6 +240,26 @@ template <class T, int VX, class P> __device__ inline void vload(P p, T (&o)[VX]
         o[0] = p[0];
     } else {
         typedef typename Vec<T, VX>::type V;
+        typedef const GLOBAL_AS V GV;
+        V v = *reinterpret_cast<GV *>(p);
+#pragma unroll
+        for (int i = 0; i < VX; i++) o[i] = v[i];
+    }
+}
+
+// VX consecutive elements from an address aligned to one element only (a row
+// above or below when the row length is not a multiple of VX); the hardware
+// serves it with the same wide load.
+template <class T, int VX> struct VecU {
+    typedef T type __attribute__((ext_vector_type(VX), aligned(sizeof(T))));
+};
+
+template <class T, int VX, class P> __device__ inline void vload_u(P p, T (&o)[VX])
+{
+    if constexpr (VX == 1) {
+        o[0] = p[0];
+    } else {
+        typedef typename VecU<T, VX>::type V;
         typedef const GLOBAL_AS V GV;
         V v = *reinterpret_cast<GV *>(p);
 #pragma unroll
@@ -367,15 +403,65 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
     for (int k = 0; k < 9; k++) D[k][ixy] = S[k][ixy];
 }
 
+// One node of a flagged block in one instruction stream for every flag a
+// block edge or body can carry, so narrow blocks (whose waves all hold edge
+// lanes) do not run a second, scalar path: the lane pulls its 9 sources (its
+// own node for a ghost copy or a pull from outside the block), collides, and
+// keeps the collided, the bounced-back or the copied values by select. Only
+// inlet/outlet nodes and fluid nodes on the block edge (reference edge rule)
+// branch to the general update.
+template <int MODE, bool MACROS>
+__device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
+                                    typename Codec<MODE>::GT *const (&D)[9], int i, int j, const Consts &c,
+                                    real_t omega)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const int nx = d.nx, ny = d.ny;
+    const size_t ixy = (size_t)j * nx + i;
+    const int flag = d.flag[ixy];
+    const bool edge = (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
+    if (flag == BOUNDARY_INLET || flag == BOUNDARY_OUTLET || (flag == FLUID && edge)) {
+        general_node<MODE, false, true, MACROS>(d, S, D, i, j, c, omega);
+        return;
+    }
+    const bool solid = (flag == SOLID_SBB || flag == MBL_SOLID_WALL);
+    const bool fluid = (flag == FLUID);
+    const bool copy = !solid && !fluid; // C_BOUNDARY and any other flag: F carried over
+    T raw[9];
+    real_t q[9];
+    const uint32_t o = (uint32_t)ixy;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int id = i - EX[k], jd = j - EY[k];
+        const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
+        raw[k] = *off32(S[k], own ? o : (uint32_t)jd * (uint32_t)nx + (uint32_t)id);
+        q[k] = C::ld(raw[k]);
+    }
+    real_t Q[9], rho, ux, uy;
+    collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const T v = fluid ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
+        __builtin_nontemporal_store(v, off32(D[k], o));
+    }
+    if (MACROS && fluid) {
+        __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o));
+        __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o));
+        __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o));
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Fused stream-collide. One wave covers 64*VX consecutive nodes of one row;
-// a workgroup covers tile_rows rows. Interior all-fluid runs use aligned
-// VX-wide loads of each source row plus one scalar for the x-shifted planes.
+// Fused stream-collide. One wave covers 64*VX consecutive nodes of one row
+// (or two rows of 32*VX); a workgroup covers tile_rows rows. Interior
+// all-fluid runs use VX-wide loads of each source row plus one scalar for the
+// x-shifted planes; every other node takes the flagged per-node update.
 // ---------------------------------------------------------------------------
 // One row segment of a tile: 64*VX consecutive nodes of row j starting at the
 // wave's first node.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, int x0)
+__device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, int x0)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
@@ -395,46 +481,7 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
     const size_t r0 = (size_t)j * nx;
 
     if constexpr (FLAGS && !PERIODIC && VX == 1) {
-        // One instruction stream for every flag a block edge or body can
-        // carry, so narrow blocks (whose waves all hold edge lanes) do not run
-        // a second, scalar path: each lane pulls its 9 sources (its own node
-        // for a ghost copy or a pull from outside the block), collides, and
-        // keeps the collided, the bounced-back or the copied values by select.
-        // Only inlet/outlet nodes and fluid nodes on the block edge (reference
-        // edge rule) branch to the general update.
-        const int i = x0;
-        const size_t ixy = r0 + i;
-        const int flag = d.flag[ixy];
-        const bool edge = (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
-        if (flag == BOUNDARY_INLET || flag == BOUNDARY_OUTLET || (flag == FLUID && edge)) {
-            general_node<MODE, PERIODIC, FLAGS, MACROS>(d, S, D, i, j, c, omega);
-            return;
-        }
-        const bool solid = (flag == SOLID_SBB || flag == MBL_SOLID_WALL);
-        const bool fluid = (flag == FLUID);
-        const bool copy = !solid && !fluid; // C_BOUNDARY and any other flag: F carried over
-        T raw[9];
-        real_t q[9];
-        const uint32_t o = (uint32_t)ixy;
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const int id = i - EX[k], jd = j - EY[k];
-            const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
-            raw[k] = *off32(S[k], own ? o : (uint32_t)jd * (uint32_t)nx + (uint32_t)id);
-            q[k] = C::ld(raw[k]);
-        }
-        real_t Q[9], rho, ux, uy;
-        collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const T v = fluid ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
-            __builtin_nontemporal_store(v, off32(D[k], o));
-        }
-        if (MACROS && fluid) {
-            __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o));
-            __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o));
-            __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o));
-        }
+        flagged_node<MODE, MACROS>(d, S, D, x0, j, c, omega);
         return;
     }
 
@@ -450,8 +497,16 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
         for (int v = 0; v < VX; v++) fast = fast && (fl[v] == FLUID);
     }
     if (!fast) {
-        for (int v = 0; v < VX; v++)
-            if (x0 + v < nx) general_node<MODE, PERIODIC, FLAGS, MACROS>(d, S, D, x0 + v, j, c, omega);
+        // block edges, non-fluid nodes and the partial segments at the row ends, node by node
+#pragma unroll 1
+        for (int v = 0; v < VX; v++) {
+            const int i = x0 + v;
+            if (i < 0 || i >= nx) continue;
+            if constexpr (FLAGS && !PERIODIC)
+                flagged_node<MODE, MACROS>(d, S, D, i, j, c, omega);
+            else
+                general_node<MODE, PERIODIC, FLAGS, MACROS>(d, S, D, i, j, c, omega);
+        }
         return;
     }
 
@@ -466,29 +521,30 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
     real_t q[9][VX];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        // pull: q_k(x) = F_k(x - e_k); EY = +1 reads row j-1, EY = -1 row j+1
+        // pull: q_k(x) = F_k(x - e_k); EY = +1 reads row j-1, EY = -1 row j+1. The
+        // segment is aligned in its own row; in the rows above and below it is
+        // aligned only when the row length is a multiple of VX (always so in
+        // periodic blocks), else it is read with element alignment.
         const uint32_t rb = (EY[k] == 1) ? rm : ((EY[k] == -1) ? rp : rc);
+        const bool al = PERIODIC || EY[k] == 0;
+        T tv[VX];
+        if (EX[k] == 0 || VX > 1) {
+            if (al)
+                vload<T, VX>(off32(S[k], rb + x0), tv);
+            else
+                vload_u<T, VX>(off32(S[k], rb + x0), tv);
+        }
         if (EX[k] == 0) {
-            T tv[VX];
-            vload<T, VX>(off32(S[k], rb + x0), tv);
 #pragma unroll
             for (int v = 0; v < VX; v++) q[k][v] = C::ld(tv[v]);
         } else if (EX[k] == 1) {
             q[k][0] = C::ld(*off32(S[k], rb + xm));
-            if constexpr (VX > 1) {
-                T tv[VX];
-                vload<T, VX>(off32(S[k], rb + x0), tv);
 #pragma unroll
-                for (int v = 1; v < VX; v++) q[k][v] = C::ld(tv[v - 1]);
-            }
+            for (int v = 1; v < VX; v++) q[k][v] = C::ld(tv[v - 1]);
         } else {
             q[k][VX - 1] = C::ld(*off32(S[k], rb + xp));
-            if constexpr (VX > 1) {
-                T tv[VX];
-                vload<T, VX>(off32(S[k], rb + x0), tv);
 #pragma unroll
-                for (int v = 0; v < VX - 1; v++) q[k][v] = C::ld(tv[v + 1]);
-            }
+            for (int v = 0; v < VX - 1; v++) q[k][v] = C::ld(tv[v + 1]);
         }
     }
     T out[9][VX];
@@ -518,7 +574,7 @@ __device__ inline void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, i
 // rows at a time, the workgroup's waves the next ones in turn. The block comes from the per-tile table when
 // there is one (one scalar load), else from a search of tile_begin.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ inline void sc_tile(const BlockDesc *__restrict__ descs, const int *__restrict__ tile_blk, int ndesc,
+__device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, const int *__restrict__ tile_blk, int ndesc,
                                int tile_rows, uint32_t parity_mask, int t)
 {
     int lo = 0;
@@ -541,11 +597,17 @@ __device__ inline void sc_tile(const BlockDesc *__restrict__ descs, const int *_
     // a wave covers 64 / tw row segments of tw consecutive lanes
     const int tw = d.tw, lane = (int)(threadIdx.x & 63);
     const int per_wave = 64 / tw, sub = lane / tw, col = lane - sub * tw;
-    const int x0 = (txi * tw + col) * VX;
+    const int base = (txi * tw + col) * VX;
     const int rows = d.trows > 0 ? d.trows : tile_rows;
     const int step = (int)(blockDim.x >> 6) * per_wave;
-    for (int r = (int)(threadIdx.x >> 6) * per_wave + sub; r < rows; r += step)
-        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, tyi * rows + r, x0);
+    for (int r = (int)(threadIdx.x >> 6) * per_wave + sub; r < rows; r += step) {
+        const int j = tyi * rows + r;
+        // Segments start at multiples of VX of the plane index, so a row of a
+        // length that is not a multiple of VX begins (j*nx mod VX) nodes into
+        // its first segment; the host sizes tiles_x for that shift.
+        const int x0 = PERIODIC ? base : base - (int)(((uint32_t)j * (uint32_t)d.nx) & (uint32_t)(VX - 1));
+        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, j, x0);
+    }
 }
 
 // One tile per workgroup when the grid covers every tile; otherwise a
@@ -558,14 +620,13 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
 {
     const int G = (int)gridDim.x;
     const int b = xcd_tile((int)blockIdx.x, G);
-    if (G >= total_tiles) {
-        if (b < total_tiles)
-            sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, b);
-        return;
+    int t0 = b, t1 = min(b + 1, total_tiles);
+    if (G < total_tiles) {
+        const int per = (total_tiles + G - 1) / G;
+        t0 = b * per;
+        t1 = min(total_tiles, t0 + per);
     }
-    const int per = (total_tiles + G - 1) / G;
-    const int t0 = b * per;
-    const int t1 = min(total_tiles, t0 + per);
+    // one call site, inlined: the descriptor and plane bases stay in scalar registers
     for (int t = t0; t < t1; t++)
         sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, t);
 }
