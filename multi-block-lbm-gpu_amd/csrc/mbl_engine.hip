@@ -89,6 +89,7 @@ struct Residual {
     BlockDesc *dd = nullptr;
     int total = 0;
     double *partials = nullptr;
+    int *tblk = nullptr; // block of each residual tile
 };
 
 struct PendingDrag {
@@ -105,6 +106,7 @@ struct mbl_context {
     int device = 0;
     int last_error = 0;
     int vx = 4, tpb = 256;
+    bool vx_set = false;  // vx chosen by mbl_set_tuning or the autotuner, not by default
     int groups = 0; // persistent grid size of the stream-collide launches (0: one workgroup per tile)
     Layout *layout = nullptr;
     Single single;
@@ -216,7 +218,8 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
     h->stream = h->own_stream;
     // default launch shape of the fused kernel per plane storage, from launch-shape sweeps on
     // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 1 for 2-byte planes,
-    // 128 threads per workgroup
+    // 128 threads per workgroup. Flagged (non-periodic) blocks default to 1 node/thread
+    // whatever the storage (measured equal on 2049-wide blocks, faster on narrow ones).
     const int st = storage_mode(h->cfg);
     h->vx = store_elem_bytes(st) == 2 ? 1 : 4;
     h->tpb = 128;
@@ -244,8 +247,10 @@ static void release_resid(mbl_context *h)
 {
     if (h->resid.dd) hipFree(h->resid.dd);
     if (h->resid.partials) hipFree(h->resid.partials);
+    if (h->resid.tblk) hipFree(h->resid.tblk);
     h->resid.dd = nullptr;
     h->resid.partials = nullptr;
+    h->resid.tblk = nullptr;
     h->resid.sig = 0;
 }
 
@@ -335,7 +340,10 @@ extern "C" int mbl_set_tuning(mbl_handle_t h, int vx, int tpb)
     if (!h) return MBL_E_NO_HANDLE;
     if (vx && vx != 1 && vx != 2 && vx != 4 && vx != 8) return MBL_E_INVALID;
     if (tpb && (tpb % 64 || tpb < 64 || tpb > 256)) return MBL_E_INVALID;
-    if (vx) h->vx = vx;
+    if (vx) {
+        h->vx = vx;
+        h->vx_set = true;
+    }
     if (tpb) h->tpb = tpb;
     drop_layouts(h);
     return 0;
@@ -731,7 +739,7 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     std::vector<int> tile_blk;
     for (int l = 0; l < cfg.LEVEL; l++) {
         L->tile_off[l] = (int)tile_blk.size();
-        int vx = h->vx;
+        int vx = (h->vx_set || L->periodic) ? h->vx : 1;
         if (L->periodic) // periodic rows wrap, so they hold whole segments
             for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
                 while (vx > 1 && A.areas[a].nx % vx) vx >>= 1;
@@ -1254,7 +1262,7 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
         bool any_solid = true;
         if (validate_flags(h, *area, pp->solid, periodic, &any_solid)) return MBL_E_INVALID;
         S.hd = make_desc(h->cfg, *area, *pp, 0, nullptr, nullptr);
-        S.vx = h->vx;
+        S.vx = (h->vx_set || periodic) ? h->vx : 1;
         if (periodic)
             while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
         S.tpb = h->tpb;
@@ -1566,7 +1574,7 @@ extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *gri
     if (ensure_resid(h, N_areas, areas, pps_d, upsx, upsy)) return;
     Residual &R = h->resid;
     const size_t slot = h->pend_l2.size();
-    int rc = launch_residual(h->stream, R.dd, N_areas, R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + 3 * slot);
+    int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + 3 * slot);
     if (rc) {
         set_err(h, rc);
         return;
@@ -1583,17 +1591,23 @@ static int ensure_resid(mbl_context *h, int N_areas, const AREA *areas, const D2
         release_resid(h);
         R.hd.resize(N_areas);
         int total = 0;
+        std::vector<int> tblk;
         for (int a = 0; a < N_areas; a++) {
+            if (areas[a].nx < 1 || areas[a].ny < 1 || areas[a].m < 1) return set_err(h, MBL_E_INVALID);
             BlockDesc d = make_desc(h->cfg, areas[a], pps_d[a], 0, upsx ? upsx[a] : nullptr, upsy ? upsy[a] : nullptr);
             d.rtiles_x = (areas[a].nx + 63) / 64;
             d.rtile_begin = total;
             total += d.rtiles_x * ((areas[a].ny + 15) / 16);
+            tblk.resize(total, a);
             R.hd[a] = d;
         }
         R.total = total;
         if (hipMalloc(&R.dd, sizeof(BlockDesc) * N_areas) != hipSuccess ||
-            hipMalloc(&R.partials, sizeof(double) * 2 * std::max(1, total)) != hipSuccess ||
-            hipMemcpy(R.dd, R.hd.data(), sizeof(BlockDesc) * N_areas, hipMemcpyHostToDevice) != hipSuccess) {
+            hipMalloc(&R.partials, sizeof(double) * 2 * total) != hipSuccess ||
+            hipMalloc(&R.tblk, sizeof(int) * total) != hipSuccess ||
+            hipMemcpy(R.dd, R.hd.data(), sizeof(BlockDesc) * N_areas, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(R.tblk, tblk.data(), sizeof(int) * total, hipMemcpyHostToDevice) != hipSuccess) {
+            release_resid(h);
             return set_err(h, MBL_E_HIP);
         }
         R.sig = sig;
@@ -1610,7 +1624,8 @@ extern "C" int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 
     Residual &R = h->resid;
     double *d_out = nullptr;
     HIPCHK(h, hipMalloc(&d_out, 3 * sizeof(double)));
-    int rc = launch_residual(h->stream, R.dd, N_areas, R.total, R.partials, 1, h->cfg.U0[0], (double)h->cfg.LY, d_out);
+    int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 1, h->cfg.U0[0], (double)h->cfg.LY,
+                             d_out);
     double v[3] = {0, 0, 0};
     if (!rc && hipMemcpyAsync(v, d_out, sizeof(v), hipMemcpyDeviceToHost, h->stream) != hipSuccess) rc = MBL_E_HIP;
     if (hipStreamSynchronize(h->stream) != hipSuccess) rc = MBL_E_HIP;
@@ -2353,6 +2368,7 @@ extern "C" int mbl_batch_autotune(mbl_batch_t B, int trial_steps, int *vx_out, i
     const int gs[4] = {0, 4 * cu, 8 * cu, 16 * cu};
     double best = 1e30;
     int bv = h->vx, bt = h->tpb, bg = h->groups;
+    h->vx_set = true;
     for (int v : vxs)
         for (int t : tpbs)
             for (int g : gs) {

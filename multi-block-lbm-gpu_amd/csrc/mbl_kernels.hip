@@ -403,53 +403,92 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
     for (int k = 0; k < 9; k++) D[k][ixy] = S[k][ixy];
 }
 
-// One node of a flagged block in one instruction stream for every flag a
-// block edge or body can carry, so narrow blocks (whose waves all hold edge
-// lanes) do not run a second, scalar path: the lane pulls its 9 sources (its
-// own node for a ghost copy or a pull from outside the block), collides, and
-// keeps the collided, the bounced-back or the copied values by select. Only
-// inlet/outlet nodes and fluid nodes on the block edge (reference edge rule)
-// branch to the general update.
+// NR nodes of one column of a flagged block (rows jr[0..NR-1], those >= ny
+// absent), each in one instruction stream for every flag a block edge or body
+// can carry, so narrow blocks (whose waves all hold edge lanes) do not run a
+// second, scalar path: a lane pulls the 9 sources of each node (its own node
+// for a ghost copy or a pull from outside the block), collides, and keeps the
+// collided, the bounced-back or the copied values by select. The loads of all
+// NR nodes are issued before any of them is used, so a lane has NR nodes' worth
+// of memory traffic in flight per round trip. Only inlet/outlet nodes and fluid
+// nodes on the block edge (reference edge rule) branch to the general update.
+template <int MODE, bool MACROS, int NR>
+__device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
+                                    typename Codec<MODE>::GT *const (&D)[9], int i, const int (&jr)[NR],
+                                    const Consts &c, real_t omega)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    const int nx = d.nx, ny = d.ny;
+    int flag[NR];
+    uint32_t o[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        o[r] = (uint32_t)jr[r] * (uint32_t)nx + (uint32_t)i;
+        flag[r] = (jr[r] < ny) ? d.flag[o[r]] : C_BOUNDARY;
+    }
+    T raw[NR][9];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int j = jr[r];
+        const bool copy = !(flag[r] == FLUID || flag[r] == SOLID_SBB || flag[r] == MBL_SOLID_WALL);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int id = i - EX[k], jd = j - EY[k];
+            const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
+            raw[r][k] = (j < ny) ? *off32(S[k], own ? o[r] : (uint32_t)jd * (uint32_t)nx + (uint32_t)id) : T{};
+        }
+    }
+    bool special[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int j = jr[r];
+        const bool edge = (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
+        special[r] = j < ny && (flag[r] == BOUNDARY_INLET || flag[r] == BOUNDARY_OUTLET || (flag[r] == FLUID && edge));
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (jr[r] >= ny || special[r]) continue;
+        const bool solid = (flag[r] == SOLID_SBB || flag[r] == MBL_SOLID_WALL);
+        const bool fluid = (flag[r] == FLUID);
+        real_t q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[k] = C::ld(raw[r][k]);
+        real_t Q[9], rho, ux, uy;
+        collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const T v = fluid ? C::st(Q[k]) : (solid ? raw[r][OPP[k]] : raw[r][k]);
+            __builtin_nontemporal_store(v, off32(D[k], o[r]));
+        }
+        if (MACROS && fluid) {
+            __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o[r]));
+            __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o[r]));
+            __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o[r]));
+        }
+    }
+    // the general update after the others, with their loaded values dead; one
+    // call site (it reads the source planes and writes only its own node)
+    unsigned todo = 0;
+#pragma unroll
+    for (int r = 0; r < NR; r++) todo |= special[r] ? 1u << r : 0u;
+    while (todo) {
+        const int r = __builtin_ctz(todo);
+        todo &= todo - 1;
+        int j = jr[0];
+#pragma unroll
+        for (int q = 1; q < NR; q++) j = (r == q) ? jr[q] : j; // selects, not an indexed (scratch) array
+        general_node<MODE, false, true, MACROS>(d, S, D, i, j, c, omega);
+    }
+}
+
 template <int MODE, bool MACROS>
 __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
                                     typename Codec<MODE>::GT *const (&D)[9], int i, int j, const Consts &c,
                                     real_t omega)
 {
-    typedef Codec<MODE> C;
-    typedef typename C::T T;
-    const int nx = d.nx, ny = d.ny;
-    const size_t ixy = (size_t)j * nx + i;
-    const int flag = d.flag[ixy];
-    const bool edge = (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
-    if (flag == BOUNDARY_INLET || flag == BOUNDARY_OUTLET || (flag == FLUID && edge)) {
-        general_node<MODE, false, true, MACROS>(d, S, D, i, j, c, omega);
-        return;
-    }
-    const bool solid = (flag == SOLID_SBB || flag == MBL_SOLID_WALL);
-    const bool fluid = (flag == FLUID);
-    const bool copy = !solid && !fluid; // C_BOUNDARY and any other flag: F carried over
-    T raw[9];
-    real_t q[9];
-    const uint32_t o = (uint32_t)ixy;
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const int id = i - EX[k], jd = j - EY[k];
-        const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
-        raw[k] = *off32(S[k], own ? o : (uint32_t)jd * (uint32_t)nx + (uint32_t)id);
-        q[k] = C::ld(raw[k]);
-    }
-    real_t Q[9], rho, ux, uy;
-    collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const T v = fluid ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
-        __builtin_nontemporal_store(v, off32(D[k], o));
-    }
-    if (MACROS && fluid) {
-        __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o));
-        __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o));
-        __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o));
-    }
+    const int jr[1] = {j};
+    flagged_rows<MODE, MACROS, 1>(d, S, D, i, jr, c, omega);
 }
 
 // ---------------------------------------------------------------------------
@@ -738,6 +777,10 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
 // ---------------------------------------------------------------------------
 // Interface line ops
 // ---------------------------------------------------------------------------
+// Every load an entry needs is issued before its write mask is tested (all
+// are in bounds for e < n), so an entry costs one dependent memory round trip
+// after the op table (two for CC/FC: index, then source node), not one more
+// for the mask.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
                                                   uint32_t parity_mask, int m0, int m1, int m2, double xm)
@@ -747,10 +790,9 @@ __global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops
     const LineOp &op = ops[blockIdx.y];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= op.n) return;
-    const uint8_t mask = op.mask[e];
-    if (!mask) return;
     const BlockDesc &db = blocks[op.dst_block];
     const int pd = (parity_mask >> db.level) & 1;
+    const uint8_t mask = op.mask[e];
     const int didx = op.dst_idx[e];
     T *D[9];
 #pragma unroll
@@ -760,39 +802,45 @@ __global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops
         const double a = 0.5 * (xm - 1.0) * xm;
         const double b = (xm - 1.0) * (xm + 1.0);
         const double cc = 0.5 * xm * (xm + 1.0);
+        double v[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            double v = a * (double)op.ring[m0][k][e] - b * (double)op.ring[m1][k][e] + cc * (double)op.ring[m2][k][e];
-            D[k][didx] = enc64<MODE>(v, k);
-        }
+        for (int k = 0; k < 9; k++)
+            v[k] = a * (double)op.ring[m0][k][e] - b * (double)op.ring[m1][k][e] + cc * (double)op.ring[m2][k][e];
+        if (!mask) return;
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k][didx] = enc64<MODE>(v[k], k);
         return;
     }
     if (op.type == OP_CF) {
+        Lbd v[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) D[k][didx] = enc64<MODE>((double)op.ring[m2][k][e], k);
+        for (int k = 0; k < 9; k++) v[k] = op.ring[m2][k][e];
+        if (!mask) return;
+#pragma unroll
+        for (int k = 0; k < 9; k++) D[k][didx] = enc64<MODE>((double)v[k], k);
         return;
     }
     const BlockDesc &sb = blocks[op.src_block];
     const int ps = (parity_mask >> sb.level) & 1;
     const int sidx = op.src_idx[e];
-    const T *S[9];
+    T sv[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) S[k] = (const T *)sb.P[ps][k];
+    for (int k = 0; k < 9; k++) sv[k] = ((const T *)sb.P[ps][k])[sidx];
+    const Lbd r = sb.rho[sidx], ux = sb.ux[sidx], uy = sb.uy[sidx];
     if (mask & 1) {
-        db.rho[didx] = sb.rho[sidx];
-        db.ux[didx] = sb.ux[sidx];
-        db.uy[didx] = sb.uy[sidx];
+        db.rho[didx] = r;
+        db.ux[didx] = ux;
+        db.uy[didx] = uy;
     }
     if (!(mask & 2)) return;
     if (op.type == OP_CC) {
 #pragma unroll
-        for (int k = 0; k < 9; k++) D[k][didx] = S[k][sidx];
+        for (int k = 0; k < 9; k++) D[k][didx] = sv[k];
     } else { // OP_FC: non-equilibrium rescaling fine -> coarse (assit_multi.h:304-311)
-        const double r = (double)sb.rho[sidx], ux = (double)sb.ux[sidx], uy = (double)sb.uy[sidx];
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            double ft = dec64<MODE>(S[k][sidx], k);
-            double fe = feq64(k, r, ux, uy);
+            const double ft = dec64<MODE>(sv[k], k);
+            const double fe = feq64(k, (double)r, (double)ux, (double)uy);
             D[k][didx] = enc64<MODE>(fe + op.para * (ft - fe), k);
         }
     }
@@ -974,6 +1022,120 @@ __global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ 
     }
 }
 
+// The whole spline step of one line in one workgroup (lines up to
+// SPL_FUSED_MAX_NC coarse points): gather the 12 columns (init_Spline3,
+// assit_multi.h:48-116), solve tridiag(l, m, u) M = rhs for all 12 by PCR at
+// once -- the matrix is shared, so each row's elimination factors are computed
+// once and applied to 12 right-hand sides -- and evaluate the fine points
+// (calculate_Spline3_gpu, :136-185) from the columns and M held in LDS. Same
+// operations, in the same order per column, as k_spline_solve + k_spline_eval.
+// One coarse row per thread; blockDim.x >= nc.
+constexpr int SPL_FUSED_MAX_NC = 303; // 27 * 8 * nc bytes of LDS within 64 KiB
+
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict__ ops,
+                                                       const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
+                                                       int slot)
+{
+    typedef typename Codec<MODE>::T T;
+    const LineOp &op = ops[blockIdx.y];
+    const int N = op.nc;
+    const BlockDesc &sb = blocks[op.src_block];
+    const int ps = (parity_mask >> sb.level) & 1;
+    extern __shared__ double lds[]; // G[12][N] | a[N] | b[N] | c[N] | d[12][N]
+    double *G = lds, *La = lds + 12 * N, *Lb = La + N, *Lc = Lb + N, *Ld = Lc + N;
+    const int ip = threadIdx.x;
+    const bool row = ip < N;
+    double a = 0.0, b = 1.0, c = 0.0, d[12];
+    if (row) {
+        const int node = op.src_idx[ip];
+        double g[12];
+        g[0] = (double)sb.rho[node];
+        g[1] = (double)sb.ux[node];
+        g[2] = (double)sb.uy[node];
+#pragma unroll
+        for (int k = 0; k < 9; k++) g[3 + k] = dec64<MODE>(((const T *)sb.P[ps][k])[node], k);
+#pragma unroll
+        for (int q = 0; q < 12; q++) {
+            G[q * N + ip] = g[q];
+            op.gcol[q][ip] = (Lbd)g[q];
+        }
+        a = (double)op.tl[ip];
+        b = (double)op.tm[ip];
+        c = (double)op.tu[ip];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 12; q++)
+        d[q] = (!row || ip == 0 || ip == N - 1)
+                   ? 0.0
+                   : (6.0 * G[q * N + ip] - 3.0 * G[q * N + ip - 1] - 3.0 * G[q * N + ip + 1]);
+    for (int s = 1; s < N; s <<= 1) {
+        __syncthreads();
+        if (row) {
+            La[ip] = a;
+            Lb[ip] = b;
+            Lc[ip] = c;
+#pragma unroll
+            for (int q = 0; q < 12; q++) Ld[q * N + ip] = d[q];
+        }
+        __syncthreads();
+        if (row) {
+            const int im = ip - s, iq = ip + s;
+            double na = 0.0, nc = 0.0, nb = b;
+            if (im >= 0) {
+                const double alpha = -a / Lb[im];
+                na = alpha * La[im];
+                nb += alpha * Lc[im];
+#pragma unroll
+                for (int q = 0; q < 12; q++) d[q] += alpha * Ld[q * N + im];
+            }
+            if (iq < N) {
+                const double gamma = -c / Lb[iq];
+                nc = gamma * Lc[iq];
+                nb += gamma * La[iq];
+#pragma unroll
+                for (int q = 0; q < 12; q++) d[q] += gamma * Ld[q * N + iq];
+            }
+            a = na;
+            b = nb;
+            c = nc;
+        }
+    }
+    __syncthreads();
+    if (row) {
+#pragma unroll
+        for (int q = 0; q < 12; q++) {
+            const double M = d[q] / b;
+            Ld[q * N + ip] = M;
+            op.varibs[(size_t)q * N + ip] = (Lbd)M;
+        }
+    }
+    __syncthreads();
+    const double *M = Ld;
+    for (int f = threadIdx.x; f < op.n; f += blockDim.x) {
+        const double xx = (double)f / (double)op.ratio;
+        const int I = (ceil(xx) == 0.0) ? (int)(ceil(xx) + 1.0) : (int)ceil(xx);
+        const double rho = spline3_s(xx, I - 1.0, 1.0, G[I - 1], G[I], M[I - 1], M[I]);
+        const double ux = spline3_s(xx, I - 1.0, 1.0, G[N + I - 1], G[N + I], M[N + I - 1], M[N + I]);
+        const double uy = spline3_s(xx, I - 1.0, 1.0, G[2 * N + I - 1], G[2 * N + I], M[2 * N + I - 1], M[2 * N + I]);
+        if (op.mask[f] & 1) {
+            const BlockDesc &db = blocks[op.dst_block];
+            const int idx = op.dst_idx[f];
+            db.rho[idx] = (Lbd)rho;
+            db.ux[idx] = (Lbd)ux;
+            db.uy[idx] = (Lbd)uy;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int o = (3 + k) * N;
+            const double ft = spline3_s(xx, I - 1.0, 1.0, G[o + I - 1], G[o + I], M[o + I - 1], M[o + I]);
+            const double fe = feq64(k, rho, ux, uy);
+            op.ring[slot][k][f] = (Lbd)(fe + op.para * (ft - fe));
+        }
+    }
+}
+
 // init_tri_A (multi/init_multi.h:30-51), final race-free state
 __global__ void k_tri_A(Lbd *l, Lbd *m, Lbd *u, int N)
 {
@@ -1059,25 +1221,21 @@ template <int MODE> __global__ __launch_bounds__(256) void k_drag_batch(const Dr
     }
 }
 
-// error_L2_area (error_L2.h:3-55): tiles of 64 x 16 nodes, FP64 partial sums
-// of |u - u_prev|^2 and |u|^2 per tile, u_prev <- u.
-// mode 0: error_L2_area (error_L2.h:3-55); mode 1: L2 distance of (ux, uy) on
-// FLUID nodes from the analytic plane Poiseuille profile
-// u_a(y) = 4 umax (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2, without touching u_prev.
-__global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restrict__ descs, int ndesc, double *partials,
-                                                        int mode, double umax, double LY)
+// error_L2_area (error_L2.h:3-55): workgroup t takes tile t of 64 x 16 nodes
+// of block tblk[t]: FP64 sums of |u - u_prev|^2 and |u|^2 over the tile,
+// u_prev <- u, weighted by the block's 1/m (exact for the power-of-two ratios
+// of nested blocks, so the sum over tiles equals the reference's per-block
+// sums divided by m, error_L2.h:64-84, up to summation order).
+// mode 0: error_L2; mode 1: L2 distance of (ux, uy) on FLUID nodes from the
+// analytic plane Poiseuille profile u_a(y) = 4 umax (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2,
+// without touching u_prev.
+__global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restrict__ descs,
+                                                        const int *__restrict__ tblk, double *partials, int mode,
+                                                        double umax, double LY)
 {
     __shared__ double sh[16];
     const int t = blockIdx.x;
-    int lo = 0, hi = ndesc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (descs[mid].rtile_begin <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const BlockDesc &d = descs[lo];
+    const BlockDesc &d = descs[tblk[t]];
     const int lt = t - d.rtile_begin;
     const int ty = lt / d.rtiles_x, tx = lt - ty * d.rtiles_x;
     const int i = tx * 64 + (threadIdx.x & 63);
@@ -1105,32 +1263,22 @@ __global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restr
     const double a1 = group_sum(s1, sh);
     const double a2 = group_sum(s2, sh);
     if (threadIdx.x == 0) {
-        partials[2 * (size_t)t] = a1;
-        partials[2 * (size_t)t + 1] = a2;
+        partials[2 * (size_t)t] = a1 * d.inv_m;
+        partials[2 * (size_t)t + 1] = a2 * d.inv_m;
     }
 }
 
-// error_L2.h:279-302 host sum: per block in block order, weighted 1/m.
-__global__ __launch_bounds__(256) void k_residual_final(const BlockDesc *__restrict__ descs, int ndesc, int total,
-                                                        const double *partials, double *out)
+// The weighted tile sums in one fixed order: result sqrt(s1)/sqrt(s2), s1, s2.
+__global__ __launch_bounds__(1024) void k_residual_final(int total, const double *__restrict__ partials, double *out)
 {
     __shared__ double sh[16];
-    double all1 = 0.0, all2 = 0.0;
-    for (int a = 0; a < ndesc; a++) {
-        const int b0 = descs[a].rtile_begin;
-        const int b1 = (a + 1 < ndesc) ? descs[a + 1].rtile_begin : total;
-        double s1 = 0.0, s2 = 0.0;
-        for (int t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
-            s1 += partials[2 * (size_t)t];
-            s2 += partials[2 * (size_t)t + 1];
-        }
-        const double g1 = group_sum(s1, sh);
-        const double g2 = group_sum(s2, sh);
-        if (threadIdx.x == 0) {
-            all1 += g1 / descs[a].m;
-            all2 += g2 / descs[a].m;
-        }
+    double g1 = 0.0, g2 = 0.0;
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+        g1 += partials[2 * (size_t)q];
+        g2 += partials[2 * (size_t)q + 1];
     }
+    const double all1 = group_sum(g1, sh);
+    const double all2 = group_sum(g2, sh);
     if (threadIdx.x == 0) {
         out[0] = sqrt(all1) / sqrt(all2);
         out[1] = all1;
@@ -1461,6 +1609,16 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
 {
     if (nops <= 0) return 0;
     if (max_nc > SPL_MAX_NC) return MBL_E_UNSUPPORTED;
+    if (max_nc <= SPL_FUSED_MAX_NC) {
+        int threads = 64;
+        while (threads < max_nc) threads <<= 1;
+        const size_t lds = 27 * sizeof(double) * (size_t)max_nc;
+        hipStream_t s = (hipStream_t)stream;
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_>), dim3(1, nops), dim3(threads), lds, s, d_ops,
+                                                 d_blocks, parity_mask, slot));
+        MBL_LAUNCH_CHECK();
+        return 0;
+    }
     int R = 1;
     while (R * SPL_TPB < max_nc) R <<= 1;
     const size_t lds = 4 * sizeof(double) * (size_t)max_nc;
@@ -1483,15 +1641,15 @@ int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N)
     return 0;
 }
 
-int launch_residual(void *stream, const BlockDesc *d_descs, int ndesc, int total_rtiles, double *partials,
+int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, int total_rtiles, double *partials,
                     int mode, double umax, double LY, double *d_out)
 {
+    if (total_rtiles <= 0) return MBL_E_INVALID;
     hipStream_t s = (hipStream_t)stream;
-    if (total_rtiles > 0)
-        hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, ndesc, partials, mode, umax,
-                           LY);
+    hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, d_tblk, partials, mode, umax,
+                       LY);
     MBL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, s, d_descs, ndesc, total_rtiles, partials, d_out);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, total_rtiles, (const double *)partials, d_out);
     MBL_LAUNCH_CHECK();
     return 0;
 }
