@@ -304,6 +304,22 @@ def convergence(cl_c, cl_w, cd_w):
     return np.array(tc), np.array(tw), fallback
 
 
+def early_overshoot(cl_c, cl_w, periods=(1, 2)):
+    """Median over cases of the warm run's Cl amplitude after the given numbers of
+    shedding periods, relative to the cold run's converged amplitude."""
+    res = {p: [] for p in periods}
+    for c in range(cl_c.shape[0]):
+        T_c = shedding_period(cl_c[c])
+        if T_c < 8:
+            continue
+        a_ref = float(np.mean(amplitude(cl_c[c], T_c)[-2 * T_c:]))
+        a_w = amplitude(cl_w[c], T_c)
+        for p in periods:
+            if p * T_c < a_w.size and a_ref > 0:
+                res[p].append(a_w[p * T_c] / a_ref)
+    return {f"amp_ratio_after_{p}T": round(float(np.median(v)), 3) for p, v in res.items() if v}
+
+
 def summary(out, tc, tw, fallback, ms_step, t_inf):
     both = (tc > 0) & (tw > 0)
     out.update({
@@ -381,6 +397,9 @@ def main(argv=None):
     ap.add_argument("--diagnose", action="store_true",
                     help="also restart from hybrids of the network's field and each case's own final cold state "
                          "at the same phase (exact density + network velocity, and the reverse)")
+    ap.add_argument("--fluct-scale", default=None,
+                    help="diagnostic: comma-separated factors a; also runs warm starts from mean + a (field - mean), "
+                         "the mean being the network's average over 8 phases")
     ap.add_argument("--rho", default="net", choices=["net", "poisson"],
                     help="warm-start density: the network's or the pressure Poisson solution of its velocity")
     args = ap.parse_args(argv)
@@ -484,6 +503,24 @@ def main(argv=None):
     out["steps_run"] = args.steps
     out["mlups"] = round(nu * args.steps / wall_c / 1e6, 1)
     summary(out, tc, tw, fallback, ms_step, t_inf)
+    out.update(early_overshoot(cl_c, cl_w))
+    if args.fluct_scale:
+        phs = np.linspace(0.0, 2 * np.pi, 8, endpoint=False)
+        mean = np.mean([predict(net, args.model, te, H, W, p) for p in phs], axis=0)
+        fsd = {}
+        for a in [float(x) for x in args.fluct_scale.split(",")]:
+            init_a = from_target(mean + a * (pred - mean), te).astype(np.float32)
+            cd_a, cl_a, _, _, _ = run(lib, te, args.steps, storage, init=init_a, log=f"fluctuation x{a}")
+            tca, twa, fba = convergence(cl_c, cl_a, cd_a)
+            both_a = (tca > 0) & (twa > 0)
+            r = {"median_steps": float(np.median(twa[both_a])) if both_a.any() else None, "fallbacks": fba}
+            if both_a.any():
+                r["speedup_median_steps"] = round(float(np.median(tca[both_a]) / np.median(twa[both_a])), 3)
+                r["speedup_total_steps"] = round(float(tca[both_a].sum() / twa[both_a].sum()), 3)
+            r.update(early_overshoot(cl_c, cl_a))
+            fsd[str(a)] = r
+            print(f"# fluctuation x{a}: {r}", file=sys.stderr, flush=True)
+        out["fluct_scale"] = fsd
     if args.diagnose:
         # phase of each case's final cold state: linear from its last upward zero crossing over one period
         ph = np.zeros(len(te))
