@@ -59,8 +59,9 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
 
 
 def stream_copy_gbs(lib, mib=4096, reps=10):
-    """Streaming-copy rate (bytes read + written per second) of the library's copy kernel over MIB MiB:
-    the measured ceiling the fused kernel is compared with besides the 8 TB/s datasheet peak."""
+    """Streaming-copy rates (bytes read + written per second) of the library's copy kernels over
+    MIB MiB: one stream (the HBM ceiling the fused kernel is compared with besides the 8 TB/s
+    datasheet peak) and 9 planes read + 9 written (the fused kernel's own access pattern)."""
     import ctypes
 
     import mbl
@@ -69,11 +70,12 @@ def stream_copy_gbs(lib, mib=4096, reps=10):
     cfg.mi[0] = 1
     h = ctypes.c_void_p()
     if lib.mbl_handle_create(ctypes.byref(h), ctypes.byref(cfg)):
-        return None
-    g = ctypes.c_double()
+        return None, None
+    g, g9 = ctypes.c_double(), ctypes.c_double()
     rc = lib.mbl_bandwidth_probe(h, mib << 20, reps, ctypes.byref(g))
+    rc9 = lib.mbl_bandwidth_probe_planes(h, (mib << 20) // 9, reps, ctypes.byref(g9))
     lib.mbl_handle_destroy(h)
-    return g.value if rc == 0 else None
+    return (g.value if rc == 0 else None), (g9.value if rc9 == 0 else None)
 
 
 def copy_calibration(mib):
@@ -162,10 +164,13 @@ def main():
                      "kernel": "k_stream_collide (periodic)", "per_launch_ms": round(r["per_launch_ms"], 5),
                      "bytes_per_node": bpn},
     }
-    copy = stream_copy_gbs(lib)
+    copy, copy9 = stream_copy_gbs(lib)
     if copy:
         out["roofline"]["copy_gbs"] = round(copy, 1)
         out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
+    if copy9:
+        out["roofline"]["copy_9planes_gbs"] = round(copy9, 1)
+        out["roofline"]["frac_of_copy_9planes"] = round(achieved / copy9, 4)
     if args.variants:
         var = {}
         for st in (1, 2, 3, 4, 5):

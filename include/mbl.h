@@ -426,6 +426,11 @@ int mbl_batch_autotune(mbl_batch_t b, int trial_steps, int *vx, int *tpb, int *g
  * counterpart. */
 int mbl_bandwidth_probe(mbl_handle_t h, size_t bytes, int reps, double *gbs);
 
+/* The same for the fused kernel's access pattern: 9 planes of `plane_bytes`
+ * read and 9 written per launch (one 16-byte element per thread and plane).
+ * No reference counterpart. */
+int mbl_bandwidth_probe_planes(mbl_handle_t h, size_t plane_bytes, int reps, double *gbs);
+
 /* ---- field output and checkpoints ---- */
 
 /* output/output.h:2-38: writes steady_<n>.dat in out_dir, a Tecplot point file

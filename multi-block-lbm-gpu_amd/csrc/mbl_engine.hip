@@ -1291,22 +1291,25 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     return 0;
 }
 
-extern "C" int mbl_bandwidth_probe(mbl_handle_t h, size_t bytes, int reps, double *gbs)
+// Times `reps` launches of a copy between two fresh buffers; the copy is
+// `planes` planes of `plane_bytes` each (1: one stream), GB/s of bytes read + written.
+static int copy_probe(mbl_context *h, size_t plane_bytes, int planes, int reps, double *gbs)
 {
-    if (!h) h = g_current;
-    if (!h) return MBL_E_NO_HANDLE;
-    if (!gbs || reps <= 0 || bytes < 16) return set_err(h, MBL_E_INVALID);
-    bytes &= ~(size_t)15;
+    const size_t bytes = plane_bytes * (size_t)planes;
     void *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, bytes) != hipSuccess) return set_err(h, MBL_E_NOMEM);
     if (hipMalloc(&b, bytes) != hipSuccess) {
         hipFree(a);
         return set_err(h, MBL_E_NOMEM);
     }
+    auto launch = [&]() {
+        return planes == 1 ? launch_stream_copy(h->stream, a, b, plane_bytes)
+                           : launch_plane_copy(h->stream, a, b, plane_bytes);
+    };
     int rc = hipMemsetAsync(a, 1, bytes, h->stream) == hipSuccess ? 0 : MBL_E_HIP;
-    if (!rc) rc = launch_stream_copy(h->stream, a, b, bytes); // warm-up
+    for (int w = 0; w < 2 && !rc; w++) rc = launch(); // warm-up
     if (!rc && hipEventRecord(h->ev0, h->stream) != hipSuccess) rc = MBL_E_HIP;
-    for (int r = 0; r < reps && !rc; r++) rc = launch_stream_copy(h->stream, a, b, bytes);
+    for (int r = 0; r < reps && !rc; r++) rc = launch();
     if (!rc && hipEventRecord(h->ev1, h->stream) != hipSuccess) rc = MBL_E_HIP;
     float ms = 0.f;
     if (!rc && (hipEventSynchronize(h->ev1) != hipSuccess || hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess))
@@ -1316,6 +1319,22 @@ extern "C" int mbl_bandwidth_probe(mbl_handle_t h, size_t bytes, int reps, doubl
     hipFree(b);
     if (!rc) *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
     return set_err(h, rc);
+}
+
+extern "C" int mbl_bandwidth_probe(mbl_handle_t h, size_t bytes, int reps, double *gbs)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!gbs || reps <= 0 || bytes < 16) return set_err(h, MBL_E_INVALID);
+    return copy_probe(h, bytes & ~(size_t)15, 1, reps, gbs);
+}
+
+extern "C" int mbl_bandwidth_probe_planes(mbl_handle_t h, size_t plane_bytes, int reps, double *gbs)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!gbs || reps <= 0 || plane_bytes < 16) return set_err(h, MBL_E_INVALID);
+    return copy_probe(h, plane_bytes & ~(size_t)15, MBL_Q, reps, gbs);
 }
 
 extern "C" int mbl_timer_begin(mbl_handle_t h)

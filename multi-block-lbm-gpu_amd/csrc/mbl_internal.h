@@ -132,6 +132,8 @@ int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, i
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
 int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes);
+// 9 planes of plane_bytes (consecutive in src and in dst) copied plane by plane
+int launch_plane_copy(void *stream, const void *src, void *dst, size_t plane_bytes);
 int launch_raster(void *stream, const RasterBlock *d_blocks, const RasterCase *d_cases, int ncases, int H, int W,
                   float *d_out);
 int launch_init_raster(void *stream, int mode, const RasterBlock *d_blocks, int nblocks, size_t max_nodes,

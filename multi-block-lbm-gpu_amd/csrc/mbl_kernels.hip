@@ -1417,23 +1417,32 @@ __global__ __launch_bounds__(256) void k_init_raster(const RasterBlock *__restri
 // Streaming copy (the measured bandwidth ceiling the fused kernel is compared
 // with): 16-byte loads, non-temporal 16-byte stores, 4 per thread, XCD-aware
 // workgroup order as in the fused kernel.
+// Streaming-copy probes (no reference counterpart), one 16-byte element per
+// thread and stream, non-temporal loads and stores, XCD-aware workgroup order:
+// the fastest of the copy shapes measured on MI355X (4 or 8 elements per
+// thread, plain stores, a linear order and grid-stride loops were all slower).
+// k_stream_copy moves one stream; k_plane_copy moves S planes of m elements
+// into S planes (the fused kernel's pattern: 9 planes read, 9 written).
 __global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
 {
     typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
-    const int G = (int)gridDim.x;
-    const size_t b = (size_t)xcd_tile((int)blockIdx.x, G);
-    const size_t base = b * 1024 + threadIdx.x;
-    u32x4 v[4];
+    const size_t i = (size_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * 256 + threadIdx.x;
+    if (i >= n16) return;
+    const u32x4 v = __builtin_nontemporal_load((const GLOBAL_AS u32x4 *)src + i);
+    __builtin_nontemporal_store(v, (GLOBAL_AS u32x4 *)dst + i);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_plane_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t m16)
+{
+    typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+    const size_t i = (size_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * 256 + threadIdx.x;
+    if (i >= m16) return;
+    u32x4 v[S];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const size_t i = base + 256 * u;
-        if (i < n16) v[u] = ((const GLOBAL_AS u32x4 *)src)[i];
-    }
+    for (int s = 0; s < S; s++) v[s] = __builtin_nontemporal_load((const GLOBAL_AS u32x4 *)src + s * m16 + i);
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const size_t i = base + 256 * u;
-        if (i < n16) __builtin_nontemporal_store(v[u], (GLOBAL_AS u32x4 *)dst + i);
-    }
+    for (int s = 0; s < S; s++) __builtin_nontemporal_store(v[s], (GLOBAL_AS u32x4 *)dst + s * m16 + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1694,9 +1703,19 @@ int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const do
 int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes)
 {
     const size_t n16 = bytes / 16;
-    if (n16 == 0 || (n16 + 1023) / 1024 > 0x7FFFFFFF) return MBL_E_INVALID;
-    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n16 + 1023) / 1024)), dim3(256), 0, (hipStream_t)stream,
+    if (n16 == 0 || (n16 + 255) / 256 > 0x7fffffffu) return MBL_E_INVALID;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        (const uint4 *)src, (uint4 *)dst, n16);
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_plane_copy(void *stream, const void *src, void *dst, size_t plane_bytes)
+{
+    const size_t m16 = plane_bytes / 16;
+    if (m16 == 0 || (m16 + 255) / 256 > 0x7fffffffu) return MBL_E_INVALID;
+    hipLaunchKernelGGL(k_plane_copy<MBL_Q>, dim3((unsigned)((m16 + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4 *)src, (uint4 *)dst, m16);
     MBL_LAUNCH_CHECK();
     return 0;
 }

@@ -99,6 +99,7 @@ def load_library(fp64=False):
         "mbl_batch_raster": (ip, [vp, ip, ip, fp]),
         "mbl_batch_init_from_raster": (ip, [vp, ip, ip, fp]),
         "mbl_bandwidth_probe": (ip, [vp, C.c_size_t, ip, dp]),
+        "mbl_bandwidth_probe_planes": (ip, [vp, C.c_size_t, ip, dp]),
         "output": (None, [ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(D2Q9)]),
         "mbl_output": (ip, [vp, ip, ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(D2Q9)]),
         "mbl_checkpoint_save": (ip, [vp, C.c_char_p, C.c_longlong, ip, C.POINTER(AREA), C.POINTER(D2Q9),

@@ -124,5 +124,5 @@ EXPORTS = [
     "mbl_batch_autotune", "mbl_batch_block", "mbl_batch_raster", "mbl_batch_init_from_raster",
     "mbl_net_create", "mbl_net_destroy", "mbl_net_num_params", "mbl_net_params", "mbl_net_forward",
     "mbl_net_loss_grad", "mbl_net_train", "output", "mbl_output", "mbl_checkpoint_save", "mbl_checkpoint_load",
-    "mbl_bandwidth_probe",
+    "mbl_bandwidth_probe", "mbl_bandwidth_probe_planes",
 ]
