@@ -117,12 +117,16 @@ template <int EB> struct Half16 {
         const uint32_t u = __float_as_uint(v);
         const uint32_t s = (u >> 16) & 0x8000u, a = u & 0x7FFFFFFFu;
         constexpr uint32_t SH = 23 - FB, HALF = (1u << (SH - 1)) - 1u;
-        uint32_t nrm = ((a + HALF + ((a >> SH) & 1u)) >> SH) - (OFF << FB);
+        // round to nearest even on the SH dropped bits and remove the exponent offset before the
+        // shift (OFF << 23 has no bits below SH): one 3-input add and one shift. Below the normal
+        // range the sum wraps, but the subnormal code is selected there.
+        uint32_t nrm = (a + ((a >> SH) & 1u) + (HALF - (OFF << 23))) >> SH;
         nrm = min(nrm, 0x7FFEu); // saturate (and infinity) one code below 0x7FFF, so no sign gives 0xFFFF
         // below 2^(1 - BIAS) the code is |v| 2^(BIAS - 1 + FB) (exact, < 2^FB + 1/2) rounded to the
-        // nearest even integer: adding 1.5 * 2^23 rounds it once, in the addition
-        const float scaled = __uint_as_float(a) * __uint_as_float((uint32_t)(127 + BIAS - 1 + FB) << 23);
-        const uint32_t sub = __float_as_uint(scaled + 12582912.0f) - 0x4B400000u;
+        // nearest even integer: adding 1.5 * 2^23 rounds it once, in the fused multiply-add
+        const float scaled = __builtin_fmaf(__uint_as_float(a), __uint_as_float((uint32_t)(127 + BIAS - 1 + FB) << 23),
+                                            12582912.0f);
+        const uint32_t sub = __float_as_uint(scaled) - 0x4B400000u;
         const uint32_t code = (a >= ((OFF + 1u) << 23) ? nrm : sub) | s;
         return (uint16_t)(a > 0x7F800000u ? 0xFFFFu : code);
     }
@@ -167,6 +171,17 @@ __device__ inline double feq64(int k, double rh, double ux, double uy)
     return W64[k] * rh * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
 }
 
+// 1/x: the hardware reciprocal estimate refined by one Newton-Raphson step,
+// r' = r + r (1 - x r), which squares the estimate's relative error (within an
+// ulp of the correctly rounded quotient, in 3 instructions instead of the ~10
+// of the IEEE division). FP64 keeps the IEEE division.
+__device__ inline float recip(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+__device__ inline double recip(double x) { return 1.0 / x; }
+
 // Fused multiply-add in compute precision. The collision below spells out
 // every fused operation and turns the compiler's own contraction off, so its
 // rounding is fixed by the source: the vector and the per-node paths, and
@@ -197,16 +212,22 @@ template <bool SHIFT>
 __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
 {
 #pragma clang fp contract(off)
-    real_t s = q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6] + q[7] + q[8];
-    real_t jx = q[1] - q[3] + q[5] - q[6] - q[7] + q[8];
-    real_t jy = q[2] - q[4] + q[5] + q[6] - q[7] - q[8];
+    // moments from shared partial sums (14 additions): east/west and north/south movers
+    const real_t e3 = q[1] + q[5] + q[8], w3 = q[3] + q[6] + q[7];
+    const real_t n3 = q[2] + q[5] + q[6], s3 = q[4] + q[7] + q[8];
+    const real_t jx = e3 - w3, jy = n3 - s3;
+    const real_t s = ((q[0] + q[2]) + q[4]) + (e3 + w3);
     real_t drho = SHIFT ? s : s - (real_t)1.0;
     rho = SHIFT ? (real_t)1.0 + s : s;
-    real_t inv = (real_t)1.0 / rho;
+    const real_t inv = recip(rho);
     ux = jx * inv;
     uy = jy * inv;
     const real_t uu = fmar(ux, ux, uy * uy);
-    // Q = q - omega (q - feq)
+    // Q = q - omega (q - feq) = (1 - omega) q + omega feq, with omega feq =
+    // (omega rho) w_k poly_k + (omega D) w_k, D = rho - 1 (shifted) or rho, and
+    // w_k poly_k = w_k (3 eu + 4.5 eu^2 - 1.5 u.u) with the weights folded into
+    // per-direction constants (three distinct weights, so the products of
+    // u.u and of omega D by w_k are shared between directions)
 #pragma unroll
     for (int k = 0; k < 9; k++) Q[k] = fmar(-omega, q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu), q[k]);
 }
