@@ -1,13 +1,16 @@
 """Learned warm start of batched cylinder cases.
 
-    python -m mbl.warmstart [--train 256] [--test 64] [--spin 40000] [--steps 60000] [--epochs 60] ...
+    python -m mbl.warmstart [--train 4096] [--test 64] [--spin 40000] [--steps 40000] [--epochs 40]
+                            [--settled 0.01] [--snap-storage f32] ...
 
 1. Training cases (two-level channels, D in {16, 24} finest cells, blockage
    D/H ~ U[0.1, 0.3], Re ~ U[80, 200], the cylinder 0.25 coarse cells above
    the channel centre line) start at rest with the parabolic profile and run
-   `spin` steps with 16-bit planes (the 1-3-12 format by default), then `nsnap` snapshots of
+   `spin` steps (FP32 planes by default), then `nsnap` snapshots of
    (rho, ux, uy) are rasterised onto an H x W grid of each channel, every
-   `every` steps.
+   `every` steps. Cases whose Cl amplitude still drifts by more than
+   `settled` (relative) over the snapshot window are not yet on their limit
+   cycle and are left out of training.
 2. Phase model (default): inputs per pixel are the solid mask, Re/200, D/H,
    sin and cos of the shedding phase (linear in time between the upward zero
    crossings of Cl that bracket the snapshot), x and y; targets
@@ -349,6 +352,13 @@ def self_init(lib, args, storage, out):
                                                              for sp in te]),
                  cl=cl_c[:, -4000:].astype(np.float32), cd=cd_c[:, -4000:].astype(np.float32))
     init = shots[0].copy()
+    if args.self_init_noise > 0:
+        # uncorrelated pixel noise of the given size (relative to U) on the velocity
+        rng = np.random.default_rng(11)
+        U = np.array([sp.U for sp in te])[:, None, None]
+        init[..., 1] += (args.self_init_noise * U * rng.standard_normal(init.shape[:3])).astype(init.dtype)
+        init[..., 2] += (args.self_init_noise * U * rng.standard_normal(init.shape[:3])).astype(init.dtype)
+        out["self_init_noise"] = args.self_init_noise
     if args.self_init_rho == "one":
         init[..., 0] = 1.0
     elif args.self_init_rho == "poisson":
@@ -368,24 +378,24 @@ def self_init(lib, args, storage, out):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=256)
+    ap.add_argument("--train", type=int, default=4096)
     ap.add_argument("--test", type=int, default=64)
     ap.add_argument("--spin", type=int, default=40000)
     ap.add_argument("--nsnap", type=int, default=32)
     ap.add_argument("--every", type=int, default=50)
     ap.add_argument("--tail", type=int, default=1500)
-    ap.add_argument("--steps", type=int, default=60000)
+    ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--H", type=int, default=48)
     ap.add_argument("--W", type=int, default=192)
     ap.add_argument("--base", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=None, help="default 60 (phase model), 400 (harmonic model)")
+    ap.add_argument("--epochs", type=int, default=None, help="default 40 (phase model), 400 (harmonic model)")
     ap.add_argument("--model", default="phase", choices=["harmonic", "phase"],
                     help="harmonic: the network predicts each pixel's mean and two shedding harmonics from the "
                          "geometry and Re; phase: it predicts the field at a given phase")
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16s"], help="planes of the held-out runs")
-    ap.add_argument("--snap-storage", default="f16d", choices=["f32", "f16s", "f16d"],
+    ap.add_argument("--snap-storage", default="f32", choices=["f32", "f16s", "f16d"],
                     help="planes of the snapshot runs (f16d: the 1-3-12 16-bit format)")
     ap.add_argument("--save", default=None, help="write the series and the trained parameters to this .npz")
     ap.add_argument("--self-init", action="store_true",
@@ -393,10 +403,15 @@ def main(argv=None):
                          "(the best any network can do at this raster)")
     ap.add_argument("--self-init-rho", default="raster", choices=["raster", "one", "poisson"],
                     help="with --self-init: keep the raster's density, start from rho = 1 or solve for it")
+    ap.add_argument("--self-init-noise", type=float, default=0.0,
+                    help="with --self-init: add uncorrelated noise of this size (relative to U) to u and v")
     ap.add_argument("--dump", default=None, help="with --self-init: save the final rasters to this .npz")
     ap.add_argument("--diagnose", action="store_true",
                     help="also restart from hybrids of the network's field and each case's own final cold state "
                          "at the same phase (exact density + network velocity, and the reverse)")
+    ap.add_argument("--settled", type=float, default=0.01,
+                    help="keep only training cases whose Cl amplitude over the snapshot window stays within this "
+                         "relative band of its mean (0: keep all)")
     ap.add_argument("--fluct-scale", default=None,
                     help="diagnostic: comma-separated factors a; also runs warm starts from mean + a (field - mean), "
                          "the mean being the network's average over 8 phases")
@@ -407,7 +422,7 @@ def main(argv=None):
     storage = abi.STORE_NATIVE if args.storage == "f32" else abi.STORE_F16S
     snap_storage = {"f32": abi.STORE_NATIVE, "f16s": abi.STORE_F16S, "f16d": abi.STORE_F16D}[args.snap_storage]
     if args.epochs is None:
-        args.epochs = 400 if args.model == "harmonic" else 60
+        args.epochs = 400 if args.model == "harmonic" else 40
     H, W = args.H, args.W
     out = {"train_cases": args.train, "test_cases": args.test, "raster": [H, W], "storage": args.storage,
            "snap_storage": args.snap_storage}
@@ -423,6 +438,21 @@ def main(argv=None):
                                                     log="train cases")
     out["snapshot_run_s"] = round(wall_tr, 2)
     out["snapshot_run_mlups"] = round(nu_tr * steps_tr / wall_tr / 1e6, 1)
+    # training cases not yet on their limit cycle during the snapshot window put noise in the targets
+    keep = np.ones(len(tr), bool)
+    if args.settled > 0:
+        for c in range(len(tr)):
+            x = cl_tr[c, args.spin:]
+            T = shedding_period(cl_tr[c])
+            if T < 8:
+                keep[c] = False
+                continue
+            a = amplitude(x, T)[T - 1:]
+            keep[c] = np.max(np.abs(a - a.mean())) <= args.settled * a.mean()
+        out["settled_cases"] = int(keep.sum())
+        tr = [sp for sp, k in zip(tr, keep) if k]
+        cl_tr = cl_tr[keep]
+        shots = [sh[keep] for sh in shots]
     # per case: the snapshots' targets and phases
     case_t, case_ph = [], []
     for c, sp in enumerate(tr):
@@ -532,6 +562,32 @@ def main(argv=None):
             ph[c] = 2 * np.pi * ((x.size - 1 - t_last) / T_c)
         exact = fin[0]
         netf = from_target(predict(net, args.model, te, H, W, ph), te).astype(np.float32)
+        # the network's error on the held-out cases at their final phase, per channel, relative to
+        # the variance of the exact fields (target units), over all pixels and over fluid pixels
+        # within 3 diameters downstream of the cylinder
+        te_t = np.stack([to_target(exact[c:c + 1], [te[c]])[0] for c in range(len(te))])
+        pr_t = predict(net, args.model, te, H, W, ph)
+        err = (pr_t - te_t) ** 2
+        near = np.zeros(err.shape[:3], bool)
+        for c, sp in enumerate(te):
+            xs = np.linspace(0.0, sp.LX - 1.0, W)[None, :]
+            ys = np.linspace(0.0, sp.LY - 1.0, H)[:, None]
+            d = 2.0 * sp.R
+            near[c] = (xs > sp.CX - d) & (xs < sp.CX + 3 * d) & (np.abs(ys - sp.CY) < 1.5 * d)
+        var = np.var(te_t, axis=(0, 1, 2))
+        out["test_rel_err"] = [round(float(v), 5) for v in np.mean(err, axis=(0, 1, 2)) / var]
+        out["test_rel_err_near_wake"] = [round(float(v), 5) for v in err[near].mean(axis=0) / var]
+        # per case: how far the cold run is from a settled limit cycle at its end (largest relative
+        # deviation of the Cl amplitude over its last 8 periods), the network's near-wake error in v,
+        # and the steps to the converged amplitude cold and warm
+        settle = []
+        for c in range(len(te)):
+            T_c = max(shedding_period(cl_c[c]), 8)
+            a = amplitude(cl_c[c, -8 * T_c:], T_c)[T_c - 1:]
+            settle.append(float(np.max(np.abs(a - a.mean())) / max(a.mean(), 1e-12)))
+        out["per_case"] = {"settle": [round(x, 4) for x in settle],
+                           "v_err_near": [round(float(err[c][near[c]][:, 2].mean() / var[2]), 4) for c in range(len(te))],
+                           "tc": [int(x) for x in tc], "tw": [int(x) for x in tw]}
         variants = {"net_matched_phase": netf,
                     "exact_rho_net_u": np.concatenate([exact[..., :1], netf[..., 1:]], -1),
                     "net_rho_exact_u": np.concatenate([netf[..., :1], exact[..., 1:]], -1)}
