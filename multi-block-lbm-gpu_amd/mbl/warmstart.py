@@ -605,6 +605,7 @@ def main(argv=None):
                  cl_warm=cl_w[:, ::10].astype(np.float32), loss=hist, params=net.get_params())
     net.close()
     print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
