@@ -785,12 +785,17 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     for (int b = 0; b < nline; b++)
         if (!check_cf_line(b)) return fail("coarse/fine line");
 
+    // marker group of each line: the level whose spline step rotates its ring markers
+    std::vector<int> mgroup(std::max(1, nline), 0);
+    for (int l = 0; l < cfg.LEVEL; l++)
+        for (int b = L->LS3[l]; b < L->LS3[l + 1] && b < nline; b++) mgroup[b] = l;
     auto make_T_or_CF = [&](int b, int type) -> OpBuild {
         OpBuild o;
         memset(&o.op, 0, sizeof(LineOp));
         const Inf_Ff_store &lc = A.lc[b];
         const AREA &af = A.areas[lc.areaf_n];
         o.op.type = type;
+        o.op.mgroup = mgroup[b];
         o.op.n = A.t3[b].N;
         o.op.dst_block = lc.areaf_n;
         o.op.src_block = lc.areaf_n;
@@ -1051,12 +1056,34 @@ static int markers_of(const Layout *L, const Ff_Store *t3, int b0, int b1, int m
     return 0;
 }
 
-static void run_ops(mbl_context *h, Layout *L, const Batch &B, CycleState &cs, int m0, int m1, int m2, double xm)
+// The markers of every line group as they stand (a group's lines rotate together).
+static int marker_set(const Layout *L, const Ff_Store *t3, MarkerSet &ms)
 {
-    if (!cs.launch || cs.rc) return;
+    for (int l = 0; l < MBL_MAX_LEVEL; l++) {
+        ms.m[l][0] = 2;
+        ms.m[l][1] = 1;
+        ms.m[l][2] = 0;
+    }
+    for (int l = 0; l < L->LEVEL; l++)
+        if (L->LS3[l + 1] > L->LS3[l]) {
+            int rc = markers_of(L, t3, L->LS3[l], L->LS3[l + 1], ms.m[l]);
+            if (rc) return rc;
+        }
+    return 0;
+}
+
+static void run_ops(mbl_context *h, Layout *L, const Batch &B, CycleState &cs, const Ff_Store *t3, double xm)
+{
+    if (!cs.launch || cs.rc || B.empty()) return;
+    MarkerSet ms;
+    int rc = marker_set(L, t3, ms);
+    if (rc) {
+        cs.rc = rc;
+        return;
+    }
     const int mode = storage_mode(h->cfg);
     for (const Sub &s : B) {
-        int rc = launch_line_ops(h->stream, mode, L->dops + s.off, s.count, s.max_n, L->db, cs.pm, m0, m1, m2, xm);
+        rc = launch_line_ops(h->stream, mode, L->dops + s.off, s.count, s.max_n, L->db, cs.pm, ms, xm);
         if (rc) cs.rc = rc;
     }
 }
@@ -1067,14 +1094,8 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
     const int mode = storage_mode(h->cfg);
     for (int i = 0; i < L->mi[level]; i++) {
         int m[3];
-        if (!(level == 0 || i == 0) && !L->bT[level].empty()) {
-            int rc = markers_of(L, t3, L->LTC[level], L->LTC[level + 1], m);
-            if (rc) {
-                cs.rc = rc;
-                return;
-            }
-            run_ops(h, L, L->bT[level], cs, m[0], m[1], m[2], 1.0 * i / L->mi[level]);
-        }
+        if (!(level == 0 || i == 0) && !L->bT[level].empty())
+            run_ops(h, L, L->bT[level], cs, t3, 1.0 * i / L->mi[level]);
         if (cs.launch && L->tiles[level] > 0) {
             int rc = launch_stream_collide(h->stream, mode, L->vx[level], h->tpb, L->periodic, true, true,
                                            L->db + L->ASP[level], L->dtile + L->tile_off[level],
@@ -1083,7 +1104,7 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
             if (rc) cs.rc = rc;
         }
         cs.pm ^= (1u << level);
-        if (L->number_line2 != 0) run_ops(h, L, L->bCC[level], cs, 0, 0, 0, 0.0);
+        if (L->number_line2 != 0) run_ops(h, L, L->bCC[level], cs, t3, 0.0);
         if (level != L->LEVEL - 1 && L->LS3[level + 1] > L->LS3[level]) {
             int rc = markers_of(L, t3, L->LS3[level], L->LS3[level + 1], m);
             if (rc) {
@@ -1102,14 +1123,8 @@ static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState
                 }
         }
         cycle(h, L, level + 1, t3, cs);
-        if (level != 0 && i == L->mi[level] - 1 && !L->bCF[level].empty()) {
-            int rc = markers_of(L, t3, L->LTC[level], L->LTC[level + 1], m);
-            if (rc) {
-                cs.rc = rc;
-                return;
-            }
-            run_ops(h, L, L->bCF[level], cs, m[0], m[1], m[2], 0.0);
-        }
+        if (level != 0 && i == L->mi[level] - 1 && !L->bCF[level].empty())
+            run_ops(h, L, L->bCF[level], cs, t3, 0.0);
     }
 }
 

@@ -63,8 +63,16 @@ struct BlockDesc {
 // one line between two blocks or between a block and a ring buffer.
 enum OpType { OP_TEMPORAL = 0, OP_CC = 1, OP_CF = 2, OP_FC = 3, OP_SPLINE = 4 };
 
+// Ring-slot markers (Ff_Store.marker) of every group of lines, a group being
+// the lines whose coarse block is on one level: they rotate together, at that
+// level's rate (evolution.h:41-46). Passed by value to the line-op kernels.
+struct MarkerSet {
+    int m[MBL_MAX_LEVEL][3];
+};
+
 struct LineOp {
     int type;
+    int mgroup;       // TEMPORAL, CF: marker group (level of the coarse block of the line)
     int n;            // entries (fine points for TEMPORAL/CF/SPLINE, Nc for CC/FC)
     int src_block;    // table index of the block read (CC, FC, SPLINE coarse block)
     int dst_block;    // table index of the block written
@@ -120,7 +128,7 @@ int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &
                         Lbd *forcex, Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed,
                         double sigma, double LX, double LY, double R, double CX, double CY, const Consts &c);
 int launch_line_ops(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
-                    const BlockDesc *d_blocks, uint32_t parity_mask, int m0, int m1, int m2, double xm);
+                    const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms, double xm);
 int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
                   const BlockDesc *d_blocks, uint32_t parity_mask, int slot);
 int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,

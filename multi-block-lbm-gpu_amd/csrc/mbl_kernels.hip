@@ -804,11 +804,12 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
 // for the mask.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
-                                                  uint32_t parity_mask, int m0, int m1, int m2, double xm)
+                                                  uint32_t parity_mask, MarkerSet ms, double xm)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
     const LineOp &op = ops[blockIdx.y];
+    const int m0 = ms.m[op.mgroup][0], m1 = ms.m[op.mgroup][1], m2 = ms.m[op.mgroup][2];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= op.n) return;
     const BlockDesc &db = blocks[op.dst_block];
@@ -1597,13 +1598,12 @@ int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &
 }
 
 int launch_line_ops(void *stream, int mode, const LineOp *d_ops, int nops, int max_n, const BlockDesc *d_blocks,
-                    uint32_t parity_mask, int m0, int m1, int m2, double xm)
+                    uint32_t parity_mask, const MarkerSet &ms, double xm)
 {
     if (nops <= 0 || max_n <= 0) return 0;
     dim3 g((max_n + 255) / 256, nops), b(256);
     hipStream_t s = (hipStream_t)stream;
-    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_line_ops<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, m0, m1, m2,
-                                             xm));
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_line_ops<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, ms, xm));
     MBL_LAUNCH_CHECK();
     return 0;
 }

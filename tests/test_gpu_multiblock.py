@@ -98,6 +98,19 @@ def test_geo_2_5_reference_mode_fp64(lib64):
     sim.close()
 
 
+def test_geo_4_9_reference_mode_fp64(lib64):
+    """The reference's larger layout: 10 blocks on 4 levels, ratio-4 lines as coded, and coarse
+    lines of 305 and 321 points -- longer than the one-workgroup spline step takes, so the spline
+    runs as the two-kernel solve + evaluate path here."""
+    geom = mbl.Geometry.preset(lib64, "GEO_4_9")
+    _, areas, lc, _, _ = geom.init(lib64)
+    assert max(lc[b].Nc for b in range(geom.g.number_line)) > 303
+    sim, o = run_pair(lib64, geom, 3)
+    assert_fields_close(sim, o, 1e-11, 1e-12)
+    assert_series_close(sim, o, 1e-10)
+    sim.close()
+
+
 def test_tiny2_fp32_close_to_oracle(lib32):
     geom = tiny2(lib32)
     sim, o = run_pair(lib32, geom, 40)
