@@ -1544,7 +1544,7 @@ extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
     int rc = 0;
     for (size_t i = 0; i < nl; i++) {
         double v = h->h_l2[3 * i];
-        if (fl) fprintf(fl, "%d\t%g\n", h->pend_l2[i], v); // error_L2.h:302
+        if (fl) fprintf(fl, "%d\t%g\n", h->pend_l2[i], v); // error_L2.h:87
         h->ser_l2_t.push_back(h->pend_l2[i]);
         h->ser_l2.push_back(v);
         h->last[0] = v;

@@ -2,7 +2,7 @@
 
 The reference writes one line per coarse step to fdrag_t.dat ("t\\tCd\\tCl\\t",
 lbm/force_block.h:212-215) and error_L2.dat ("t\\tresidual",
-lbm/error_L2.h:300-303) and stops there. These functions read those files (or
+lbm/error_L2.h:85-88) and stops there. These functions read those files (or
 the series returned by the library) and reduce them to the quantities the
 benchmark cases are judged on: time-averaged Cd, Cl rms and amplitude,
 Strouhal number, and the time to a converged Cd.

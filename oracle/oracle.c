@@ -955,7 +955,7 @@ int orc_sim_diag(orc_sim *s, double out[3])
 }
 
 /* EXT: velocity L2 distance from the analytic plane Poiseuille profile over
- * FLUID nodes, 1/m block weights (the normalisation of error_L2.h:297-302) */
+ * FLUID nodes, 1/m block weights (the normalisation of error_L2.h:82-87) */
 double orc_sim_poiseuille_l2(const orc_sim *s)
 {
     double num = 0.0, den = 0.0;

@@ -92,3 +92,21 @@ def test_odd_width_periodic(lib32, lib64):
     o.close()
     g = run_gpu(lib64, nx=37, ny=23, steps=9)
     assert np.max(np.abs(g - F)) < 1e-13
+
+
+def test_bandwidth_probes(lib32):
+    """The streaming-copy probes bench.py reports next to the roofline: both
+    return a rate between a tenth of the 8 TB/s peak and the peak itself, and
+    reject a zero repetition count."""
+    import ctypes
+    cfg = mbl.default_config(lib32, LEVEL=1, write_files=0)
+    cfg.mi[0] = 1
+    h = ctypes.c_void_p()
+    assert lib32.mbl_handle_create(ctypes.byref(h), ctypes.byref(cfg)) == 0
+    g = ctypes.c_double()
+    assert lib32.mbl_bandwidth_probe(h, 256 << 20, 3, ctypes.byref(g)) == 0
+    assert 800.0 < g.value < 8000.0, g.value
+    assert lib32.mbl_bandwidth_probe_planes(h, 32 << 20, 3, ctypes.byref(g)) == 0
+    assert 800.0 < g.value < 8000.0, g.value
+    assert lib32.mbl_bandwidth_probe(h, 256 << 20, 0, ctypes.byref(g)) != 0
+    lib32.mbl_handle_destroy(h)
