@@ -488,6 +488,12 @@ static int seg_tiles(int nx, int vx, int tw)
 
 static int seg_waste(int nx, int vx, int tw) { return seg_tiles(nx, vx, tw) * tw * vx - nx; }
 
+static void set_tiles_x(BlockDesc &d, int tiles_x)
+{
+    d.tiles_x = tiles_x;
+    d.tx_rcp = 0xFFFFFFFFu / (uint32_t)tiles_x;
+}
+
 static BlockDesc make_desc(const mbl_config &cfg, const AREA &a, const D2Q9 &pp, int level, Lbd *upx, Lbd *upy)
 {
     BlockDesc d;
@@ -753,11 +759,12 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
                 if (!A.pps[a].F[k] || !A.pps[a].f[k] || A.pps[a].F[k] == A.pps[a].f[k]) return fail("F/f planes");
             BlockDesc d = make_desc(cfg, ar, A.pps[a], l, nullptr, nullptr);
             if (A.bc) d.bc = A.bc[a];
-            // 32-lane row segments when they waste fewer lanes than 64-lane ones
+            // 32-lane row segments when they waste fewer lanes than 64-lane ones (periodic
+            // blocks keep 64: the kernel relies on it there)
             d.tw = 64;
-            if (seg_waste(ar.nx, vx, 32) < seg_waste(ar.nx, vx, 64)) d.tw = 32;
+            if (!L->periodic && seg_waste(ar.nx, vx, 32) < seg_waste(ar.nx, vx, 64)) d.tw = 32;
             d.trows = rows * (64 / d.tw);
-            d.tiles_x = seg_tiles(ar.nx, vx, d.tw);
+            set_tiles_x(d, seg_tiles(ar.nx, vx, d.tw));
             d.tile_begin = tiles;
             tiles += d.tiles_x * ((ar.ny + d.trows - 1) / d.trows);
             tile_blk.resize(L->tile_off[l] + tiles, a - L->ASP[l]);
@@ -1281,10 +1288,14 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
         if (periodic)
             while (S.vx > 1 && area->nx % S.vx) S.vx >>= 1;
         S.tpb = h->tpb;
-        S.rows = S.tpb / 64;
+        // rows per wave: the 1-4-11 and 1-3-12 planes are VALU-bound, and four rows per wave
+        // share one tile's index arithmetic (16384^2 on MI355X: 127k -> 139k/140k MLUPS); the
+        // other storages are bandwidth-bound and lose parallelism with more (measured slower)
+        const int st = storage_mode(h->cfg);
+        S.rows = S.tpb / 64 * ((st == ST_F16C || st == ST_F16D) ? 4 : 1);
         S.hd.tw = 64;
         S.hd.trows = 0;
-        S.hd.tiles_x = seg_tiles(area->nx, S.vx, 64);
+        set_tiles_x(S.hd, seg_tiles(area->nx, S.vx, 64));
         S.hd.tile_begin = 0;
         S.tiles = S.hd.tiles_x * ((area->ny + S.rows - 1) / S.rows);
         S.periodic = periodic;

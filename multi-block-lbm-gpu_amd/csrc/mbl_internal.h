@@ -49,6 +49,7 @@ struct BlockDesc {
     int level;
     int m;
     int tiles_x;          // stream-collide tiles along x
+    uint32_t tx_rcp;      // 0xFFFFFFFF / tiles_x (tile row and column by a multiply-high)
     int tw;               // tile width in lanes (64, or 32: a wave then covers two rows)
     int trows;            // rows per tile (0: the launch's tile_rows)
     int tile_begin;       // first stream-collide tile of the block inside its launch

@@ -116,18 +116,20 @@ template <int EB> struct Half16 {
     {
         const uint32_t u = __float_as_uint(v);
         const uint32_t s = (u >> 16) & 0x8000u, a = u & 0x7FFFFFFFu;
-        constexpr uint32_t SH = 23 - FB, HALF = (1u << (SH - 1)) - 1u;
-        // round to nearest even on the SH dropped bits and remove the exponent offset before the
-        // shift (OFF << 23 has no bits below SH): one 3-input add and one shift. Below the normal
-        // range the sum wraps, but the subnormal code is selected there.
-        uint32_t nrm = (a + ((a >> SH) & 1u) + (HALF - (OFF << 23))) >> SH;
-        nrm = min(nrm, 0x7FFEu); // saturate (and infinity) one code below 0x7FFF, so no sign gives 0xFFFF
-        // below 2^(1 - BIAS) the code is |v| 2^(BIAS - 1 + FB) (exact, < 2^FB + 1/2) rounded to the
-        // nearest even integer: adding 1.5 * 2^23 rounds it once, in the fused multiply-add
-        const float scaled = __builtin_fmaf(__uint_as_float(a), __uint_as_float((uint32_t)(127 + BIAS - 1 + FB) << 23),
-                                            12582912.0f);
-        const uint32_t sub = __float_as_uint(scaled) - 0x4B400000u;
-        const uint32_t code = (a >= ((OFF + 1u) << 23) ? nrm : sub) | s;
+        constexpr uint32_t SH = 23 - FB, HALF = (1u << (SH - 1)) - 1u, MAGIC = 0x4B400000u; // bits of 1.5 * 2^23
+        // Two candidate codes, both offset by MAGIC (whose low 16 bits are zero):
+        // sub: |v| 2^(BIAS - 1 + FB) rounded to the nearest even integer by adding 1.5 * 2^23 in the
+        //   fused multiply-add: the code below 2^(1 - BIAS), and at least the normal code above it
+        //   (equal in the first normal binade, larger beyond).
+        // nrm: the normal code, rounded to nearest even on the SH dropped bits with the exponent
+        //   offset of the first normal binade removed before the shift (one 3-input add, one
+        //   shift, one add); below that binade the sum wraps and nrm exceeds MAGIC + 2^19.
+        // So min(nrm, sub) is the code everywhere, and the third operand saturates overflow and
+        // infinity one code below 0x7FFF, so no sign gives 0xFFFF: one 3-input minimum.
+        const uint32_t nrm = ((a + ((a >> SH) & 1u) + (HALF - ((OFF + 1u) << 23))) >> SH) + (MAGIC + (1u << FB));
+        const uint32_t sub = __float_as_uint(
+            __builtin_fmaf(__uint_as_float(a), __uint_as_float((uint32_t)(127 + BIAS - 1 + FB) << 23), 12582912.0f));
+        const uint32_t code = min(min(nrm, sub), MAGIC + 0x7FFEu) | s;
         return (uint16_t)(a > 0x7F800000u ? 0xFFFFu : code);
     }
 };
@@ -192,22 +194,30 @@ __device__ inline double fmar(double a, double b, double c) { return __builtin_f
 // Equilibrium in the stored domain, compute precision. drho = rho - 1 is
 // passed separately so the shifted form keeps its small magnitude.
 // feq = w rho (1 + 3 eu + 4.5 eu^2 - 1.5 u.u)  (lbm/assit_func.h:2-9)
-template <bool SHIFT>
-__device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t uy, real_t uu)
+// 3 eu + 4.5 eu^2 - 1.5 u.u for direction k
+__device__ inline real_t poly_k(int k, real_t ux, real_t uy, real_t uu)
 {
 #pragma clang fp contract(off)
-    const real_t w = (real_t)W64[k];
     real_t eu = (real_t)0.0; // e_k . u with e_k in {-1, 0, 1}^2: one rounding at most
     if (EX[k] == 1) eu = ux;
     if (EX[k] == -1) eu = -ux;
     if (EY[k] == 1) eu = (EX[k] == 0) ? uy : eu + uy;
     if (EY[k] == -1) eu = (EX[k] == 0) ? -uy : eu - uy;
-    const real_t poly = fmar(eu, fmar((real_t)4.5, eu, (real_t)3.0), (real_t)-1.5 * uu);
+    return fmar(eu, fmar((real_t)4.5, eu, (real_t)3.0), (real_t)-1.5 * uu);
+}
+
+template <bool SHIFT>
+__device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t uy, real_t uu)
+{
+#pragma clang fp contract(off)
+    const real_t w = (real_t)W64[k], poly = poly_k(k, ux, uy, uu);
     if (SHIFT) return w * fmar(rho, poly, drho);
     return w * fmar(rho, poly, rho);
 }
 
 // BGK on one node: q[9] in, Q[9] out, macros out (evolution_block.h:60-70, 100-111).
+// Relaxation in the difference form Q = q - omega (q - feq): q - feq is formed
+// before the scaling, so the result carries one rounding of q's size.
 template <bool SHIFT>
 __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
 {
@@ -223,11 +233,6 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
     ux = jx * inv;
     uy = jy * inv;
     const real_t uu = fmar(ux, ux, uy * uy);
-    // Q = q - omega (q - feq) = (1 - omega) q + omega feq, with omega feq =
-    // (omega rho) w_k poly_k + (omega D) w_k, D = rho - 1 (shifted) or rho, and
-    // w_k poly_k = w_k (3 eu + 4.5 eu^2 - 1.5 u.u) with the weights folded into
-    // per-direction constants (three distinct weights, so the products of
-    // u.u and of omega D by w_k are shared between directions)
 #pragma unroll
     for (int k = 0; k < 9; k++) Q[k] = fmar(-omega, q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu), q[k]);
 }
@@ -436,16 +441,17 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
 template <int MODE, bool MACROS, int NR>
 __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
                                     typename Codec<MODE>::GT *const (&D)[9], int i, const int (&jr)[NR],
-                                    const Consts &c, real_t omega)
+                                    const uint32_t (&ro)[NR], const Consts &c, real_t omega)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
     const int nx = d.nx, ny = d.ny;
+    const uint32_t unx = (uint32_t)nx;
     int flag[NR];
     uint32_t o[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        o[r] = (uint32_t)jr[r] * (uint32_t)nx + (uint32_t)i;
+        o[r] = ro[r] + (uint32_t)i;
         flag[r] = (jr[r] < ny) ? d.flag[o[r]] : C_BOUNDARY;
     }
     T raw[NR][9];
@@ -457,7 +463,9 @@ __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MOD
         for (int k = 0; k < 9; k++) {
             const int id = i - EX[k], jd = j - EY[k];
             const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
-            raw[r][k] = (j < ny) ? *off32(S[k], own ? o[r] : (uint32_t)jd * (uint32_t)nx + (uint32_t)id) : T{};
+            // row jd's offset from row j's (wraps only where own holds)
+            const uint32_t rk = EY[k] == 1 ? ro[r] - unx : (EY[k] == -1 ? ro[r] + unx : ro[r]);
+            raw[r][k] = (j < ny) ? *off32(S[k], own ? o[r] : rk + (uint32_t)id) : T{};
         }
     }
     bool special[NR];
@@ -505,11 +513,12 @@ __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MOD
 
 template <int MODE, bool MACROS>
 __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
-                                    typename Codec<MODE>::GT *const (&D)[9], int i, int j, const Consts &c,
-                                    real_t omega)
+                                    typename Codec<MODE>::GT *const (&D)[9], int i, int j, uint32_t r0,
+                                    const Consts &c, real_t omega)
 {
     const int jr[1] = {j};
-    flagged_rows<MODE, MACROS, 1>(d, S, D, i, jr, c, omega);
+    const uint32_t ro[1] = {r0};
+    flagged_rows<MODE, MACROS, 1>(d, S, D, i, jr, ro, c, omega);
 }
 
 // ---------------------------------------------------------------------------
@@ -518,10 +527,10 @@ __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MOD
 // all-fluid runs use VX-wide loads of each source row plus one scalar for the
 // x-shifted planes; every other node takes the flagged per-node update.
 // ---------------------------------------------------------------------------
-// One row segment of a tile: 64*VX consecutive nodes of row j starting at the
-// wave's first node.
+// One row segment of a tile: 64*VX consecutive nodes of row j (element offset
+// r0 = j * nx in the planes) starting at the wave's first node.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, int x0)
+__device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, uint32_t r0, int x0)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
@@ -538,10 +547,9 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
         D[k] = (GT *)d.P[par ^ 1][k];
     }
     const real_t omega = (real_t)d.omega;
-    const size_t r0 = (size_t)j * nx;
 
     if constexpr (FLAGS && !PERIODIC && VX == 1) {
-        flagged_node<MODE, MACROS>(d, S, D, x0, j, c, omega);
+        flagged_node<MODE, MACROS>(d, S, D, x0, j, r0, c, omega);
         return;
     }
 
@@ -552,7 +560,7 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
         fast = (x0 >= 1) && (x0 + VX <= nx - 1) && (j >= 1) && (j <= ny - 2);
     if (FLAGS && fast) {
         int fl[VX];
-        vload<int, VX>((const GLOBAL_AS int *)d.flag + r0 + x0, fl);
+        vload<int, VX>(off32((const GLOBAL_AS int *)d.flag, r0 + (uint32_t)x0), fl);
 #pragma unroll
         for (int v = 0; v < VX; v++) fast = fast && (fl[v] == FLUID);
     }
@@ -563,21 +571,24 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
             const int i = x0 + v;
             if (i < 0 || i >= nx) continue;
             if constexpr (FLAGS && !PERIODIC)
-                flagged_node<MODE, MACROS>(d, S, D, i, j, c, omega);
+                flagged_node<MODE, MACROS>(d, S, D, i, j, r0, c, omega);
             else
                 general_node<MODE, PERIODIC, FLAGS, MACROS>(d, S, D, i, j, c, omega);
         }
         return;
     }
 
-    int jm = j - 1, jp = j + 1, xm = x0 - 1, xp = x0 + VX;
+    // rows j -1 and j + 1 (periodic blocks wrap)
+    const uint32_t unx = (uint32_t)nx;
+    uint32_t rm = r0 - unx, rp = r0 + unx;
+    int xm = x0 - 1, xp = x0 + VX;
     if (PERIODIC) {
-        jm = (j == 0) ? ny - 1 : jm;
-        jp = (j == ny - 1) ? 0 : jp;
+        rm = (j == 0) ? (uint32_t)(ny - 1) * unx : rm;
+        rp = (j == ny - 1) ? 0u : rp;
         xm = (x0 == 0) ? nx - 1 : xm;
         xp = (xp >= nx) ? 0 : xp;
     }
-    const uint32_t rm = (uint32_t)jm * (uint32_t)nx, rp = (uint32_t)jp * (uint32_t)nx, rc = (uint32_t)r0;
+    const uint32_t rc = r0;
     real_t q[9][VX];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -651,22 +662,34 @@ __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, con
         }
     }
     const BlockDesc &d = descs[lo];
-    const int lt = t - d.tile_begin;
-    const int tyi = lt / d.tiles_x;
-    const int txi = lt - tyi * d.tiles_x;
-    // a wave covers 64 / tw row segments of tw consecutive lanes
-    const int tw = d.tw, lane = (int)(threadIdx.x & 63);
-    const int per_wave = 64 / tw, sub = lane / tw, col = lane - sub * tw;
-    const int base = (txi * tw + col) * VX;
+    // the tile's column and row in its block: lt / tiles_x is the multiply-high by the host's
+    // 0xFFFFFFFF / tiles_x (the quotient or one less for lt < 2^31) and one correction, in
+    // scalar registers
+    const uint32_t lt = (uint32_t)(t - d.tile_begin), ntx = (uint32_t)d.tiles_x;
+    uint32_t tyi = __umulhi(lt, d.tx_rcp), txi = lt - tyi * ntx;
+    if (txi >= ntx) {
+        tyi++;
+        txi -= ntx;
+    }
+    // a wave covers one row segment of 64 lanes or two of 32 (tw is 64 or 32; always 64 in
+    // periodic blocks, so there a wave's row and row offsets are scalar)
+    const int tw = PERIODIC ? 64 : d.tw, lane = (int)(threadIdx.x & 63);
+    const bool two = !PERIODIC && (tw == 32);
+    const int per_wave = two ? 2 : 1, sub = two ? (lane >> 5) : 0, col = lane & (tw - 1);
+    const int base = ((int)txi * tw + col) * VX;
     const int rows = d.trows > 0 ? d.trows : tile_rows;
     const int step = (int)(blockDim.x >> 6) * per_wave;
-    for (int r = (int)(threadIdx.x >> 6) * per_wave + sub; r < rows; r += step) {
-        const int j = tyi * rows + r;
+    const uint32_t unx = (uint32_t)d.nx;
+    // the wave's first row rw and its row offset are scalar; a lane adds its segment's row (sub)
+    for (int rw = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * per_wave; rw < rows; rw += step) {
+        if (rw + sub >= rows) break;
+        const int jw = (int)tyi * rows + rw;
+        const uint32_t r0 = (uint32_t)jw * unx + (sub ? unx : 0u);
         // Segments start at multiples of VX of the plane index, so a row of a
         // length that is not a multiple of VX begins (j*nx mod VX) nodes into
         // its first segment; the host sizes tiles_x for that shift.
-        const int x0 = PERIODIC ? base : base - (int)(((uint32_t)j * (uint32_t)d.nx) & (uint32_t)(VX - 1));
-        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, j, x0);
+        const int x0 = PERIODIC ? base : base - (int)(r0 & (uint32_t)(VX - 1));
+        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, jw + sub, r0, x0);
     }
 }
 
