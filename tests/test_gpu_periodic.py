@@ -8,6 +8,7 @@ Tolerances, after 20 steps at tau = 0.6 from u ~ N(0, 0.01^2):
   FP32 planes (f - w)    <= 2e-7
   FP16 planes (f - w)    <= 2e-4   (binary16: 11-bit significand of |f - w| <~ 0.05)
   BF16 planes (f - w)    <= 2e-3   (bfloat16: 8-bit significand)
+  1-4-11, 1-3-12 planes  <= 1e-4   (12- and 13-bit significands)
 """
 import numpy as np
 import pytest
@@ -54,7 +55,8 @@ def test_fp64_matches_oracle(lib64, ref):
 
 
 @pytest.mark.parametrize("storage,tol", [(abi.STORE_NATIVE, 2e-6), (abi.STORE_F32S, 2e-7),
-                                         (abi.STORE_F16S, 2e-4), (abi.STORE_BF16S, 2e-3)])
+                                         (abi.STORE_F16S, 2e-4), (abi.STORE_BF16S, 2e-3),
+                                         (abi.STORE_F16C, 1e-4), (abi.STORE_F16D, 1e-4)])
 def test_fp32_storage_modes(lib32, ref, storage, tol):
     g = run_gpu(lib32, storage=storage)
     err = np.max(np.abs(g - ref))
@@ -68,6 +70,25 @@ def test_tilings_bitwise_equal(lib32, vx, tpb):
     base = run_gpu(lib32, tuning=(1, 256), nx=128, ny=40, steps=7)
     g = run_gpu(lib32, tuning=(vx, tpb), nx=128, ny=40, steps=7)
     assert np.array_equal(base, g)
+
+
+@pytest.mark.parametrize("storage", [abi.STORE_F16C, abi.STORE_F16D])
+@pytest.mark.parametrize("nx,ny", [(96, 61), (128, 40)])
+def test_custom_16bit_row_tiles(lib32, storage, nx, ny):
+    """The 1-4-11 and 1-3-12 planes take four rows per wave in a single block
+    (tiles of 4, 8 or 16 rows for 64, 128 or 256 threads), so 61 and 40 rows
+    leave a partial last tile: every workgroup size gives the same bits, and
+    the result stays within the format's rounding of the FP64 oracle."""
+    o = oracle.OraclePeriodic(nx, ny, tau=TAU)
+    o.run(STEPS)
+    F = np.stack([o.get(("F", k)) for k in range(9)])
+    o.close()
+    runs = [run_gpu(lib32, storage=storage, tuning=(1, tpb), nx=nx, ny=ny) for tpb in (64, 128, 256)]
+    for g in runs[1:]:
+        assert np.array_equal(runs[0], g)
+    err = np.max(np.abs(runs[0] - F))
+    print("storage", storage, nx, ny, "max err", err)
+    assert err < 1e-4
 
 
 def test_mass_and_momentum_conserved(lib64):
