@@ -244,8 +244,10 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
 // collided on the deviations f - w_k (exact while f lies within [w_k / 2, 2 w_k],
 // Sterbenz, which the flow speeds of the method keep; outside it, one rounding of
 // the small result), so the moments and q - feq carry no rounding of f's size; the
-// relaxation lands on the unshifted f with one fused multiply-add. On C1 this takes the Cd error against the FP64 oracle from 1.1e-5
-// to 6.2e-6. Shifted planes and the FP64 build collide their loaded values as they are.
+// relaxation lands on the unshifted f with one fused multiply-add. On the two-level
+// cylinder channel at Re = 20 (20,000 coarse steps) this takes the drag coefficient's
+// difference from an FP64 run of the same case from 1.1e-5 to 6.2e-6 (relative).
+// Shifted planes and the FP64 build collide their loaded values as they are.
 template <class C>
 __device__ inline void collide_c(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
 {
