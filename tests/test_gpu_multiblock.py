@@ -204,3 +204,34 @@ def test_launch_shapes_bitwise_equal(lib32, storage):
             assert np.array_equal(x, y)
         assert np.array_equal(l0["residual"], l["residual"])
         assert np.array_equal(d0["Cd"], d["Cd"])
+
+
+@pytest.fixture(scope="module")
+def c1_oracle(lib32):
+    """The FP64 oracle on the two-level cylinder channel (300 x 80 coarse cells, Re = 20) over
+    the full 20,000 coarse steps: Cd and Cl series."""
+    geom = mbl.c1_geometry(lib32)
+    sim = mbl.MultiBlockSim(lib32, geom)
+    o = oracle.OracleSim(geom.g, sim.cfg)
+    sim.close()
+    o.run(20000)
+    cd, cl = o.series(3), o.series(4)
+    o.close()
+    return cd, cl
+
+
+@pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F32S])
+def test_c1_fp32_within_1e5(lib32, c1_oracle, storage):
+    """Both FP32 paths (planes holding f, as the reference, and planes holding f - w) end the
+    full C1 run within 1e-5 of the oracle's drag coefficient (measured 6.2e-6 and 4.9e-6) and
+    within 2e-5 of its lift over the whole run (8.8e-6 and 3.3e-6)."""
+    from mbl import cases
+    ocd, ocl = c1_oracle
+    _, ser = cases.run(lib32, mbl.c1_geometry(lib32), 20000, storage=storage, D=None)
+    cd, cl = np.asarray(ser["Cd"]), np.asarray(ser["Cl"])
+    assert cd.shape == ocd.shape
+    err_cd = abs(cd[-1] - ocd[-1]) / abs(ocd[-1])
+    err_cl = np.max(np.abs(cl - ocl))
+    print("storage", storage, "Cd rel err", err_cd, "Cl abs err", err_cl)
+    assert err_cd < 1e-5
+    assert err_cl < 2e-5
