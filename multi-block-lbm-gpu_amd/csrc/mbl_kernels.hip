@@ -218,8 +218,8 @@ __device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t u
 // BGK on one node: q[9] in, Q[9] out, macros out (evolution_block.h:60-70, 100-111).
 // Relaxation in the difference form Q = qa - omega (q - feq), qa the value q stands
 // for in the output's domain (q itself, or q + w_k when q is the shifted copy of an
-// unshifted input): q - feq is formed before the scaling, so the result carries one
-// rounding of qa's size.
+// unshifted input): q - feq is formed (in one fused multiply-add) before the scaling,
+// so the result carries one rounding of qa's size.
 template <bool SHIFT>
 __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy,
                                const real_t (&qa)[9])
@@ -237,7 +237,11 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
     uy = jy * inv;
     const real_t uu = fmar(ux, ux, uy * uy);
 #pragma unroll
-    for (int k = 0; k < 9; k++) Q[k] = fmar(-omega, q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu), qa[k]);
+    for (int k = 0; k < 9; k++) {
+        // feq = w_k t (qeq's form); q - feq is one fused multiply-add, one rounding
+        const real_t t = fmar(rho, poly_k(k, ux, uy, uu), SHIFT ? drho : rho);
+        Q[k] = fmar(-omega, fmar(-(real_t)W64[k], t, q[k]), qa[k]);
+    }
 }
 
 // The collision of one node for codec C. Binary32 planes holding f itself are
@@ -246,7 +250,7 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
 // the small result), so the moments and q - feq carry no rounding of f's size; the
 // relaxation lands on the unshifted f with one fused multiply-add. On the two-level
 // cylinder channel at Re = 20 (20,000 coarse steps) this takes the drag coefficient's
-// difference from an FP64 run of the same case from 1.1e-5 to 6.2e-6 (relative).
+// difference from an FP64 run of the same case from 1.1e-5 to 5.5e-6 (relative).
 // Shifted planes and the FP64 build collide their loaded values as they are.
 template <class C>
 __device__ inline void collide_c(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)

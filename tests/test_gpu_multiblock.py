@@ -223,8 +223,8 @@ def c1_oracle(lib32):
 @pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F32S])
 def test_c1_fp32_within_1e5(lib32, c1_oracle, storage):
     """Both FP32 paths (planes holding f, as the reference, and planes holding f - w) end the
-    full C1 run within 1e-5 of the oracle's drag coefficient (measured 6.2e-6 and 4.9e-6) and
-    within 2e-5 of its lift over the whole run (8.8e-6 and 3.3e-6)."""
+    full C1 run within 1e-5 of the oracle's drag coefficient (measured 5.5e-6 and 4.5e-6) and
+    within 2e-5 of its lift over the whole run (6.5e-6 and 3.8e-6)."""
     from mbl import cases
     ocd, ocl = c1_oracle
     _, ser = cases.run(lib32, mbl.c1_geometry(lib32), 20000, storage=storage, D=None)
