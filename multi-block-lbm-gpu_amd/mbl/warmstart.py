@@ -267,7 +267,7 @@ def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None):
         if done in marks:
             shots.append(b.raster(snap[3], snap[4]))
             shot_t.append(done)
-        if log and done % (8 * chunk) < m:
+        if log and done % (4 * chunk) < m:
             print(f"# {log}: step {done}/{steps} {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     b.sync()
     wall = time.perf_counter() - t0
