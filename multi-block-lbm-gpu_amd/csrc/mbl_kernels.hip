@@ -216,13 +216,10 @@ __device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t u
 }
 
 // BGK on one node: q[9] in, Q[9] out, macros out (evolution_block.h:60-70, 100-111).
-// Relaxation in the difference form Q = qa - omega (q - feq), qa the value q stands
-// for in the output's domain (q itself, or q + w_k when q is the shifted copy of an
-// unshifted input): q - feq is formed (in one fused multiply-add) before the scaling,
-// so the result carries one rounding of qa's size.
+// Relaxation in the difference form Q = q - omega (q - feq): q - feq is formed
+// before the scaling, so the result carries one rounding of q's size.
 template <bool SHIFT>
-__device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy,
-                               const real_t (&qa)[9])
+__device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
 {
 #pragma clang fp contract(off)
     // moments from shared partial sums (14 additions): east/west and north/south movers
@@ -237,32 +234,7 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
     uy = jy * inv;
     const real_t uu = fmar(ux, ux, uy * uy);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        // feq = w_k t (qeq's form); q - feq is one fused multiply-add, one rounding
-        const real_t t = fmar(rho, poly_k(k, ux, uy, uu), SHIFT ? drho : rho);
-        Q[k] = fmar(-omega, fmar(-(real_t)W64[k], t, q[k]), qa[k]);
-    }
-}
-
-// The collision of one node for codec C. Binary32 planes holding f itself are
-// collided on the deviations f - w_k (exact while f lies within [w_k / 2, 2 w_k],
-// Sterbenz, which the flow speeds of the method keep; outside it, one rounding of
-// the small result), so the moments and q - feq carry no rounding of f's size; the
-// relaxation lands on the unshifted f with one fused multiply-add. On the two-level
-// cylinder channel at Re = 20 (20,000 coarse steps) this takes the drag coefficient's
-// difference from an FP64 run of the same case from 1.1e-5 to 5.5e-6 (relative).
-// Shifted planes and the FP64 build collide their loaded values as they are.
-template <class C>
-__device__ inline void collide_c(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
-{
-    if constexpr (C::SHIFT || sizeof(real_t) != 4) {
-        collide<C::SHIFT>(q, omega, Q, rho, ux, uy, q);
-    } else {
-        real_t qs[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) qs[k] = q[k] - (real_t)W64[k];
-        collide<true>(qs, omega, Q, rho, ux, uy, q);
-    }
+    for (int k = 0; k < 9; k++) Q[k] = fmar(-omega, q[k] - qeq<SHIFT>(k, rho, drho, ux, uy, uu), q[k]);
 }
 
 template <bool SHIFT> __device__ inline void equilibrium(real_t rho, real_t ux, real_t uy, real_t (&Q)[9])
@@ -358,7 +330,7 @@ __device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>:
             }
             q[k] = C::ld(S[k][(size_t)jd * nx + id]);
         }
-        collide_c<C>(q, omega, Q, rho, ux, uy);
+        collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
     } else {
         // outer-edge Dirichlet equilibrium (evolution_block.h:47-57)
         rho = (real_t)c.rho0;
@@ -512,7 +484,7 @@ __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MOD
 #pragma unroll
         for (int k = 0; k < 9; k++) q[k] = C::ld(raw[r][k]);
         real_t Q[9], rho, ux, uy;
-        collide_c<C>(q, omega, Q, rho, ux, uy);
+        collide<C::SHIFT>(q, omega, Q, rho, ux, uy);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             const T v = fluid ? C::st(Q[k]) : (solid ? raw[r][OPP[k]] : raw[r][k]);
@@ -653,7 +625,7 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
         real_t qq[9], Q[9], rho, ux, uy;
 #pragma unroll
         for (int k = 0; k < 9; k++) qq[k] = q[k][v];
-        collide_c<C>(qq, omega, Q, rho, ux, uy);
+        collide<C::SHIFT>(qq, omega, Q, rho, ux, uy);
 #pragma unroll
         for (int k = 0; k < 9; k++) out[k][v] = C::st(Q[k]);
         mr[v] = (Lbd)rho;
