@@ -394,6 +394,8 @@ def main(argv=None):
                          "geometry and Re; phase: it predicts the field at a given phase")
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--net-seed", type=int, default=2024,
+                    help="seed of the network's initial weights; the batch order's seeds move with it")
     ap.add_argument("--storage", default="f32", choices=["f32", "f16s"], help="planes of the held-out runs")
     ap.add_argument("--snap-storage", default="f32", choices=["f32", "f16s", "f16d"],
                     help="planes of the snapshot runs (f16d: the 1-3-12 16-bit format)")
@@ -481,7 +483,7 @@ def main(argv=None):
 
     # 2. training; the learning rate steps down (lr, lr/3, lr/10 over 50%, 30%, 20% of the epochs)
     batch = min(args.batch, Xt.shape[0])
-    net = Net(lib, H, W, in_ch, out_ch, base=args.base, max_batch=max(batch, args.batch), seed=2024)
+    net = Net(lib, H, W, in_ch, out_ch, base=args.base, max_batch=max(batch, args.batch), seed=args.net_seed)
     t0 = time.perf_counter()
     hist = []
     e1 = max(1, int(0.5 * args.epochs))
@@ -490,13 +492,14 @@ def main(argv=None):
         done_ep = 0
         while done_ep < ep:  # calls of a few epochs: Adam state carries over, progress is printed
             k = min(ep - done_ep, 5 if args.model == "phase" else 25)
-            hist.extend(net.train(Xt, Tt, batch=batch, epochs=k, lr=lr, seed=99 + len(hist)))
+            hist.extend(net.train(Xt, Tt, batch=batch, epochs=k, lr=lr, seed=99 + len(hist) + args.net_seed - 2024))
             done_ep += k
             print(f"# epoch {len(hist)} loss {hist[-1]:.4e} {time.perf_counter() - t0:.1f} s", file=sys.stderr,
                   flush=True)
     hist = np.array(hist)
     out["train_s"] = round(time.perf_counter() - t0, 2)
     out["epochs"] = int(hist.size)
+    out["net_seed"] = args.net_seed
     out["epoch_loss_first_last"] = [float(hist[0]), float(hist[-1])]
     # validation on the held-out cases' snapshots, in the target units of the phase model
     vcases = range(len(tr) - nval, len(tr))
