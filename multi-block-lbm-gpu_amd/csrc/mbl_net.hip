@@ -204,10 +204,11 @@ extern "C" int mbl_net_create(mbl_handle_t h, int H, int W, int in_ch, int out_c
 extern "C" void mbl_net_destroy(mbl_net_t n)
 {
     if (!n) return;
-    if (n->s) hipStreamSynchronize(n->s);
+    // best-effort teardown: nothing to report an error to
+    if (n->s) (void)hipStreamSynchronize(n->s);
     for (void *p : n->allocs)
-        if (p) hipFree(p);
-    if (n->s) hipStreamDestroy(n->s);
+        if (p) (void)hipFree(p);
+    if (n->s) (void)hipStreamDestroy(n->s);
     delete n;
 }
 
@@ -370,17 +371,19 @@ static int stage(mbl_net *n, const float *in, const float *target, int N, bool d
     return 0;
 }
 
-static double loss_and_grad(mbl_net *n, int N)
+static int loss_and_grad(mbl_net *n, int N, double *loss)
 {
     const int P0 = P_of(n, N, 0);
     int nb = mse(n->s, n->out, 32, n->dtarget, n->out_ch, P0, n->gout, 32, n->lossp);
     backward(n, N);
     std::vector<float> lp(nb);
-    hipMemcpyAsync(lp.data(), n->lossp, sizeof(float) * nb, hipMemcpyDeviceToHost, n->s);
-    hipStreamSynchronize(n->s);
+    if (hipMemcpyAsync(lp.data(), n->lossp, sizeof(float) * nb, hipMemcpyDeviceToHost, n->s) != hipSuccess ||
+        hipStreamSynchronize(n->s) != hipSuccess)
+        return MBL_E_HIP;
     double s = 0.0;
     for (float x : lp) s += x;
-    return s / ((double)P0 * n->out_ch);
+    *loss = s / ((double)P0 * n->out_ch);
+    return 0;
 }
 
 extern "C" int mbl_net_forward(mbl_net_t n, const float *in, int N, float *out)
@@ -404,7 +407,8 @@ extern "C" int mbl_net_loss_grad(mbl_net_t n, const float *in, const float *targ
     refresh_weights(n);
     if (stage(n, in, target, N, false)) return MBL_E_HIP;
     forward(n, N);
-    double l = loss_and_grad(n, N);
+    double l = 0.0;
+    if (loss_and_grad(n, N, &l)) return MBL_E_HIP;
     if (loss) *loss = l;
     if (grad && hipMemcpy(grad, n->g, sizeof(float) * n->nparam, hipMemcpyDeviceToHost) != hipSuccess)
         return MBL_E_HIP;
@@ -421,7 +425,7 @@ extern "C" int mbl_net_train(mbl_net_t n, const float *in, const float *target, 
     float *din = nullptr, *dt = nullptr;
     if (hipMalloc(&din, sizeof(float) * P1 * n->in_ch * nsamples) != hipSuccess) return MBL_E_NOMEM;
     if (hipMalloc(&dt, sizeof(float) * P1 * n->out_ch * nsamples) != hipSuccess) {
-        hipFree(din);
+        (void)hipFree(din);
         return MBL_E_NOMEM;
     }
     int rc = 0;
@@ -441,7 +445,7 @@ extern "C" int mbl_net_train(mbl_net_t n, const float *in, const float *target, 
         }
         double sum = 0.0;
         int nb = 0;
-        for (int b0 = 0; b0 + batch <= nsamples; b0 += batch) {
+        for (int b0 = 0; b0 + batch <= nsamples && !rc; b0 += batch) {
             for (int k = 0; k < batch && !rc; k++) {
                 const int sidx = order[b0 + k];
                 if (hipMemcpyAsync(n->din + P1 * n->in_ch * k, din + P1 * n->in_ch * sidx,
@@ -450,16 +454,19 @@ extern "C" int mbl_net_train(mbl_net_t n, const float *in, const float *target, 
                                    sizeof(float) * P1 * n->out_ch, hipMemcpyDeviceToDevice, n->s) != hipSuccess)
                     rc = MBL_E_HIP;
             }
+            if (rc) break; // a batch that was not staged whole is not trained on
             refresh_weights(n);
             forward(n, batch);
-            sum += loss_and_grad(n, batch);
+            double l = 0.0;
+            if ((rc = loss_and_grad(n, batch, &l)) != 0) break;
+            sum += l;
             nb++;
             adam(n->s, n->w, n->g, n->m, n->v, n->nparam, lr, ++n->adam_t);
         }
         if (epoch_loss) epoch_loss[ep] = nb ? sum / nb : 0.0;
     }
     if (hipStreamSynchronize(n->s) != hipSuccess) rc = MBL_E_HIP;
-    hipFree(din);
-    hipFree(dt);
+    (void)hipFree(din);
+    (void)hipFree(dt);
     return rc;
 }
