@@ -356,10 +356,14 @@ class MultiBlockSim:
         self.flush()
         lib, h = self.lib, self.h
         n0 = lib.mbl_diagnostics_series(h, 0, 0, None, None, None)
+        if n0 < 0:
+            check(lib, n0, "diagnostics series")
         t0 = (C.c_int * max(1, n0))()
         r0 = np.zeros(max(1, n0))
         lib.mbl_diagnostics_series(h, 0, n0, t0, r0.ctypes.data_as(C.POINTER(C.c_double)), None)
         n1 = lib.mbl_diagnostics_series(h, 1, 0, None, None, None)
+        if n1 < 0:
+            check(lib, n1, "diagnostics series")
         t1 = (C.c_int * max(1, n1))()
         cd, cl = np.zeros(max(1, n1)), np.zeros(max(1, n1))
         lib.mbl_diagnostics_series(h, 1, n1, t1, cd.ctypes.data_as(C.POINTER(C.c_double)),
