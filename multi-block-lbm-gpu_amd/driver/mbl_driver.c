@@ -25,7 +25,7 @@ static double wall_s(void)
 {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
-    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
 static void *dalloc(size_t bytes)
