@@ -144,3 +144,19 @@ def test_harmonic_fit_recovers_mean_and_two_harmonics():
     noisy = snaps + rng.normal(0, 1e-3, snaps.shape)
     assert np.abs(ws.harmonic_fit(noisy, ph) - coef).max() < 5e-3
     assert ws.static_inputs(ws.c5_specs(2, seed=1), 16, 32).shape == (2, 16, 32, ws.STATIC_CH)
+
+
+def test_case_set_configuration():
+    """Training and held-out cases: D in {16, 24} finest cells (8 or 12 coarse), blockage
+    D/H ~ U[0.1, 0.3] (H in whole coarse cells), Re ~ U[80, 200], the cylinder 0.25 coarse
+    cells above the channel's centre line; the two seeded sets do not share a case."""
+    tr, te = ws.c5_specs(256, seed=101), ws.c5_specs(64, seed=202)
+    for s in tr + te:
+        dc = int(round(2.0 * s.R))
+        assert dc in (8, 12)
+        assert 80.0 <= s.Re <= 200.0
+        assert 0.1 - 0.01 <= dc / (s.LY - 1) <= 0.3 + 0.01
+        assert s.CY - (s.LY - 1) // 2 == 0.25
+    assert {int(round(4.0 * s.R)) for s in tr} == {16, 24}
+    key = lambda s: (s.LX, s.LY, round(s.Re, 9))
+    assert not ({key(s) for s in tr} & {key(s) for s in te})
