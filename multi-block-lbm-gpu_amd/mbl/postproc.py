@@ -4,7 +4,7 @@ The reference writes one line per coarse step to fdrag_t.dat ("t\\tCd\\tCl\\t",
 lbm/force_block.h:212-215) and error_L2.dat ("t\\tresidual",
 lbm/error_L2.h:85-88) and stops there. These functions read those files (or
 the series returned by the library) and reduce them to the quantities the
-benchmark cases are judged on: time-averaged Cd, Cl rms and amplitude,
+benchmark cases report: time-averaged Cd, Cl rms and amplitude,
 Strouhal number, and the time to a converged Cd.
 
 Time is in coarse steps (dt = 1 coarse lattice unit); St = f D / U with D the
