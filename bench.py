@@ -131,7 +131,7 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design, not a run)", file=sys.stderr)
         sys.exit(2)
     import mbl

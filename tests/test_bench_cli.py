@@ -1,5 +1,5 @@
 """bench.py's command line on the CPU: a request for more than one GPU is refused with a
-message (this build measures one GPU's share; the sharding is a design), and the
+message (the library measures one GPU's share; the sharding is a design), and the
 storage table it reports traffic for names every storage mode of the library."""
 import os
 import subprocess
