@@ -360,6 +360,12 @@ int mbl_last_diagnostics(mbl_handle_t h, double out[5]);
 int mbl_diagnostics_series(mbl_handle_t h, int kind, int cap, int *t, double *a, double *b);
 void mbl_diagnostics_clear(mbl_handle_t h);
 
+/* Divergence check (no reference counterpart; the reference checks nothing):
+ * every error_L2 record also carries the smallest density of any node, and a
+ * flush that finds one <= 0 records MBL_E_DIVERGED. Returns the step number t
+ * of the first such error_L2 call, or -1 while none has been seen. */
+int mbl_divergence_step(mbl_handle_t h);
+
 /* Velocity L2 distance from the analytic plane Poiseuille profile
  * u_a(y) = 4 U0[0] (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2 (the profile left
  * commented at lbm/init_block.h:20), over the FLUID nodes of all blocks with

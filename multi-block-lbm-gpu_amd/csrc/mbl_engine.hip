@@ -92,6 +92,9 @@ struct Residual {
     int *tblk = nullptr; // block of each residual tile
 };
 
+// doubles per error_L2 record: residual, sum |u - u_prev|^2 / m, sum |u|^2 / m, min density
+constexpr int L2_REC = 4;
+
 struct PendingDrag {
     int t;
     int m;
@@ -120,6 +123,7 @@ struct mbl_context {
     std::vector<int> ser_l2_t, ser_drag_t;
     std::vector<double> ser_l2, ser_cd, ser_cl;
     double last[5] = {0, 0, 0, 0, 0};
+    int diverged_step = -1; // first step whose smallest density was <= 0
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
 };
@@ -223,9 +227,9 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
     const int st = storage_mode(h->cfg);
     h->vx = store_elem_bytes(st) == 2 ? 1 : 4;
     h->tpb = 128;
-    if (hipMalloc(&h->d_l2, sizeof(double) * 3 * h->cap) != hipSuccess ||
+    if (hipMalloc(&h->d_l2, sizeof(double) * L2_REC * h->cap) != hipSuccess ||
         hipMalloc(&h->d_drag, sizeof(double) * 2 * h->cap) != hipSuccess ||
-        hipHostMalloc(&h->h_l2, sizeof(double) * 3 * h->cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&h->h_l2, sizeof(double) * L2_REC * h->cap, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&h->h_drag, sizeof(double) * 2 * h->cap, hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
         mbl_handle_destroy(h);
@@ -438,10 +442,20 @@ static int plane_io(mbl_handle_t h, const D2Q9 *pp, int which, int k, size_t n, 
     if (!h) return MBL_E_NO_HANDLE;
     if (!pp || k < 0 || k > 8 || !host) return set_err(h, MBL_E_INVALID);
     void *plane = which ? (void *)pp->f[k] : (void *)pp->F[k];
+    if (!plane) return set_err(h, MBL_E_INVALID);
+    int mode = storage_mode(h->cfg);
+    // D2Q9 carries no dimensions: n is checked against the device allocation that holds the plane
+    hipDeviceptr_t base = nullptr;
+    size_t alloc = 0;
+    if (hipMemGetAddressRange(&base, &alloc, (hipDeviceptr_t)plane) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(h, MBL_E_INVALID);
+    }
+    const size_t before = (size_t)((const char *)plane - (const char *)base);
+    if (before > alloc || n > (alloc - before) / store_elem_bytes(mode)) return set_err(h, MBL_E_INVALID);
     double *tmp = nullptr;
     HIPCHK(h, hipMalloc(&tmp, n * sizeof(double) + 8));
     int rc = 0;
-    int mode = storage_mode(h->cfg);
     if (read) {
         rc = launch_decode(h->stream, mode, plane, k, n, tmp);
         if (!rc && hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
@@ -1543,7 +1557,7 @@ extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
     if (!h) return MBL_E_NO_HANDLE;
     const size_t nl = h->pend_l2.size(), nd = h->pend_drag.size();
     if (!nl && !nd) return 0;
-    if (nl) HIPCHK(h, hipMemcpyAsync(h->h_l2, h->d_l2, sizeof(double) * 3 * nl, hipMemcpyDeviceToHost, h->stream));
+    if (nl) HIPCHK(h, hipMemcpyAsync(h->h_l2, h->d_l2, sizeof(double) * L2_REC * nl, hipMemcpyDeviceToHost, h->stream));
     if (nd) HIPCHK(h, hipMemcpyAsync(h->h_drag, h->d_drag, sizeof(double) * 2 * nd, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     const double U0_2 = std::sqrt(h->cfg.U0[0] * h->cfg.U0[0] + h->cfg.U0[1] * h->cfg.U0[1]);
@@ -1554,13 +1568,22 @@ extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
     }
     int rc = 0;
     for (size_t i = 0; i < nl; i++) {
-        double v = h->h_l2[3 * i];
+        const double *rec = h->h_l2 + L2_REC * i;
+        double v = rec[0];
         if (fl) fprintf(fl, "%d\t%g\n", h->pend_l2[i], v); // error_L2.h:87
         h->ser_l2_t.push_back(h->pend_l2[i]);
         h->ser_l2.push_back(v);
         h->last[0] = v;
         // a non-finite sum means a non-finite velocity somewhere; 0/0 of an all-rest field is not an error
-        if (!std::isfinite(h->h_l2[3 * i + 1]) || !std::isfinite(h->h_l2[3 * i + 2])) rc = MBL_E_NOT_FINITE;
+        if (!std::isfinite(rec[1]) || !std::isfinite(rec[2])) rc = MBL_E_NOT_FINITE;
+        // density <= 0 on some node: the run has diverged (first such step kept)
+        if (rec[3] <= 0.0) {
+            if (h->diverged_step < 0) {
+                h->diverged_step = h->pend_l2[i];
+                fprintf(stderr, "mbl: density %g <= 0 at step %d\n", rec[3], h->pend_l2[i]);
+            }
+            if (!rc) rc = MBL_E_DIVERGED;
+        }
     }
     for (size_t i = 0; i < nd; i++) {
         double fx = h->h_drag[2 * i], fy = h->h_drag[2 * i + 1];
@@ -1619,7 +1642,7 @@ extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *gri
     if (ensure_resid(h, N_areas, areas, pps_d, upsx, upsy)) return;
     Residual &R = h->resid;
     const size_t slot = h->pend_l2.size();
-    int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + 3 * slot);
+    int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + L2_REC * slot);
     if (rc) {
         set_err(h, rc);
         return;
@@ -1648,7 +1671,7 @@ static int ensure_resid(mbl_context *h, int N_areas, const AREA *areas, const D2
         }
         R.total = total;
         if (hipMalloc(&R.dd, sizeof(BlockDesc) * N_areas) != hipSuccess ||
-            hipMalloc(&R.partials, sizeof(double) * 2 * total) != hipSuccess ||
+            hipMalloc(&R.partials, sizeof(double) * 3 * total) != hipSuccess ||
             hipMalloc(&R.tblk, sizeof(int) * total) != hipSuccess ||
             hipMemcpy(R.dd, R.hd.data(), sizeof(BlockDesc) * N_areas, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(R.tblk, tblk.data(), sizeof(int) * total, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1668,10 +1691,10 @@ extern "C" int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 
     if (ensure_resid(h, N_areas, areas, pps_d, nullptr, nullptr)) return h->last_error;
     Residual &R = h->resid;
     double *d_out = nullptr;
-    HIPCHK(h, hipMalloc(&d_out, 3 * sizeof(double)));
+    HIPCHK(h, hipMalloc(&d_out, L2_REC * sizeof(double)));
     int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 1, h->cfg.U0[0], (double)h->cfg.LY,
                              d_out);
-    double v[3] = {0, 0, 0};
+    double v[L2_REC] = {0, 0, 0, 0};
     if (!rc && hipMemcpyAsync(v, d_out, sizeof(v), hipMemcpyDeviceToHost, h->stream) != hipSuccess) rc = MBL_E_HIP;
     if (hipStreamSynchronize(h->stream) != hipSuccess) rc = MBL_E_HIP;
     hipFree(d_out);
@@ -1746,6 +1769,13 @@ extern "C" void mbl_diagnostics_clear(mbl_handle_t h)
     h->ser_drag_t.clear();
     h->ser_cd.clear();
     h->ser_cl.clear();
+}
+
+extern "C" int mbl_divergence_step(mbl_handle_t h)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    return h->diverged_step;
 }
 
 // ---------------------------------------------------------------------------

@@ -98,6 +98,101 @@ def force_statistics(cd, cl, D, U, dt=1.0, transient=0.5):
     }
 
 
+def _integral(y, a, b):
+    """Integral over [a, b] (sample-index units) of the piecewise-linear interpolant of y."""
+    y = np.asarray(y, dtype=np.float64)
+
+    def at(t):
+        i = min(int(np.floor(t)), y.size - 2)
+        return y[i] + (t - i) * (y[i + 1] - y[i])
+
+    ia, ib = int(np.ceil(a)), int(np.floor(b))
+    if ib < ia:  # a and b inside one interval
+        return 0.5 * (at(a) + at(b)) * (b - a)
+    s = 0.5 * (at(a) + y[ia]) * (ia - a) + 0.5 * (y[ib] + at(b)) * (b - ib)
+    if ib > ia:
+        s += np.sum(0.5 * (y[ia:ib] + y[ia + 1:ib + 1]))
+    return float(s)
+
+
+def _peak(y, i):
+    """Extremum of the parabola through samples i-1, i, i+1 of y."""
+    if i <= 0 or i >= y.size - 1:
+        return float(y[i])
+    a, b, c = y[i - 1], y[i], y[i + 1]
+    den = a - 2.0 * b + c
+    if den == 0.0:
+        return float(b)
+    d = 0.5 * (a - c) / den
+    return float(b - 0.25 * (a - c) * d)
+
+
+def upward_crossings(x, level):
+    """Times (fractional sample index) where x crosses `level` upward, linearly interpolated."""
+    x = np.asarray(x, dtype=np.float64) - level
+    s = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+    return s + (-x[s]) / (x[s + 1] - x[s])
+
+
+def phase_locked_statistics(cd, cl, D, U, start=0, dt=1.0):
+    """Cd mean, Cl mean/rms/amplitude and St over the whole shedding periods of the series
+    from `start` on. The periods are bounded by upward crossings of Cl through its mean
+    (interpolated between samples, the mean taken over whole periods and the crossings
+    found again with it), the averages are integrals of the piecewise-linear series between
+    the first and last crossing, the amplitude is half the peak-to-peak of each period
+    (parabolic peak refinement) averaged over the periods, and St = D / (U T) with T the mean
+    period. A phase shift of the signal moves the window with it, so two runs that differ
+    only in phase give the same statistics. Returns None when fewer than two periods exist."""
+    cd = np.asarray(cd, dtype=np.float64)[start:]
+    cl = np.asarray(cl, dtype=np.float64)[start:]
+    level = float(np.mean(cl))
+    tc = None
+    for _ in range(3):
+        t = upward_crossings(cl, level)
+        if t.size < 3:
+            return None
+        tc = t
+        new = _integral(cl, tc[0], tc[-1]) / (tc[-1] - tc[0])
+        if new == level:
+            break
+        level = new
+    T = tc[-1] - tc[0]
+    n = tc.size - 1
+    cd_mean = _integral(cd, tc[0], tc[-1]) / T
+    cl_mean = _integral(cl, tc[0], tc[-1]) / T
+    cl_rms = np.sqrt(max(0.0, _integral((cl - cl_mean) ** 2, tc[0], tc[-1]) / T))
+    amps = []
+    for p in range(n):
+        i0, i1 = int(np.ceil(tc[p])), int(np.floor(tc[p + 1])) + 1
+        seg = cl[i0:i1]
+        if seg.size < 3:
+            continue
+        hi = _peak(cl, i0 + int(np.argmax(seg)))
+        lo = _peak(cl, i0 + int(np.argmin(seg)))
+        amps.append(0.5 * (hi - lo))
+    period = T / n * dt
+    return {"Cd_mean": float(cd_mean), "Cl_mean": float(cl_mean), "Cl_rms": float(cl_rms),
+            "Cl_amp": float(np.mean(amps)) if amps else float("nan"), "St": float(D / (U * period)),
+            "period": float(period), "periods": int(n)}
+
+
+def estimator_spread(cd, cl, D, U, start=0):
+    """The phase-locked statistics over the first and the second half of the whole periods
+    after `start`: their relative difference per quantity bounds what the estimator itself
+    (window choice, residual drift of the limit cycle) contributes to a comparison."""
+    full = phase_locked_statistics(cd, cl, D, U, start)
+    if full is None:
+        return None
+    cl_ = np.asarray(cl, dtype=np.float64)[start:]
+    tc = upward_crossings(cl_, full["Cl_mean"])
+    mid = int(tc[tc.size // 2])
+    a = phase_locked_statistics(cd[:start + mid + 2], cl[:start + mid + 2], D, U, start)
+    b = phase_locked_statistics(cd, cl, D, U, start + mid - 1)
+    if a is None or b is None:
+        return None
+    return {k: abs(a[k] - b[k]) / max(abs(full[k]), 1e-300) for k in ("Cd_mean", "Cl_rms", "Cl_amp", "St")}
+
+
 def time_to_converged(x, rel=0.005, window=None):
     """First step after which the running mean of x over `window` steps (or the
     value itself) stays within rel of its final value; -1 if never."""

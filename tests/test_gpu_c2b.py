@@ -1,0 +1,109 @@
+"""The three-level cylinder channel at Re = 100 (2048 x 512 coarse cells, two nested ratio-2
+blocks, cylinder D = 16 coarse = 64 finest cells) on the GPU.
+
+* FP64 build against the FP64 oracle over 300 coarse steps (1,200 finest): every field and
+  ring slot, the markers and the per-step residual / Cd / Cl series. Its coarse lines have
+  257 points, so the spline runs in the one-workgroup path for lines longer than 256 points
+  (the range where the reference's init_Spline3 races across thread blocks,
+  multi/FH/assit_multi.h:73-106).
+* The shedding statistics the case reports (Cd mean, Cl rms, Cl amplitude, St) after 50,000
+  coarse steps of the FP32 paths and of 16-bit planes against the FP64 build, with a
+  phase-locked estimator (whole periods between interpolated Cl zero crossings); the FP64
+  run alone gives the estimator's own spread.
+"""
+import numpy as np
+import pytest
+
+import mbl
+from mbl import abi, cases, postproc
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+D, U = 16.0, 0.1
+LONG = 50000
+START = LONG // 2
+
+
+def test_c2b_fp64_matches_oracle(lib64):
+    geom = mbl.c2b_geometry(lib64)
+    _, _, lc, _, _ = geom.init(lib64)
+    ncs = [lc[b].Nc for b in range(geom.g.number_line)]
+    assert max(ncs) == 257  # lines beyond one 256-thread block of the reference
+    sim = mbl.MultiBlockSim(lib64, geom)
+    o = oracle.OracleSim(geom.g, sim.cfg)
+    sim.step(300)
+    o.run(300)
+    for a in range(sim.na):
+        assert np.array_equal(sim.read(a, "solid"), o.get(a, "solid").astype(np.int32))
+        for fld in ("rho", "ux", "uy"):
+            g, r = sim.read(a, fld), o.get(a, fld)
+            assert np.max(np.abs(g - r)) / max(1e-3, np.max(np.abs(r))) < 1e-11, (a, fld)
+        for k in range(9):
+            err = np.max(np.abs(sim.read(a, ("F", k)) - o.get(a, ("F", k))))
+            assert err < 1e-12, (a, k, err)
+    for b in range(sim.nl):
+        assert list(sim.t3[b].marker) == o.markers(b)
+    l2, dr = sim.series()
+    cd_scale = max(1.0, np.max(np.abs(o.series(3))))
+    assert np.max(np.abs(l2["residual"] - o.series(0))) / np.max(np.abs(o.series(0))) < 1e-10
+    assert np.max(np.abs(dr["Cd"] - o.series(3))) / cd_scale < 1e-10
+    assert np.max(np.abs(dr["Cl"] - o.series(4))) / cd_scale < 1e-10
+    sim.close()
+    o.close()
+
+
+def long_run(lib, storage=abi.STORE_NATIVE):
+    _, ser = cases.run(lib, mbl.c2b_geometry(lib), LONG, storage=storage, chunk=5000)
+    return np.asarray(ser["Cd"]), np.asarray(ser["Cl"])
+
+
+@pytest.fixture(scope="module")
+def c2b_fp64(lib64):
+    cd, cl = long_run(lib64)
+    s = postproc.phase_locked_statistics(cd, cl, D, U, START)
+    spread = postproc.estimator_spread(cd, cl, D, U, START)
+    print("FP64", s, "estimator spread", spread)
+    return s, spread
+
+
+def rel_errors(s, ref):
+    return {k: abs(s[k] - ref[k]) / abs(ref[k]) for k in ("Cd_mean", "Cl_rms", "Cl_amp", "St")}
+
+
+def test_c2b_fp64_sheds(c2b_fp64):
+    """Literature sanity, not parity: a confined cylinder at Re = 100 sheds with St of order
+    0.15-0.2, and the estimator's own spread between the two halves of the window is small."""
+    s, spread = c2b_fp64
+    assert s["periods"] >= 20
+    assert 0.12 < s["St"] < 0.22
+    assert 1.0 < s["Cd_mean"] < 2.0
+    assert s["Cl_amp"] > 0.1
+    assert spread["Cd_mean"] < 1e-4 and spread["St"] < 1e-4
+
+
+@pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F32S])
+def test_c2b_fp32_statistics(lib32, c2b_fp64, storage):
+    """FP32 planes against the FP64 build, per result, after 50,000 coarse steps: within
+    1e-5 relative, or within the estimator's own spread on FP64 where that is larger."""
+    ref, spread = c2b_fp64
+    cd, cl = long_run(lib32, storage)
+    s = postproc.phase_locked_statistics(cd, cl, D, U, START)
+    err = rel_errors(s, ref)
+    print("storage", storage, s, "rel err", err)
+    for k, e in err.items():
+        assert e < max(1e-5, 2.0 * spread[k]), (k, e, spread[k])
+
+
+@pytest.mark.parametrize("storage,tol", [(abi.STORE_F16D, 5e-3), (abi.STORE_F16C, 1e-2)])
+def test_c2b_16bit_statistics(lib32, c2b_fp64, storage, tol):
+    """16-bit planes holding f - w (the precision variant of this case): Cd mean and St
+    within 0.5 % of FP64 for the 1-3-12 format (1 % for 1-4-11), Cl rms/amplitude within
+    5 %."""
+    ref, _ = c2b_fp64
+    cd, cl = long_run(lib32, storage)
+    s = postproc.phase_locked_statistics(cd, cl, D, U, START)
+    err = rel_errors(s, ref)
+    print("storage", storage, s, "rel err", err)
+    assert err["Cd_mean"] < tol and err["St"] < tol
+    assert err["Cl_rms"] < 5e-2 and err["Cl_amp"] < 5e-2

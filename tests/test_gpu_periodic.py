@@ -5,7 +5,7 @@ multi/FH/evolution_block.h:3-126 with periodic pulls).
 Tolerances, after 20 steps at tau = 0.6 from u ~ N(0, 0.01^2), about 5x the measured
 maximum error of each storage (in brackets):
   FP64 planes            |F - F_oracle| <= 1e-13
-  FP32 planes (f)        <= 2e-6   [3.6e-7: binary32 rounding of f ~ 0.03-0.45 in the collision]
+  FP32 planes (f)        <= 2e-7   [3.6e-8: binary32 rounding of f ~ 0.03-0.45 at the store]
   FP32 planes (f - w)    <= 1e-8   [2.2e-9]
   FP16 planes (f - w)    <= 1.5e-5 [2.7e-6: binary16, 11-bit significand of |f - w| <~ 0.01]
   BF16 planes (f - w)    <= 1.5e-4 [2.6e-5: bfloat16, 8-bit significand]
@@ -57,7 +57,7 @@ def test_fp64_matches_oracle(lib64, ref):
     assert err < 1e-13
 
 
-@pytest.mark.parametrize("storage,tol", [(abi.STORE_NATIVE, 2e-6), (abi.STORE_F32S, 1e-8),
+@pytest.mark.parametrize("storage,tol", [(abi.STORE_NATIVE, 2e-7), (abi.STORE_F32S, 1e-8),
                                          (abi.STORE_F16S, 1.5e-5), (abi.STORE_BF16S, 1.5e-4),
                                          (abi.STORE_F16C, 1e-5), (abi.STORE_F16D, 1e-4)])
 def test_fp32_storage_modes(lib32, ref, storage, tol):

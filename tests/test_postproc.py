@@ -42,3 +42,33 @@ def test_read_reference_formats(tmp_path):
     q.write_text("0\t0.5\n1\t0.25\n")
     t, r = postproc.read_error_l2(str(q))
     assert list(r) == [0.5, 0.25]
+
+
+@pytest.mark.parametrize("phase", [0.0, 1.3, 4.0])
+def test_phase_locked_statistics_of_a_sinusoid(phase):
+    """Whole periods between interpolated zero crossings: a sinusoid with a period of a
+    non-integer number of samples gives St, the mean, rms and amplitude to far below the
+    1e-5 bar, whatever its phase (a shifted signal gives the same numbers)."""
+    D, U = 16.0, 0.1
+    period = 981.37
+    t = np.arange(50000, dtype=float)
+    w = 2 * np.pi * t / period + phase
+    cl = 0.02 + 0.32 * np.sin(w)
+    cd = 1.318 + 0.011 * np.sin(2 * w + 0.3)
+    s = postproc.phase_locked_statistics(cd, cl, D, U, start=5000)
+    assert s["St"] == pytest.approx(D / (U * period), rel=1e-7)
+    assert s["Cd_mean"] == pytest.approx(1.318, rel=1e-7)
+    assert s["Cl_mean"] == pytest.approx(0.02, abs=1e-7)
+    assert s["Cl_rms"] == pytest.approx(0.32 / np.sqrt(2), rel=1e-6)
+    assert s["Cl_amp"] == pytest.approx(0.32, rel=1e-7)
+    sp = postproc.estimator_spread(cd, cl, D, U, start=5000)
+    assert max(sp.values()) < 1e-6
+
+
+def test_phase_locked_statistics_detects_a_changed_period():
+    """A 1e-4 change of the period moves St by 1e-4 (the estimator resolves it)."""
+    D, U = 16.0, 0.1
+    t = np.arange(50000, dtype=float)
+    a = postproc.phase_locked_statistics(np.ones_like(t), np.sin(2 * np.pi * t / 981.37), D, U, 5000)
+    b = postproc.phase_locked_statistics(np.ones_like(t), np.sin(2 * np.pi * t / (981.37 * 1.0001)), D, U, 5000)
+    assert (a["St"] - b["St"]) / a["St"] == pytest.approx(1e-4, rel=2e-3)
