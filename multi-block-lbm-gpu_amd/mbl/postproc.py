@@ -134,7 +134,7 @@ def upward_crossings(x, level):
     return s + (-x[s]) / (x[s + 1] - x[s])
 
 
-def phase_locked_statistics(cd, cl, D, U, start=0, dt=1.0):
+def phase_locked_statistics(cd, cl, D, U, start=0, dt=1.0, periods=None, near=None):
     """Cd mean, Cl mean/rms/amplitude and St over the whole shedding periods of the series
     from `start` on. The periods are bounded by upward crossings of Cl through its mean
     (interpolated between samples, the mean taken over whole periods and the crossings
@@ -142,16 +142,23 @@ def phase_locked_statistics(cd, cl, D, U, start=0, dt=1.0):
     the first and last crossing, the amplitude is half the peak-to-peak of each period
     (parabolic peak refinement) averaged over the periods, and St = D / (U T) with T the mean
     period. A phase shift of the signal moves the window with it, so two runs that differ
-    only in phase give the same statistics. Returns None when fewer than two periods exist."""
+    only in phase give the same statistics; `periods` fixes the number of whole periods
+    (from the first crossing after `start`) so that two runs are compared over the same
+    periods, and `near` (a step index, e.g. another run's "t_first") starts the window at
+    the crossing closest to it instead, so that a run whose phase differs a little still
+    covers the same periods. Returns None when fewer than two periods (or fewer than
+    `periods`) exist."""
     cd = np.asarray(cd, dtype=np.float64)[start:]
     cl = np.asarray(cl, dtype=np.float64)[start:]
     level = float(np.mean(cl))
     tc = None
     for _ in range(3):
         t = upward_crossings(cl, level)
-        if t.size < 3:
+        if near is not None and t.size:
+            t = t[int(np.argmin(np.abs(t + start - near))):]
+        if t.size < 3 or (periods is not None and t.size < periods + 1):
             return None
-        tc = t
+        tc = t if periods is None else t[:periods + 1]
         new = _integral(cl, tc[0], tc[-1]) / (tc[-1] - tc[0])
         if new == level:
             break
@@ -173,7 +180,7 @@ def phase_locked_statistics(cd, cl, D, U, start=0, dt=1.0):
     period = T / n * dt
     return {"Cd_mean": float(cd_mean), "Cl_mean": float(cl_mean), "Cl_rms": float(cl_rms),
             "Cl_amp": float(np.mean(amps)) if amps else float("nan"), "St": float(D / (U * period)),
-            "period": float(period), "periods": int(n)}
+            "period": float(period), "periods": int(n), "t_first": float(tc[0] + start)}
 
 
 def estimator_spread(cd, cl, D, U, start=0):

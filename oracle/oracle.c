@@ -695,6 +695,8 @@ struct orc_sim {
     double *ser[5];
     int nser, capser;
     int steps;
+    orc_block_hook hook; /* test hook: replaces the stream-collide of a block (NULL: stb1-3) */
+    void *hook_ctx;
 };
 
 /* lbm/force_block.h:2-75 (count) and :78-127 (enumerate, i-major) */
@@ -786,7 +788,12 @@ static void evolution_rec(orc_sim *s, int level)
                                        &ft[t->marker[1]], &ft[t->marker[2]], (1.0 * i / s->mi[level]));
             }
         }
-        for (int a = s->ASP[level]; a < s->ASP[level + 1]; a++) evolution_area_stb(&s->cfg, &s->blk[a], periodic);
+        for (int a = s->ASP[level]; a < s->ASP[level + 1]; a++) {
+            if (s->hook)
+                s->hook(s->hook_ctx, a, level, i);
+            else
+                evolution_area_stb(&s->cfg, &s->blk[a], periodic);
+        }
         if (s->number_line2 != 0) {
             for (int c = s->LCCP[level]; c < s->LCCP[level + 1]; c++)
                 c_c_boundary_transfer(&s->blk[s->LCC[c].areac_n], &s->blk[s->LCC[c].areaf_n], &s->LCC[c]);
@@ -1063,6 +1070,29 @@ int orc_sim_ring(const orc_sim *s, int b, int slot, int k, double *out)
 void orc_sim_markers(const orc_sim *s, int b, int out[3])
 {
     for (int p = 0; p < 3; p++) out[p] = s->t3[b].marker[p];
+}
+
+/* Test hooks: the cycle of evolution_rec with each block's stream-collide
+ * replaced by a caller function (the caller then sets the block's state
+ * through orc_sim_set), and single interface operations on one line. */
+void orc_sim_set_block_hook(orc_sim *s, orc_block_hook hook, void *ctx)
+{
+    s->hook = hook;
+    s->hook_ctx = ctx;
+}
+
+/* space_interpolation_Spline3 of line b into ring slot marker[0] (no rotation) */
+void orc_sim_spline_line(orc_sim *s, int b) { space_interpolation_spline3(s, b); }
+
+/* which 0: c_f_boundary_transfer of line b from slot marker[2]; 1: f_c_boundary_transfer */
+void orc_sim_transfer_line(orc_sim *s, int b, int which)
+{
+    if (which == 0) {
+        F_k3 *ft = &s->t3[b].f_t1;
+        c_f_boundary_transfer(&s->blk[s->LC[b].areaf_n], &s->LC[b], s->t3[b].N, &ft[s->t3[b].marker[2]]);
+    } else {
+        f_c_boundary_transfer(&s->blk[s->LF[b].areac_n], &s->blk[s->LF[b].areaf_n], &s->LF[b]);
+    }
 }
 
 int orc_sim_spline_M(const orc_sim *s, int b, double *out)

@@ -31,6 +31,8 @@ int orc_geo_init(const mbl_geometry *g, P4 *AreaDefine, AREA *Areas,
 
 /* ---- one whole simulation, allocated like Multi-block-LBM.cu:main ---- */
 typedef struct orc_sim orc_sim;
+/* test hook called in place of a block's stream-collide: (ctx, area, level, substep) */
+typedef void (*orc_block_hook)(void *ctx, int a, int level, int substep);
 
 /* Creates the blocks, runs init_General, counts/enumerates the drag links of
  * the last block and initialises the interface rings and tridiagonal arrays. */
@@ -61,6 +63,9 @@ int orc_sim_ring(const orc_sim *s, int b, int slot, int k, double *out);
 void orc_sim_markers(const orc_sim *s, int b, int out[3]);
 /* spline second-derivative columns of CF line b after the last spline (12*Nc) */
 int orc_sim_spline_M(const orc_sim *s, int b, double *out);
+void orc_sim_set_block_hook(orc_sim *s, orc_block_hook hook, void *ctx);
+void orc_sim_spline_line(orc_sim *s, int b);
+void orc_sim_transfer_line(orc_sim *s, int b, int which);
 
 /* ---- standalone kernels for unit tests ---- */
 /* tridiag(l, m, u) x = b for nrhs column-major right-hand sides, in place */

@@ -68,6 +68,9 @@ def load():
         "orc_periodic_run": (C.c_double, [vp, ip]),
         "orc_periodic_get": (ip, [vp, ip, dp]),
         "orc_threads": (ip, []),
+        "orc_sim_set_block_hook": (None, [vp, vp, vp]),
+        "orc_sim_spline_line": (None, [vp, ip]),
+        "orc_sim_transfer_line": (None, [vp, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -75,6 +78,9 @@ def load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+BLOCK_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int)
 
 
 def _dptr(a):
@@ -156,6 +162,21 @@ class OracleSim:
         out = np.zeros(n)
         self.lib.orc_sim_ring(self.p, b, slot, k, _dptr(out))
         return out
+
+    def set_block_hook(self, fn):
+        """fn(area, level, substep) replaces each block's stream-collide (None restores it)."""
+        if fn is None:
+            self._hook = None
+            self.lib.orc_sim_set_block_hook(self.p, None, None)
+            return
+        self._hook = BLOCK_HOOK(lambda ctx, a, level, i: fn(a, level, i))
+        self.lib.orc_sim_set_block_hook(self.p, C.cast(self._hook, C.c_void_p), None)
+
+    def spline_line(self, b):
+        self.lib.orc_sim_spline_line(self.p, b)
+
+    def transfer_line(self, b, which):
+        self.lib.orc_sim_transfer_line(self.p, b, which)
 
     def close(self):
         if self.p:

@@ -62,9 +62,13 @@ def long_run(lib, storage=abi.STORE_NATIVE):
 def c2b_fp64(lib64):
     cd, cl = long_run(lib64)
     s = postproc.phase_locked_statistics(cd, cl, D, U, START)
-    spread = postproc.estimator_spread(cd, cl, D, U, START)
-    print("FP64", s, "estimator spread", spread)
-    return s, spread
+    # the runs are compared over the same whole periods: one fewer than FP64 has, so a run
+    # whose phase lags a little still holds them all
+    n = s["periods"] - 1
+    ref = postproc.phase_locked_statistics(cd, cl, D, U, START, periods=n)
+    drift = postproc.estimator_spread(cd, cl, D, U, START)
+    print("FP64", ref, "first half vs second half", drift)
+    return ref, n, drift
 
 
 def rel_errors(s, ref):
@@ -73,26 +77,28 @@ def rel_errors(s, ref):
 
 def test_c2b_fp64_sheds(c2b_fp64):
     """Literature sanity, not parity: a confined cylinder at Re = 100 sheds with St of order
-    0.15-0.2, and the estimator's own spread between the two halves of the window is small."""
-    s, spread = c2b_fp64
-    assert s["periods"] >= 20
+    0.15-0.2. The two halves of the window differ by the limit cycle's remaining drift at
+    25,000-50,000 coarse steps (measured: Cd mean 1.5e-3, St 8e-4, Cl amplitude 3.4e-3); a
+    comparison of two runs over the same whole periods does not see it."""
+    s, n, drift = c2b_fp64
+    assert n >= 20
     assert 0.12 < s["St"] < 0.22
     assert 1.0 < s["Cd_mean"] < 2.0
     assert s["Cl_amp"] > 0.1
-    assert spread["Cd_mean"] < 1e-4 and spread["St"] < 1e-4
+    assert max(drift.values()) < 2e-2
 
 
 @pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F32S])
 def test_c2b_fp32_statistics(lib32, c2b_fp64, storage):
-    """FP32 planes against the FP64 build, per result, after 50,000 coarse steps: within
-    1e-5 relative, or within the estimator's own spread on FP64 where that is larger."""
-    ref, spread = c2b_fp64
+    """FP32 planes against the FP64 build, per result, after 50,000 coarse steps, over the
+    same whole shedding periods: within 1e-5 relative."""
+    ref, n, _ = c2b_fp64
     cd, cl = long_run(lib32, storage)
-    s = postproc.phase_locked_statistics(cd, cl, D, U, START)
+    s = postproc.phase_locked_statistics(cd, cl, D, U, START, periods=n, near=ref["t_first"])
     err = rel_errors(s, ref)
     print("storage", storage, s, "rel err", err)
     for k, e in err.items():
-        assert e < max(1e-5, 2.0 * spread[k]), (k, e, spread[k])
+        assert e < 1e-5, (k, e)
 
 
 @pytest.mark.parametrize("storage,tol", [(abi.STORE_F16D, 5e-3), (abi.STORE_F16C, 1e-2)])
@@ -100,9 +106,9 @@ def test_c2b_16bit_statistics(lib32, c2b_fp64, storage, tol):
     """16-bit planes holding f - w (the precision variant of this case): Cd mean and St
     within 0.5 % of FP64 for the 1-3-12 format (1 % for 1-4-11), Cl rms/amplitude within
     5 %."""
-    ref, _ = c2b_fp64
+    ref, n, _ = c2b_fp64
     cd, cl = long_run(lib32, storage)
-    s = postproc.phase_locked_statistics(cd, cl, D, U, START)
+    s = postproc.phase_locked_statistics(cd, cl, D, U, START, periods=n, near=ref["t_first"])
     err = rel_errors(s, ref)
     print("storage", storage, s, "rel err", err)
     assert err["Cd_mean"] < tol and err["St"] < tol
