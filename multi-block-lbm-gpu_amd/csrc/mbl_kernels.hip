@@ -30,8 +30,15 @@ __device__ constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
 __device__ constexpr double W64[9] = {4. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 9, 1. / 36, 1. / 36, 1. / 36, 1. / 36};
 
 // ---------------------------------------------------------------------------
-// Plane storage codecs. `ld` returns the stored quantity q in compute
-// precision: q = f for ST_NATIVE, q = f - w_k for the shifted modes.
+// Plane storage codecs. `ld` returns the stored quantity q in the codec's
+// arithmetic type R: q = f for ST_NATIVE, q = f - w_k for the shifted modes.
+// Planes of binary32 (and binary64) are updated in binary64 arithmetic: the
+// fused kernel is bound by HBM traffic there, and MI355X's binary64 vector rate
+// leaves its time per launch unchanged (16384^2 periodic block, measured), while
+// binary32 arithmetic biased the shedding of the three-level Re = 100 channel by
+// 1e-4 (Strouhal number, mean drag) against a binary64 run, and binary64
+// arithmetic over the same binary32 planes removes it (1e-7). The 16-bit
+// formats, whose kernels are bound by vector instructions, compute in binary32.
 // ---------------------------------------------------------------------------
 template <int MODE> struct Codec;
 
@@ -55,23 +62,26 @@ template <class T> __device__ inline GLOBAL_AS T *off32(GLOBAL_AS T *b, uint32_t
 template <> struct Codec<ST_NATIVE> {
     typedef Lbd T;
     typedef GLOBAL_AS Lbd GT;
+    typedef double R; // arithmetic type of the node update (see Codec's comment)
     static constexpr bool SHIFT = false;
-    __device__ static inline real_t ld(T v) { return (real_t)v; }
-    __device__ static inline T st(real_t v) { return (T)v; }
+    __device__ static inline R ld(T v) { return (R)v; }
+    __device__ static inline T st(R v) { return (T)v; }
 };
 
 #ifndef MBL_FP64
 template <> struct Codec<ST_F32S> {
     typedef float T;
     typedef GLOBAL_AS float GT;
+    typedef double R;
     static constexpr bool SHIFT = true;
-    __device__ static inline float ld(T v) { return v; }
-    __device__ static inline T st(float v) { return v; }
+    __device__ static inline R ld(T v) { return (R)v; }
+    __device__ static inline T st(R v) { return (T)v; }
 };
 
 template <> struct Codec<ST_F16S> {
     typedef _Float16 T;
     typedef GLOBAL_AS _Float16 GT;
+    typedef float R;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return (float)v; }
     __device__ static inline T st(float v) { return (_Float16)v; } // round to nearest even
@@ -80,6 +90,7 @@ template <> struct Codec<ST_F16S> {
 template <> struct Codec<ST_BF16S> {
     typedef uint16_t T;
     typedef GLOBAL_AS uint16_t GT;
+    typedef float R;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return __uint_as_float(((uint32_t)v) << 16); }
     __device__ static inline T st(float v)
@@ -138,6 +149,7 @@ template <int EB> struct Half16 {
 template <> struct Codec<ST_F16C> {
     typedef uint16_t T;
     typedef GLOBAL_AS uint16_t GT;
+    typedef float R;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return Half16<4>::ld(v); }
     __device__ static inline T st(float v) { return Half16<4>::st(v); }
@@ -147,6 +159,7 @@ template <> struct Codec<ST_F16C> {
 template <> struct Codec<ST_F16D> {
     typedef uint16_t T;
     typedef GLOBAL_AS uint16_t GT;
+    typedef float R;
     static constexpr bool SHIFT = true;
     __device__ static inline float ld(T v) { return Half16<3>::ld(v); }
     __device__ static inline T st(float v) { return Half16<3>::st(v); }
@@ -162,7 +175,7 @@ template <int MODE> __device__ inline double dec64(typename Codec<MODE>::T v, in
 
 template <int MODE> __device__ inline typename Codec<MODE>::T enc64(double f, int k)
 {
-    return Codec<MODE>::st((real_t)(Codec<MODE>::SHIFT ? f - W64[k] : f));
+    return Codec<MODE>::st((typename Codec<MODE>::R)(Codec<MODE>::SHIFT ? f - W64[k] : f));
 }
 
 // lbm/assit_func.h:2-9 in double
@@ -195,22 +208,21 @@ __device__ inline double fmar(double a, double b, double c) { return __builtin_f
 // passed separately so the shifted form keeps its small magnitude.
 // feq = w rho (1 + 3 eu + 4.5 eu^2 - 1.5 u.u)  (lbm/assit_func.h:2-9)
 // 3 eu + 4.5 eu^2 - 1.5 u.u for direction k
-__device__ inline real_t poly_k(int k, real_t ux, real_t uy, real_t uu)
+template <class R> __device__ inline R poly_k(int k, R ux, R uy, R uu)
 {
 #pragma clang fp contract(off)
-    real_t eu = (real_t)0.0; // e_k . u with e_k in {-1, 0, 1}^2: one rounding at most
+    R eu = (R)0.0; // e_k . u with e_k in {-1, 0, 1}^2: one rounding at most
     if (EX[k] == 1) eu = ux;
     if (EX[k] == -1) eu = -ux;
     if (EY[k] == 1) eu = (EX[k] == 0) ? uy : eu + uy;
     if (EY[k] == -1) eu = (EX[k] == 0) ? -uy : eu - uy;
-    return fmar(eu, fmar((real_t)4.5, eu, (real_t)3.0), (real_t)-1.5 * uu);
+    return fmar(eu, fmar((R)4.5, eu, (R)3.0), (R)-1.5 * uu);
 }
 
-template <bool SHIFT>
-__device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t uy, real_t uu)
+template <bool SHIFT, class R> __device__ inline R qeq(int k, R rho, R drho, R ux, R uy, R uu)
 {
 #pragma clang fp contract(off)
-    const real_t w = (real_t)W64[k], poly = poly_k(k, ux, uy, uu);
+    const R w = (R)W64[k], poly = poly_k<R>(k, ux, uy, uu);
     if (SHIFT) return w * fmar(rho, poly, drho);
     return w * fmar(rho, poly, rho);
 }
@@ -220,27 +232,26 @@ __device__ inline real_t qeq(int k, real_t rho, real_t drho, real_t ux, real_t u
 // for in the output's domain (q itself, or q + w_k when q is the shifted copy of an
 // unshifted input): q - feq is formed (in one fused multiply-add) before the scaling,
 // so the result carries one rounding of qa's size.
-template <bool SHIFT>
-__device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy,
-                               const real_t (&qa)[9])
+template <bool SHIFT, class R>
+__device__ inline void collide(const R (&q)[9], R omega, R (&Q)[9], R &rho, R &ux, R &uy, const R (&qa)[9])
 {
 #pragma clang fp contract(off)
     // moments from shared partial sums (14 additions): east/west and north/south movers
-    const real_t e3 = q[1] + q[5] + q[8], w3 = q[3] + q[6] + q[7];
-    const real_t n3 = q[2] + q[5] + q[6], s3 = q[4] + q[7] + q[8];
-    const real_t jx = e3 - w3, jy = n3 - s3;
-    const real_t s = ((q[0] + q[2]) + q[4]) + (e3 + w3);
-    real_t drho = SHIFT ? s : s - (real_t)1.0;
-    rho = SHIFT ? (real_t)1.0 + s : s;
-    const real_t inv = recip(rho);
+    const R e3 = q[1] + q[5] + q[8], w3 = q[3] + q[6] + q[7];
+    const R n3 = q[2] + q[5] + q[6], s3 = q[4] + q[7] + q[8];
+    const R jx = e3 - w3, jy = n3 - s3;
+    const R s = ((q[0] + q[2]) + q[4]) + (e3 + w3);
+    R drho = SHIFT ? s : s - (R)1.0;
+    rho = SHIFT ? (R)1.0 + s : s;
+    const R inv = recip(rho);
     ux = jx * inv;
     uy = jy * inv;
-    const real_t uu = fmar(ux, ux, uy * uy);
+    const R uu = fmar(ux, ux, uy * uy);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         // feq = w_k t (qeq's form); q - feq is one fused multiply-add, one rounding
-        const real_t t = fmar(rho, poly_k(k, ux, uy, uu), SHIFT ? drho : rho);
-        Q[k] = fmar(-omega, fmar(-(real_t)W64[k], t, q[k]), qa[k]);
+        const R t = fmar(rho, poly_k<R>(k, ux, uy, uu), SHIFT ? drho : rho);
+        Q[k] = fmar(-omega, fmar(-(R)W64[k], t, q[k]), qa[k]);
     }
 }
 
@@ -252,25 +263,25 @@ __device__ inline void collide(const real_t (&q)[9], real_t omega, real_t (&Q)[9
 // cylinder channel at Re = 20 (20,000 coarse steps) this takes the drag coefficient's
 // difference from an FP64 run of the same case from 1.1e-5 to 5.5e-6 (relative).
 // Shifted planes and the FP64 build collide their loaded values as they are.
-template <class C>
-__device__ inline void collide_c(const real_t (&q)[9], real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
+template <class C, class R = typename C::R>
+__device__ inline void collide_c(const R (&q)[9], R omega, R (&Q)[9], R &rho, R &ux, R &uy)
 {
-    if constexpr (C::SHIFT || sizeof(real_t) != 4) {
-        collide<C::SHIFT>(q, omega, Q, rho, ux, uy, q);
+    if constexpr (C::SHIFT || sizeof(R) != 4) {
+        collide<C::SHIFT, R>(q, omega, Q, rho, ux, uy, q);
     } else {
-        real_t qs[9];
+        R qs[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) qs[k] = q[k] - (real_t)W64[k];
-        collide<true>(qs, omega, Q, rho, ux, uy, q);
+        for (int k = 0; k < 9; k++) qs[k] = q[k] - (R)W64[k];
+        collide<true, R>(qs, omega, Q, rho, ux, uy, q);
     }
 }
 
-template <bool SHIFT> __device__ inline void equilibrium(real_t rho, real_t ux, real_t uy, real_t (&Q)[9])
+template <bool SHIFT, class R> __device__ inline void equilibrium(R rho, R ux, R uy, R (&Q)[9])
 {
 #pragma clang fp contract(off)
-    const real_t uu = fmar(ux, ux, uy * uy);
+    const R uu = fmar(ux, ux, uy * uy);
 #pragma unroll
-    for (int k = 0; k < 9; k++) Q[k] = qeq<SHIFT>(k, rho, rho - (real_t)1.0, ux, uy, uu);
+    for (int k = 0; k < 9; k++) Q[k] = qeq<SHIFT, R>(k, rho, rho - (R)1.0, ux, uy, uu);
 }
 
 // Bijective XCD-aware tile order: hardware workgroup b runs on XCD b % 8;
@@ -342,13 +353,14 @@ template <class T, int VX, class P> __device__ inline void vstore(P p, const T (
 // ---------------------------------------------------------------------------
 template <int MODE, bool PERIODIC>
 __device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9], int i, int j,
-                                 const Consts &c, real_t omega, real_t (&Q)[9], real_t &rho, real_t &ux, real_t &uy)
+                                 const Consts &c, typename Codec<MODE>::R omega, typename Codec<MODE>::R (&Q)[9], typename Codec<MODE>::R &rho, typename Codec<MODE>::R &ux, typename Codec<MODE>::R &uy)
 {
+    typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
     const int nx = d.nx, ny = d.ny;
     bool edge = !PERIODIC && (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
     if (!edge) {
-        real_t q[9];
+        R q[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             int id = i - EX[k], jd = j - EY[k];
@@ -361,18 +373,19 @@ __device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>:
         collide_c<C>(q, omega, Q, rho, ux, uy);
     } else {
         // outer-edge Dirichlet equilibrium (evolution_block.h:47-57)
-        rho = (real_t)c.rho0;
-        ux = (real_t)c.U0x;
-        uy = (real_t)c.U0y;
-        equilibrium<C::SHIFT>(rho, ux, uy, Q);
+        rho = (R)c.rho0;
+        ux = (R)c.U0x;
+        uy = (R)c.U0y;
+        equilibrium<C::SHIFT, R>(rho, ux, uy, Q);
     }
 }
 
 template <int MODE, bool PERIODIC, bool FLAGS, bool MACROS>
 __device__ inline void general_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
                                     typename Codec<MODE>::GT *const (&D)[9], int i, int j, const Consts &c,
-                                    real_t omega)
+                                    typename Codec<MODE>::R omega)
 {
+    typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
     typedef typename C::T T;
     const int nx = d.nx, ny = d.ny;
@@ -399,7 +412,7 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
         return;
     }
     if (flag == FLUID) {
-        real_t Q[9], rho, ux, uy;
+        R Q[9], rho, ux, uy;
         fluidlike<MODE, PERIODIC>(d, S, i, j, c, omega, Q, rho, ux, uy);
 #pragma unroll
         for (int k = 0; k < 9; k++) D[k][ixy] = C::st(Q[k]);
@@ -417,32 +430,32 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
         // neighbour's density, outlet imposes rho0 with the neighbour's velocity.
         const bool inlet = (flag == BOUNDARY_INLET);
         const size_t n = inlet ? ixy + 1 : ixy - 1;
-        real_t QN[9], rN, uN, vN;
+        R QN[9], rN, uN, vN;
         if (d.flag[n] == C_BOUNDARY) {
 #pragma unroll
             for (int k = 0; k < 9; k++) QN[k] = C::ld(S[k][n]);
-            rN = (real_t)d.rho[n];
-            uN = (real_t)d.ux[n];
-            vN = (real_t)d.uy[n];
+            rN = (R)d.rho[n];
+            uN = (R)d.ux[n];
+            vN = (R)d.uy[n];
         } else {
             fluidlike<MODE, PERIODIC>(d, S, inlet ? i + 1 : i - 1, j, c, omega, QN, rN, uN, vN);
         }
-        real_t rs, us, vs;
+        R rs, us, vs;
         if (inlet) {
             const double y = j * d.dx + d.base_y;
             rs = rN;
-            us = (real_t)(4.0 * c.umax * (c.LY - 1.5 - y) * (y - 0.5) / (c.LY - 2.0) / (c.LY - 2.0));
-            vs = (real_t)0.0;
+            us = (R)(4.0 * c.umax * (c.LY - 1.5 - y) * (y - 0.5) / (c.LY - 2.0) / (c.LY - 2.0));
+            vs = (R)0.0;
         } else {
-            rs = (real_t)c.rho0;
+            rs = (R)c.rho0;
             us = uN;
             vs = vN;
         }
-        const real_t uuN = uN * uN + vN * vN, uus = us * us + vs * vs;
+        const R uuN = uN * uN + vN * vN, uus = us * us + vs * vs;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            real_t v = qeq<C::SHIFT>(k, rs, rs - (real_t)1.0, us, vs, uus) +
-                       (QN[k] - qeq<C::SHIFT>(k, rN, rN - (real_t)1.0, uN, vN, uuN));
+            R v = qeq<C::SHIFT, R>(k, rs, rs - (R)1.0, us, vs, uus) +
+                       (QN[k] - qeq<C::SHIFT, R>(k, rN, rN - (R)1.0, uN, vN, uuN));
             D[k][ixy] = C::st(v);
         }
         if (MACROS) {
@@ -469,8 +482,9 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
 template <int MODE, bool MACROS, int NR>
 __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
                                     typename Codec<MODE>::GT *const (&D)[9], int i, const int (&jr)[NR],
-                                    const uint32_t (&ro)[NR], const Consts &c, real_t omega)
+                                    const uint32_t (&ro)[NR], const Consts &c, typename Codec<MODE>::R omega)
 {
+    typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
     typedef typename C::T T;
     const int nx = d.nx, ny = d.ny;
@@ -508,10 +522,10 @@ __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MOD
         if (jr[r] >= ny || special[r]) continue;
         const bool solid = (flag[r] == SOLID_SBB || flag[r] == MBL_SOLID_WALL);
         const bool fluid = (flag[r] == FLUID);
-        real_t q[9];
+        R q[9];
 #pragma unroll
         for (int k = 0; k < 9; k++) q[k] = C::ld(raw[r][k]);
-        real_t Q[9], rho, ux, uy;
+        R Q[9], rho, ux, uy;
         collide_c<C>(q, omega, Q, rho, ux, uy);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
@@ -542,8 +556,9 @@ __device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MOD
 template <int MODE, bool MACROS>
 __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
                                     typename Codec<MODE>::GT *const (&D)[9], int i, int j, uint32_t r0,
-                                    const Consts &c, real_t omega)
+                                    const Consts &c, typename Codec<MODE>::R omega)
 {
+    typedef typename Codec<MODE>::R R;
     const int jr[1] = {j};
     const uint32_t ro[1] = {r0};
     flagged_rows<MODE, MACROS, 1>(d, S, D, i, jr, ro, c, omega);
@@ -560,6 +575,7 @@ __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MOD
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
 __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, uint32_t r0, int x0)
 {
+    typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
     typedef typename C::T T;
     const Consts &c = d.bc;
@@ -574,7 +590,7 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
         S[k] = (const GT *)d.P[par][k];
         D[k] = (GT *)d.P[par ^ 1][k];
     }
-    const real_t omega = (real_t)d.omega;
+    const R omega = (R)d.omega;
 
     if constexpr (FLAGS && !PERIODIC && VX == 1) {
         flagged_node<MODE, MACROS>(d, S, D, x0, j, r0, c, omega);
@@ -617,7 +633,7 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
         xp = (xp >= nx) ? 0 : xp;
     }
     const uint32_t rc = r0;
-    real_t q[9][VX];
+    R q[9][VX];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         // pull: q_k(x) = F_k(x - e_k); EY = +1 reads row j-1, EY = -1 row j+1. The
@@ -650,7 +666,7 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
     Lbd mr[VX], mx[VX], my[VX];
 #pragma unroll
     for (int v = 0; v < VX; v++) {
-        real_t qq[9], Q[9], rho, ux, uy;
+        R qq[9], Q[9], rho, ux, uy;
 #pragma unroll
         for (int k = 0; k < 9; k++) qq[k] = q[k][v];
         collide_c<C>(qq, omega, Q, rho, ux, uy);
@@ -840,7 +856,7 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
         double eu = EX[k] * ux + EY[k] * uy;
         double poly = 3.0 * eu + 4.5 * eu * eu - 1.5 * uv;
         double q = C::SHIFT ? W64[k] * ((rho - 1.0) + rho * poly) : W64[k] * rho * (1.0 + poly);
-        T v = C::st((real_t)q);
+        T v = C::st((typename C::R)q);
         ((T *)d.P[0][k])[ixy] = v;
         ((T *)d.P[1][k])[ixy] = v;
     }
@@ -1509,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_init_raster(const RasterBlock *__restri
                           div / 3.0;
         const double neq = -(b.tau - 1.0) * W64[k] * rho * 3.0 * qg;
         const double q = (C::SHIFT ? W64[k] * ((rho - 1.0) + rho * poly) : W64[k] * rho * (1.0 + poly)) + neq;
-        const T v = C::st((real_t)q);
+        const T v = C::st((typename C::R)q);
         ((T *)b.P[0][k])[ixy] = v;
         ((T *)b.P[1][k])[ixy] = v;
     }
