@@ -127,10 +127,25 @@ def _peak(y, i):
     return float(b - 0.25 * (a - c) * d)
 
 
-def upward_crossings(x, level):
-    """Times (fractional sample index) where x crosses `level` upward, linearly interpolated."""
+def upward_crossings(x, level, hyst=None):
+    """Times (fractional sample index) where x crosses `level` upward, linearly interpolated.
+    A crossing counts only once x has been below level - hyst since the previous one
+    (hysteresis, default a tenth of the standard deviation), so sample noise near the level
+    (16-bit planes) does not add crossings."""
     x = np.asarray(x, dtype=np.float64) - level
-    s = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+    if hyst is None:
+        hyst = 0.1 * float(np.std(x))
+    cand = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
+    low = np.nonzero(x < -hyst)[0]
+    keep = []
+    last = -1
+    for i in cand:
+        # armed: some sample below -hyst after the previous kept crossing and at or before i
+        k = np.searchsorted(low, last + 1)
+        if k < low.size and low[k] <= i:
+            keep.append(i)
+            last = i
+    s = np.array(keep, dtype=np.int64)
     return s + (-x[s]) / (x[s + 1] - x[s])
 
 

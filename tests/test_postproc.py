@@ -72,3 +72,17 @@ def test_phase_locked_statistics_detects_a_changed_period():
     a = postproc.phase_locked_statistics(np.ones_like(t), np.sin(2 * np.pi * t / 981.37), D, U, 5000)
     b = postproc.phase_locked_statistics(np.ones_like(t), np.sin(2 * np.pi * t / (981.37 * 1.0001)), D, U, 5000)
     assert (a["St"] - b["St"]) / a["St"] == pytest.approx(1e-4, rel=2e-3)
+
+
+def test_crossings_ignore_sample_noise():
+    """Noise of a few percent of the amplitude (16-bit planes) adds no zero crossings, and the
+    statistics stay within the noise's own effect."""
+    D, U = 16.0, 0.1
+    period = 981.37
+    t = np.arange(50000, dtype=float)
+    rng = np.random.default_rng(5)
+    clean = 0.32 * np.sin(2 * np.pi * t / period)
+    noisy = clean + 0.01 * rng.standard_normal(t.size)
+    assert len(postproc.upward_crossings(noisy, 0.0)) == len(postproc.upward_crossings(clean, 0.0))
+    s = postproc.phase_locked_statistics(np.ones_like(t), noisy, D, U, start=5000)
+    assert s["St"] == pytest.approx(D / (U * period), rel=1e-4)
