@@ -8,6 +8,15 @@ seconds. The roofline figure uses the algorithmic 72 B per node update of
 FP32 planes (9 loads + 9 stores of 4 B) against the 8 TB/s HBM peak, with the
 per-launch time taken from HIP events on the library's stream.
 
+FP32 planes are updated in FP64 arithmetic (same per-launch time as FP32
+arithmetic on this HBM-bound kernel; `dtype` names the arithmetic, `storage`
+the planes).
+
+The metric's second half, the time to a drag coefficient converged within
++-0.5 %, is measured on the two-level (Re = 20) and three-level (Re = 100)
+cylinder channels for plain FP32 planes against 16-bit 1-3-12 planes
+(`time_to_converged_cd`; about 40 s, --no-converge skips it).
+
 The CPU baseline times the FP64 oracle (three passes per step, OpenMP) on a
 bounded sample of the same workload (a smaller periodic block, a few steps).
 
@@ -112,6 +121,38 @@ def cpu_baseline(target_s):
                       f"{secs:.1f} s"}
 
 
+def converged_cd():
+    """The metric's second half: coarse steps and wall seconds until the drag coefficient stays
+    within +-0.5 % of the FP64 run's converged value, plain FP32 planes against 16-bit 1-3-12
+    planes, on the two-level channel at Re = 20 (steady Cd, 20,000 coarse steps) and the
+    three-level channel at Re = 100 (Cd averaged over one shedding period, 50,000 coarse
+    steps), with the per-result error of each against FP64."""
+    from mbl import precision
+
+    print("# bench: time to converged Cd (two-level Re=20, three-level Re=100)", file=sys.stderr, flush=True)
+    res = {}
+    for case in ("c1", "c2b"):
+        r = precision.case_c1(["f32", "f16d"]) if case == "c1" else precision.case_c2b(["f32", "f16d"])
+        rows = {x["storage"]: x for x in r["rows"]}
+        f32 = rows["f32"]
+        entry = {"node_updates_per_step": r["node_updates_per_step"], "steps_run": r["steps"]}
+        for st, x in rows.items():
+            e = {"steps": x["t_conv_steps"], "wall_s": None if x["t_conv_s"] is None else round(x["t_conv_s"], 4),
+                 "mlups": round(x["mlups"], 1)}
+            if st != "f64":
+                errs = {k: round(v, 8) for k, v in x.items() if k.endswith("_err")}
+                e["err_vs_f64"] = errs
+                if x["t_conv_s"] is not None and f32["t_conv_s"]:
+                    e["wall_vs_f32"] = round(x["t_conv_s"] / f32["t_conv_s"], 4)
+            entry[st] = e
+        res["two_level_re20" if case == "c1" else "three_level_re100"] = entry
+    ws = os.path.join(REPO, "profiles", "warmstart_c5.json")
+    if os.path.exists(ws):
+        with open(ws) as fh:
+            res["learned_warm_start_recorded"] = json.load(fh)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +162,7 @@ def main():
     ap.add_argument("--storage", type=int, default=0, help="0 f32, 1 f32 shifted, 2 f16 shifted, 3 bf16 shifted, 4 1-4-11 half shifted, 5 1-3-12 half shifted")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged-Cd runs (about 40 s)")
     ap.add_argument("--variants", action="store_true", help="also time the 16-bit storage modes")
     ap.add_argument("--vx", type=int, default=0, help="nodes per thread (0: library default)")
     ap.add_argument("--tpb", type=int, default=0, help="threads per workgroup (0: library default)")
@@ -154,11 +196,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64" if args.storage in (0, 1) else "f32",
         "data": "synthetic (seeded random velocity field)",
         "config": {"workload": f"periodic {args.n}x{args.n} D2Q9 BGK block, tau=0.6, "
                                f"{STORAGE_NAMES[args.storage]} planes, {bpn} B/node/step",
-                   "nodes": r["nodes"], "storage": STORAGE_NAMES[args.storage]},
+                   "nodes": r["nodes"], "storage": STORAGE_NAMES[args.storage],
+                   "arithmetic": "f64" if args.storage in (0, 1) else "f32"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.storage),
                      "kernel": "k_stream_collide (periodic)", "per_launch_ms": round(r["per_launch_ms"], 5),
@@ -180,6 +223,8 @@ def main():
                                       "achieved_gbs": round(b * v["nodes"] / (v["per_launch_ms"] * 1e-3) / 1e9, 1),
                                       "bytes_per_node": b}
         out["storage_variants"] = var
+    if not args.no_converge:
+        out["time_to_converged_cd"] = converged_cd()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

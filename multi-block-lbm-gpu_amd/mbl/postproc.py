@@ -130,11 +130,12 @@ def _peak(y, i):
 def upward_crossings(x, level, hyst=None):
     """Times (fractional sample index) where x crosses `level` upward, linearly interpolated.
     A crossing counts only once x has been below level - hyst since the previous one
-    (hysteresis, default a tenth of the standard deviation), so sample noise near the level
+    (hysteresis, default half the standard deviation, about a third of a sinusoid's
+    amplitude), so sample noise near the level
     (16-bit planes) does not add crossings."""
     x = np.asarray(x, dtype=np.float64) - level
     if hyst is None:
-        hyst = 0.1 * float(np.std(x))
+        hyst = 0.5 * float(np.std(x))
     cand = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
     low = np.nonzero(x < -hyst)[0]
     keep = []
