@@ -5,7 +5,8 @@ drag coefficient, on the two-level (C1) and three-level (C2b) cylinder channels.
                             [--markdown FILE]
 
 For each case the FP64 build runs first (the reference of every comparison; it equals the
-FP64 oracle to 1e-12 on these layouts), then each storage of the FP32 build. Reported per run:
+FP64 C restatement of the reference to 1e-12 on these layouts), then each storage of the FP32
+build. Reported per run:
 
 * C1 (Re = 20, steady, 20,000 coarse steps): the final Cd and its relative error; the
   analytic-profile velocity L2 is reported separately (`poiseuille`).
