@@ -350,6 +350,23 @@ void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *grid, AREA *ar
 
 /* ---- additions (no counterpart in the reference) ---- */
 
+/* Allocation-free variant of error_L2. No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It computes
+ * the residual of error_L2 (same kernels and summation order, so the same
+ * bits) and updates u_prev <- u, but records nothing in the diagnostics series
+ * and writes no file: d_out (caller device memory, 4 doubles) receives
+ * {residual, sum|u - u_prev|^2/m, sum|u|^2/m, smallest density}. It calls no
+ * allocation routine: the block descriptors and per-tile partial sums live in
+ * a fixed 4 MiB device array per GPU (held until the process ends), which
+ * bounds a layout to 64 blocks and about 170,000 residual tiles of 64 x 16
+ * nodes (about 178M nodes); larger layouts are rejected with MBL_E_INVALID
+ * before any work. Synchronous on the handle's stream. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other threads, other
+ * handles on other streams) race. */
+int mbl_error_L2_noalloc(mbl_handle_t h, Lbi N_areas, const AREA *areas, const D2Q9 *pps_d,
+                         Lbd *const *upsx, Lbd *const *upsy, double *d_out);
+
 /* Latest values recorded by error_L2 / getDrag_block, after a flush:
  * residual, Fx, Fy, Cd, Cl. */
 int mbl_last_diagnostics(mbl_handle_t h, double out[5]);

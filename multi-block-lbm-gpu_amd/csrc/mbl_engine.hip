@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -124,11 +125,43 @@ struct mbl_context {
     std::vector<double> ser_l2, ser_cd, ser_cl;
     double last[5] = {0, 0, 0, 0, 0};
     int diverged_step = -1; // first step whose smallest density was <= 0
+    void *scratch = nullptr;  // device address of this device's allocation-free scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
 };
 
 static mbl_context *g_current = nullptr;
+
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry point (mbl_error_L2_noalloc): a
+// file-scope array of fixed size, one copy per GPU for the life of the process.
+// In host code `scratch` names a host-side stand-in, not the device array: its
+// device address is looked up once per device (hipGetSymbolAddress, at handle
+// creation with that device current) and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr size_t MBL_NOALLOC_SCRATCH_BYTES = (size_t)4 << 20;
+constexpr int MBL_NOALLOC_MAX_BLOCKS = 64;
+static __device__ __align__(256) unsigned char scratch[MBL_NOALLOC_SCRATCH_BYTES];
+constexpr int MBL_MAX_DEVICES = 64;
+static void *g_scratch_ptr[MBL_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The current device's scratch address (looked up on the first call per device).
+static int scratch_ptr(int device, void **out)
+{
+    if (device < 0 || device >= MBL_MAX_DEVICES) return MBL_E_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return MBL_E_HIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
 
 static int set_err(mbl_context *h, int code)
 {
@@ -220,6 +253,12 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
         return MBL_E_HIP;
     }
     h->stream = h->own_stream;
+    // the allocation-free entry point's scratch address on this device
+    if (scratch_ptr(h->device, &h->scratch) != 0) {
+        hipStreamDestroy(h->own_stream);
+        delete h;
+        return MBL_E_HIP;
+    }
     // default launch shape of the fused kernel per plane storage, from launch-shape sweeps on
     // MI355X (16384^2 periodic block): 4 nodes/thread for 4-byte planes, 1 for 2-byte planes,
     // 128 threads per workgroup. Flagged (non-periodic) blocks default to 1 node/thread
@@ -1642,7 +1681,7 @@ extern "C" void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *gri
     if (ensure_resid(h, N_areas, areas, pps_d, upsx, upsy)) return;
     Residual &R = h->resid;
     const size_t slot = h->pend_l2.size();
-    int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + L2_REC * slot);
+    int rc = launch_residual(h->stream, R.dd, R.tblk, (int)R.hd.size(), R.total, R.partials, 0, 0.0, 0.0, h->d_l2 + L2_REC * slot);
     if (rc) {
         set_err(h, rc);
         return;
@@ -1692,13 +1731,55 @@ extern "C" int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 
     Residual &R = h->resid;
     double *d_out = nullptr;
     HIPCHK(h, hipMalloc(&d_out, L2_REC * sizeof(double)));
-    int rc = launch_residual(h->stream, R.dd, R.tblk, R.total, R.partials, 1, h->cfg.U0[0], (double)h->cfg.LY,
+    int rc = launch_residual(h->stream, R.dd, R.tblk, (int)R.hd.size(), R.total, R.partials, 1, h->cfg.U0[0], (double)h->cfg.LY,
                              d_out);
     double v[L2_REC] = {0, 0, 0, 0};
     if (!rc && hipMemcpyAsync(v, d_out, sizeof(v), hipMemcpyDeviceToHost, h->stream) != hipSuccess) rc = MBL_E_HIP;
     if (hipStreamSynchronize(h->stream) != hipSuccess) rc = MBL_E_HIP;
     hipFree(d_out);
     *out = v[0];
+    return set_err(h, rc);
+}
+
+// error_L2 without any allocation: block descriptors and per-tile partial sums in the
+// device's fixed scratch array (no tile table: each tile finds its block by a search),
+// the record {residual, sum|u-u_prev|^2/m, sum|u|^2/m, min rho} written to the caller's
+// device buffer. Same kernels and summation order as error_L2, so the same bits.
+extern "C" int mbl_error_L2_noalloc(mbl_handle_t h, Lbi N_areas, const AREA *areas, const D2Q9 *pps_d,
+                                    Lbd *const *upsx, Lbd *const *upsy, double *d_out)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (N_areas <= 0 || N_areas > MBL_NOALLOC_MAX_BLOCKS || !areas || !pps_d || !upsx || !upsy || !d_out) {
+        fprintf(stderr, "mbl_error_L2_noalloc: invalid arguments (1 to %d blocks)\n", MBL_NOALLOC_MAX_BLOCKS);
+        return MBL_E_INVALID;
+    }
+    BlockDesc hd[MBL_NOALLOC_MAX_BLOCKS];
+    long long total = 0;
+    for (int a = 0; a < N_areas; a++) {
+        if (areas[a].nx < 1 || areas[a].ny < 1 || areas[a].m < 1) return MBL_E_INVALID;
+        BlockDesc d = make_desc(h->cfg, areas[a], pps_d[a], 0, upsx[a], upsy[a]);
+        d.rtiles_x = (areas[a].nx + 63) / 64;
+        d.rtile_begin = (int)std::min<long long>(total, 0x7fffffff);
+        total += (long long)d.rtiles_x * ((areas[a].ny + 15) / 16);
+        hd[a] = d;
+    }
+    const size_t desc_bytes = (sizeof(BlockDesc) * N_areas + 255) & ~(size_t)255;
+    const size_t need = desc_bytes + sizeof(double) * 3 * (size_t)total;
+    if (need > MBL_NOALLOC_SCRATCH_BYTES) {
+        fprintf(stderr, "mbl_error_L2_noalloc: layout needs %zu bytes of scratch, the maximum is %zu\n", need,
+                MBL_NOALLOC_SCRATCH_BYTES);
+        return MBL_E_INVALID;
+    }
+    for (int a = 0; a < N_areas; a++)
+        if (!pps_d[a].ux || !pps_d[a].uy || !pps_d[a].rho || !upsx[a] || !upsy[a]) return MBL_E_INVALID;
+    if (!h->scratch) return set_err(h, MBL_E_HIP);
+    BlockDesc *d_desc = (BlockDesc *)h->scratch;
+    double *partials = (double *)((char *)h->scratch + desc_bytes);
+    HIPCHK(h, hipMemcpyAsync(d_desc, hd, sizeof(BlockDesc) * N_areas, hipMemcpyHostToDevice, h->stream));
+    int rc = launch_residual(h->stream, d_desc, nullptr, N_areas, (int)total, partials, 0, 0.0, 0.0, d_out);
+    // the descriptors' host copy lives on this stack frame: the call returns once the work is done
+    if (hipStreamSynchronize(h->stream) != hipSuccess && !rc) rc = MBL_E_HIP;
     return set_err(h, rc);
 }
 

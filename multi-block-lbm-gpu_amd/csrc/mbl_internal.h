@@ -135,9 +135,9 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
 int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
                      const BlockDesc *d_blocks, uint32_t parity_mask);
 int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N);
-// d_tblk[t]: block of residual tile t.
-int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, int total_rtiles, double *partials,
-                    int mode, double umax, double LY, double *d_out);
+// d_tblk[t]: block of residual tile t (null: found from the descriptors' rtile_begin).
+int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, int ndesc, int total_rtiles,
+                    double *partials, int mode, double umax, double LY, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
 int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes);

@@ -558,7 +558,6 @@ __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MOD
                                     typename Codec<MODE>::GT *const (&D)[9], int i, int j, uint32_t r0,
                                     const Consts &c, typename Codec<MODE>::R omega)
 {
-    typedef typename Codec<MODE>::R R;
     const int jr[1] = {j};
     const uint32_t ro[1] = {r0};
     flagged_rows<MODE, MACROS, 1>(d, S, D, i, jr, ro, c, omega);
@@ -1340,12 +1339,26 @@ template <int MODE> __global__ __launch_bounds__(256) void k_drag_batch(const Dr
 // analytic plane Poiseuille profile u_a(y) = 4 umax (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2,
 // without touching u_prev.
 __global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restrict__ descs,
-                                                        const int *__restrict__ tblk, double *partials, int mode,
-                                                        double umax, double LY)
+                                                        const int *__restrict__ tblk, int ndesc, double *partials,
+                                                        int mode, double umax, double LY)
 {
     __shared__ double sh[16];
     const int t = blockIdx.x;
-    const BlockDesc &d = descs[tblk[t]];
+    int blk;
+    if (tblk) {
+        blk = tblk[t];
+    } else { // no tile table: the last block whose first residual tile is <= t
+        int lo = 0, hi = ndesc - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].rtile_begin <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        blk = lo;
+    }
+    const BlockDesc &d = descs[blk];
     const int lt = t - d.rtile_begin;
     const int ty = lt / d.rtiles_x, tx = lt - ty * d.rtiles_x;
     const int i = tx * 64 + (threadIdx.x & 63);
@@ -1766,13 +1779,13 @@ int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N)
     return 0;
 }
 
-int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, int total_rtiles, double *partials,
-                    int mode, double umax, double LY, double *d_out)
+int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, int ndesc, int total_rtiles,
+                    double *partials, int mode, double umax, double LY, double *d_out)
 {
-    if (total_rtiles <= 0) return MBL_E_INVALID;
+    if (total_rtiles <= 0 || ndesc <= 0) return MBL_E_INVALID;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, d_tblk, partials, mode, umax,
-                       LY);
+    hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, d_tblk, ndesc, partials, mode,
+                       umax, LY);
     MBL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, total_rtiles, (const double *)partials, d_out);
     MBL_LAUNCH_CHECK();
