@@ -383,6 +383,15 @@ void mbl_diagnostics_clear(mbl_handle_t h);
  * of the first such error_L2 call, or -1 while none has been seen. */
 int mbl_divergence_step(mbl_handle_t h);
 
+/* Launches of the scheduled refinement cycle that evolution() issues from
+ * `level` (no reference counterpart): evolution() records the cycle of
+ * evolution.h:2-65 in its order, derives every dependence between its
+ * operations and merges independent operations of one kind into shared
+ * launches (stream-collides of adjacent levels, interface ops, splines).
+ * Returns the total and the count per kind (pointers may be NULL), or
+ * MBL_E_INVALID before the first evolution() call from that level. */
+int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int *interface_ops, int *splines);
+
 /* Velocity L2 distance from the analytic plane Poiseuille profile
  * u_a(y) = 4 U0[0] (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2 (the profile left
  * commented at lbm/init_block.h:20), over the FLUID nodes of all blocks with

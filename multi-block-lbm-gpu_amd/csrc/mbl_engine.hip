@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -33,6 +34,41 @@ struct Sub {
 };
 typedef std::vector<Sub> Batch;
 
+// What one interface op reads and writes (see build_layout's op_access).
+struct OpAccess {
+    std::vector<uint64_t> rd, wr; // node units (sorted)
+    uint32_t epochs = 0;          // levels whose collide epoch the op belongs to
+    int ring_line = -1;           // line whose ring the op writes (spline) or reads (temporal, c_f)
+    bool ring_write = false;
+};
+
+// One operation of the refinement cycle as evolution.h issues it, recorded with the level
+// parities and ring markers in force at that point.
+struct CycleItem {
+    int kind;       // 0 stream-collide of `level`, 1 interface ops, 2 spline
+    int level;
+    int off, count; // interface / spline ops: range of the layout's op table
+    double xm;      // temporal interpolation point of the interface ops
+    bool temporal;  // the range holds temporal ops (xm matters)
+    uint32_t pm;
+    MarkerSet ms;
+};
+
+// A merged launch of the scheduled cycle: items of one kind with no dependence among them.
+struct CycleLaunch {
+    int kind;
+    std::vector<int> items;  // indices into the recorded item sequence
+    int lo = 0, hi = 0;      // collide: levels lo..hi (adjacent, same nodes per thread)
+    int op_off = 0, op_count = 0, max_n = 0, max_nc = 0; // interface/spline: range of the program's op table
+};
+
+struct CycleProgram {
+    bool built = false;
+    int nitems = 0;
+    std::vector<CycleLaunch> launches;
+    LineOp *dops = nullptr; // ops of every interface/spline launch, each launch contiguous
+};
+
 // Device tables for one multi-block layout (areas, planes, lines) as passed
 // to evolution(). Rebuilt when the caller's pointers or geometry change.
 struct Layout {
@@ -45,7 +81,7 @@ struct Layout {
     BlockDesc *db = nullptr;
     int tiles[MBL_MAX_LEVEL] = {0};
     int tile_off[MBL_MAX_LEVEL] = {0}; // first entry of each level in dtile
-    int *dtile = nullptr;              // per tile: descriptor index within its level
+    int *dtile = nullptr;              // per tile: index of its block in db (tiles numbered across levels)
     int rows = 0;                      // rows per stream-collide tile
     int vx[MBL_MAX_LEVEL] = {0};
     bool periodic = false;
@@ -57,8 +93,15 @@ struct Layout {
     Batch bT[MBL_MAX_LEVEL], bCC[MBL_MAX_LEVEL], bS[MBL_MAX_LEVEL], bCF[MBL_MAX_LEVEL];
     std::vector<std::vector<int>> batch_lines_T, batch_lines_S, batch_lines_CF; // line ids per level
     std::map<uint64_t, hipGraphExec_t> graphs;
+    std::vector<int> level_of;  // level of each block
+    std::vector<OpAccess> acc;  // per op of dops: what it reads and writes (cycle scheduling)
+    CycleProgram prog[MBL_MAX_LEVEL]; // scheduled launches of evolution() from each level
     void release()
     {
+        for (auto &p : prog) {
+            if (p.dops) hipFree(p.dops);
+            p = CycleProgram();
+        }
         for (auto &g : graphs) hipGraphExecDestroy(g.second);
         graphs.clear();
         if (db) hipFree(db);
@@ -577,10 +620,13 @@ static BlockDesc make_desc(const mbl_config &cfg, const AREA &a, const D2Q9 &pp,
 // Reads the node flags of a block and checks the rules the fused kernel
 // relies on: bounce-back solids off the block edge, outlets with a fluid or
 // ghost x-1 neighbour.
-static int validate_flags(mbl_context *h, const AREA &a, const int *d_solid, bool periodic, bool *any_solid)
+static int validate_flags(mbl_context *h, const AREA &a, const int *d_solid, bool periodic, bool *any_solid,
+                          std::vector<int> *flags_out = nullptr)
 {
     const size_t n = (size_t)a.nx * a.ny;
-    std::vector<int> f(n);
+    std::vector<int> local;
+    std::vector<int> &f = flags_out ? *flags_out : local;
+    f.assign(n, 0);
     // the handle's stream does not synchronise with the null stream: wait for
     // the work that wrote the flags before the host copy
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -654,6 +700,7 @@ static void cc_nodes(const Inf_Ff_store &L, int ip, int &i1, int &j1, int &i2, i
 // ---------------------------------------------------------------------------
 struct OpBuild {
     LineOp op;
+    int line = -1; // coarse/fine line whose ring the op writes (spline) or reads (temporal, c_f)
     std::vector<int> src, dst;
     std::vector<uint8_t> mask;
     std::vector<uint64_t> rd;                          // read units
@@ -790,12 +837,14 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     if (L->periodic && nblocks != 1) return fail("periodic mode needs exactly one block");
     // blocks
     L->hb.resize(nblocks);
-    std::vector<int> level_of(nblocks, -1);
+    std::vector<int> &level_of = L->level_of;
+    level_of.assign(nblocks, -1);
     for (int l = 0; l < cfg.LEVEL; l++)
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) level_of[a] = l;
+    std::vector<std::vector<int>> hflags(nblocks); // node flags on the host (scheduling rules below)
     const int rows = std::max(h->tpb / 64, MBL_TILE_ROWS);
     L->rows = rows;
-    std::vector<int> tile_blk;
+    std::vector<int> tile_blk; // per tile: the block's index in the layout's descriptor table
     for (int l = 0; l < cfg.LEVEL; l++) {
         L->tile_off[l] = (int)tile_blk.size();
         int vx = (h->vx_set || L->periodic) ? h->vx : 1;
@@ -818,11 +867,12 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
             if (!L->periodic && seg_waste(ar.nx, vx, 32) < seg_waste(ar.nx, vx, 64)) d.tw = 32;
             d.trows = rows * (64 / d.tw);
             set_tiles_x(d, seg_tiles(ar.nx, vx, d.tw));
-            d.tile_begin = tiles;
+            // tiles are numbered across all levels (a launch may cover several adjacent levels)
+            d.tile_begin = L->tile_off[l] + tiles;
             tiles += d.tiles_x * ((ar.ny + d.trows - 1) / d.trows);
-            tile_blk.resize(L->tile_off[l] + tiles, a - L->ASP[l]);
+            tile_blk.resize(L->tile_off[l] + tiles, a);
             L->hb[a] = d;
-            if (validate_flags(h, ar, A.pps[a].solid, L->periodic, nullptr)) {
+            if (validate_flags(h, ar, A.pps[a].solid, L->periodic, nullptr, &hflags[a])) {
                 L->release();
                 delete L;
                 return MBL_E_INVALID;
@@ -856,6 +906,7 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         const AREA &af = A.areas[lc.areaf_n];
         o.op.type = type;
         o.op.mgroup = mgroup[b];
+        o.line = b;
         o.op.n = A.t3[b].N;
         o.op.dst_block = lc.areaf_n;
         o.op.src_block = lc.areaf_n;
@@ -912,6 +963,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         const FM_spline3 &S3 = A.s3[b];
         int r = af.m / ac.m;
         o.op.type = OP_SPLINE;
+        o.op.mgroup = mgroup[b];
+        o.line = b;
         o.op.n = lc.Nf;
         o.op.nc = lc.Nc;
         o.op.ratio = r;
@@ -994,6 +1047,44 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         return o;
     };
 
+    // What each op touches, for the scheduling of a whole refinement cycle (schedule_cycle):
+    // node units, the ring of its line, and the levels whose collide epoch it belongs to. An
+    // op that reads a block's planes or macros, or writes its F, must stay between the same two
+    // collides of that block's level as in the reference order (the collide rewrites every F of
+    // the block and the macros of its fluid nodes, and reads every F). A spline writes only the
+    // macros of its fine line; those are C_BOUNDARY nodes, which no collide writes, and a
+    // collide reads the macros of a C_BOUNDARY node only next to an inlet/outlet node, so the
+    // spline is tied to the fine level's epoch only where a written node is not C_BOUNDARY or
+    // the fine block holds inlet/outlet nodes.
+    auto op_access = [&](const OpBuild &o) -> OpAccess {
+        OpAccess ac;
+        for (uint64_t u : o.rd) ac.rd.push_back(u);
+        for (auto &w : o.wr) ac.wr.push_back(w.first);
+        std::sort(ac.rd.begin(), ac.rd.end());
+        ac.rd.erase(std::unique(ac.rd.begin(), ac.rd.end()), ac.rd.end());
+        std::sort(ac.wr.begin(), ac.wr.end());
+        ac.wr.erase(std::unique(ac.wr.begin(), ac.wr.end()), ac.wr.end());
+        ac.ring_line = o.line;
+        ac.ring_write = (o.op.type == OP_SPLINE);
+        const int ls = level_of[o.op.src_block], ld = level_of[o.op.dst_block];
+        switch (o.op.type) {
+        case OP_TEMPORAL:
+        case OP_CF: ac.epochs = 1u << ld; break;
+        case OP_CC:
+        case OP_FC: ac.epochs = (1u << ls) | (1u << ld); break;
+        case OP_SPLINE: {
+            ac.epochs = 1u << ls;
+            const std::vector<int> &fl = hflags[o.op.dst_block];
+            bool tie = false;
+            for (int v : fl) tie = tie || v == BOUNDARY_INLET || v == BOUNDARY_OUTLET;
+            for (int d : o.dst) tie = tie || fl[d] != C_BOUNDARY;
+            if (tie) ac.epochs |= 1u << ld;
+            break;
+        }
+        default: ac.epochs = (1u << ls) | (1u << ld);
+        }
+        return ac;
+    };
     // Batches, each scheduled on its own; ops appended to `all` grouped by sub-batch.
     auto add_batch = [&](std::vector<OpBuild> ops, Batch &batch) -> bool {
         for (auto &o : ops)
@@ -1010,6 +1101,7 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
                 sb.max_n = std::max(sb.max_n, ops[j].op.n);
                 sb.max_nc = std::max(sb.max_nc, ops[j].op.nc);
                 sb.spline = (ops[j].op.type == OP_SPLINE);
+                L->acc.push_back(op_access(ops[j]));
                 all.push_back(std::move(ops[j]));
             }
             if (sb.count) batch.push_back(sb);
@@ -1092,10 +1184,20 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
 // ---------------------------------------------------------------------------
 // refinement cycle (evolution.h:2-65)
 // ---------------------------------------------------------------------------
+// cycle() walks the recursion in the reference order, rotates the ring markers as
+// evolution.h:41-46 does and records every operation (CycleItem) with the parities and
+// markers in force. schedule_cycle() turns the recorded order into launches: it derives
+// every read-after-write, write-after-read and write-after-write dependence on the data the
+// operations touch (collide epochs per level, interface node units, ring slots), gives each
+// operation the earliest position its dependences allow, and merges operations of one kind
+// at one position into one launch (stream-collides of adjacent levels, interface ops of any
+// line, splines of any level). Every read still sees the value it sees in the reference
+// order, so results are bit-identical; the coarse step of a three-level layout needs 14
+// launches instead of 22.
 struct CycleState {
     uint32_t pm;
-    bool launch;
     int rc;
+    std::vector<CycleItem> items;
 };
 
 static int markers_of(const Layout *L, const Ff_Store *t3, int b0, int b1, int m[3])
@@ -1132,60 +1234,272 @@ static int marker_set(const Layout *L, const Ff_Store *t3, MarkerSet &ms)
     return 0;
 }
 
-static void run_ops(mbl_context *h, Layout *L, const Batch &B, CycleState &cs, const Ff_Store *t3, double xm)
+static void record_ops(Layout *L, const Batch &B, CycleState &cs, const Ff_Store *t3, int kind, double xm,
+                       bool temporal)
 {
-    if (!cs.launch || cs.rc || B.empty()) return;
+    if (cs.rc || B.empty()) return;
     MarkerSet ms;
     int rc = marker_set(L, t3, ms);
     if (rc) {
         cs.rc = rc;
         return;
     }
-    const int mode = storage_mode(h->cfg);
-    for (const Sub &s : B) {
-        rc = launch_line_ops(h->stream, mode, L->dops + s.off, s.count, s.max_n, L->db, cs.pm, ms, xm);
-        if (rc) cs.rc = rc;
-    }
+    for (const Sub &sb : B)
+        if (sb.count > 0) cs.items.push_back(CycleItem{kind, -1, sb.off, sb.count, xm, temporal, cs.pm, ms});
 }
 
-static void cycle(mbl_context *h, Layout *L, int level, Ff_Store *t3, CycleState &cs)
+static void cycle(Layout *L, int level, Ff_Store *t3, CycleState &cs)
 {
     if (level == L->LEVEL || cs.rc) return; // level checked before mi[level] is read
-    const int mode = storage_mode(h->cfg);
     for (int i = 0; i < L->mi[level]; i++) {
-        int m[3];
         if (!(level == 0 || i == 0) && !L->bT[level].empty())
-            run_ops(h, L, L->bT[level], cs, t3, 1.0 * i / L->mi[level]);
-        if (cs.launch && L->tiles[level] > 0) {
-            int rc = launch_stream_collide(h->stream, mode, L->vx[level], h->tpb, L->periodic, true, true,
-                                           L->db + L->ASP[level], L->dtile + L->tile_off[level],
-                                           L->ASP[level + 1] - L->ASP[level], L->tiles[level], L->rows, cs.pm,
-                                           h->groups);
-            if (rc) cs.rc = rc;
+            record_ops(L, L->bT[level], cs, t3, 1, 1.0 * i / L->mi[level], true);
+        if (L->tiles[level] > 0) {
+            MarkerSet ms;
+            marker_set(L, t3, ms);
+            cs.items.push_back(CycleItem{0, level, 0, 0, 0.0, false, cs.pm, ms});
         }
         cs.pm ^= (1u << level);
-        if (L->number_line2 != 0) run_ops(h, L, L->bCC[level], cs, t3, 0.0);
+        if (L->number_line2 != 0) record_ops(L, L->bCC[level], cs, t3, 1, 0.0, false);
         if (level != L->LEVEL - 1 && L->LS3[level + 1] > L->LS3[level]) {
+            int m[3];
             int rc = markers_of(L, t3, L->LS3[level], L->LS3[level + 1], m);
             if (rc) {
                 cs.rc = rc;
                 return;
             }
-            if (cs.launch && !cs.rc)
-                for (const Sub &s : L->bS[level]) {
-                    rc = launch_spline(h->stream, mode, L->dops + s.off, s.count, s.max_nc, s.max_n, L->db, cs.pm, m[0]);
-                    if (rc) cs.rc = rc;
-                }
+            record_ops(L, L->bS[level], cs, t3, 2, 0.0, false);
             for (int b = L->LS3[level]; b < L->LS3[level + 1]; b++)
                 for (int p = 0; p < 3; p++) {
                     int bb = t3[b].marker[p] - 1;
                     t3[b].marker[p] = (bb == -1) ? 2 : bb;
                 }
         }
-        cycle(h, L, level + 1, t3, cs);
+        cycle(L, level + 1, t3, cs);
         if (level != 0 && i == L->mi[level] - 1 && !L->bCF[level].empty())
-            run_ops(h, L, L->bCF[level], cs, t3, 0.0);
+            record_ops(L, L->bCF[level], cs, t3, 1, 0.0, false);
     }
+}
+
+// Dependence-driven positions of the recorded items, then the merged launches.
+static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, CycleProgram &P)
+{
+    const int n = (int)items.size();
+    struct Res {
+        int writer = -1;
+        std::vector<int> readers;
+    };
+    std::unordered_map<uint64_t, Res> res;
+    // resources: node units as the interface ops name them; a level's collide epoch; a line's ring
+    auto epoch_key = [](int l) { return unit(0xFFFFFE, (size_t)l, 0); };
+    auto ring_key = [](int b) { return unit(0xFFFFFD, (size_t)b, 0); };
+    std::vector<int> pos(n, 0);
+    for (int j = 0; j < n; j++) {
+        const CycleItem &it = items[j];
+        std::vector<uint64_t> rd, wr;
+        if (it.kind == 0) {
+            wr.push_back(epoch_key(it.level));
+        } else {
+            for (int o = it.off; o < it.off + it.count; o++) {
+                const OpAccess &ac = L->acc[o];
+                rd.insert(rd.end(), ac.rd.begin(), ac.rd.end());
+                wr.insert(wr.end(), ac.wr.begin(), ac.wr.end());
+                for (int l = 0; l < L->LEVEL; l++)
+                    if (ac.epochs & (1u << l)) rd.push_back(epoch_key(l));
+                if (ac.ring_line >= 0) (ac.ring_write ? wr : rd).push_back(ring_key(ac.ring_line));
+            }
+        }
+        std::sort(rd.begin(), rd.end());
+        rd.erase(std::unique(rd.begin(), rd.end()), rd.end());
+        std::sort(wr.begin(), wr.end());
+        wr.erase(std::unique(wr.begin(), wr.end()), wr.end());
+        int p = 0;
+        for (uint64_t u : rd) {
+            auto it2 = res.find(u);
+            if (it2 != res.end() && it2->second.writer >= 0) p = std::max(p, pos[it2->second.writer] + 1);
+        }
+        for (uint64_t u : wr) {
+            auto it2 = res.find(u);
+            if (it2 == res.end()) continue;
+            if (it2->second.writer >= 0) p = std::max(p, pos[it2->second.writer] + 1);
+            for (int r : it2->second.readers)
+                if (r != j) p = std::max(p, pos[r] + 1);
+        }
+        pos[j] = p;
+        for (uint64_t u : rd) {
+            Res &r = res[u];
+            if (r.readers.empty() || r.readers.back() != j) r.readers.push_back(j);
+        }
+        for (uint64_t u : wr) {
+            Res &r = res[u];
+            r.writer = j;
+            r.readers.clear();
+        }
+    }
+    int npos = 0;
+    for (int v : pos) npos = std::max(npos, v + 1);
+    if (getenv("MBL_PRINT_SCHEDULE")) { // the recorded order with each item's position and its ops
+        static const char *TN[] = {"T", "CC", "CF", "FC", "S"};
+        for (int j = 0; j < n; j++) {
+            fprintf(stderr, "mbl schedule: item %2d pos %2d ", j, pos[j]);
+            if (items[j].kind == 0) {
+                fprintf(stderr, "collide level %d\n", items[j].level);
+                continue;
+            }
+            for (int o = items[j].off; o < items[j].off + items[j].count; o++)
+                fprintf(stderr, "%s(%d->%d) ", TN[L->hops[o].type], L->hops[o].src_block, L->hops[o].dst_block);
+            fprintf(stderr, "\n");
+        }
+    }
+    P.launches.clear();
+    for (int q = 0; q < npos; q++) {
+        // stream-collides: runs of adjacent levels with the same nodes per thread
+        std::vector<int> col;
+        for (int j = 0; j < n; j++)
+            if (pos[j] == q && items[j].kind == 0) col.push_back(j);
+        std::sort(col.begin(), col.end(), [&](int a, int b) { return items[a].level < items[b].level; });
+        for (size_t c = 0; c < col.size();) {
+            CycleLaunch cl;
+            cl.kind = 0;
+            cl.lo = cl.hi = items[col[c]].level;
+            cl.items.push_back(col[c]);
+            size_t d = c + 1;
+            while (d < col.size() && items[col[d]].level == cl.hi + 1 && L->vx[items[col[d]].level] == L->vx[cl.lo]) {
+                cl.hi++;
+                cl.items.push_back(col[d]);
+                d++;
+            }
+            P.launches.push_back(cl);
+            c = d;
+        }
+        // interface ops: one launch per temporal interpolation point (ops without one join the first)
+        std::vector<CycleLaunch> ifl;
+        for (int j = 0; j < n; j++) {
+            if (pos[j] != q || items[j].kind != 1) continue;
+            CycleLaunch *tgt = nullptr;
+            for (auto &x : ifl) {
+                bool xt = false;
+                double xx = 0.0;
+                for (int k : x.items)
+                    if (items[k].temporal) {
+                        xt = true;
+                        xx = items[k].xm;
+                    }
+                if (!items[j].temporal || !xt || xx == items[j].xm) {
+                    tgt = &x;
+                    break;
+                }
+            }
+            if (!tgt) {
+                ifl.push_back(CycleLaunch());
+                tgt = &ifl.back();
+                tgt->kind = 1;
+            }
+            tgt->items.push_back(j);
+        }
+        for (auto &x : ifl) P.launches.push_back(x);
+        // splines: the one-workgroup path and the two-kernel path separately
+        CycleLaunch sp[2];
+        for (int j = 0; j < n; j++) {
+            if (pos[j] != q || items[j].kind != 2) continue;
+            int mx = 0;
+            for (int o = items[j].off; o < items[j].off + items[j].count; o++) mx = std::max(mx, L->hops[o].nc);
+            CycleLaunch &t = sp[launch_spline_path(mx)];
+            t.kind = 2;
+            t.items.push_back(j);
+        }
+        for (auto &t : sp)
+            if (!t.items.empty()) P.launches.push_back(t);
+    }
+    // the program's op table: each interface/spline launch's ops contiguous
+    std::vector<LineOp> tab;
+    for (auto &cl : P.launches) {
+        if (cl.kind == 0) continue;
+        cl.op_off = (int)tab.size();
+        for (int j : cl.items)
+            for (int o = items[j].off; o < items[j].off + items[j].count; o++) {
+                tab.push_back(L->hops[o]);
+                cl.max_n = std::max(cl.max_n, L->hops[o].n);
+                cl.max_nc = std::max(cl.max_nc, L->hops[o].nc);
+            }
+        cl.op_count = (int)tab.size() - cl.op_off;
+    }
+    if (P.dops) hipFree(P.dops);
+    P.dops = nullptr;
+    if (!tab.empty()) {
+        if (hipMalloc(&P.dops, sizeof(LineOp) * tab.size()) != hipSuccess) return set_err(h, MBL_E_NOMEM);
+        if (hipMemcpy(P.dops, tab.data(), sizeof(LineOp) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return set_err(h, MBL_E_HIP);
+    }
+    P.nitems = n;
+    P.built = true;
+    return 0;
+}
+
+// The level parities and markers of a merged launch: each level's bit and each line group's
+// markers from the item that involves it (items merged into one launch agree on them).
+static int launch_state(const Layout *L, const std::vector<CycleItem> &items, const CycleLaunch &cl, uint32_t &pm,
+                        MarkerSet &ms)
+{
+    uint32_t known = 0;
+    bool msk[MBL_MAX_LEVEL] = {false};
+    pm = items[cl.items[0]].pm;
+    ms = items[cl.items[0]].ms;
+    auto take_bit = [&](int l, uint32_t from) -> int {
+        const uint32_t b = 1u << l;
+        if ((known & b) && ((pm ^ from) & b)) return MBL_E_INVALID;
+        pm = (pm & ~b) | (from & b);
+        known |= b;
+        return 0;
+    };
+    for (int j : cl.items) {
+        const CycleItem &it = items[j];
+        if (it.kind == 0) {
+            if (take_bit(it.level, it.pm)) return MBL_E_INVALID;
+            continue;
+        }
+        for (int o = it.off; o < it.off + it.count; o++) {
+            const LineOp &op = L->hops[o];
+            if (take_bit(L->level_of[op.src_block], it.pm) || take_bit(L->level_of[op.dst_block], it.pm))
+                return MBL_E_INVALID;
+            const int g = op.mgroup;
+            if (msk[g]) {
+                for (int p = 0; p < 3; p++)
+                    if (ms.m[g][p] != it.ms.m[g][p]) return MBL_E_INVALID;
+            } else {
+                for (int p = 0; p < 3; p++) ms.m[g][p] = it.ms.m[g][p];
+                msk[g] = true;
+            }
+        }
+    }
+    return 0;
+}
+
+static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P)
+{
+    if ((int)items.size() != P.nitems) return MBL_E_INVALID;
+    const int mode = storage_mode(h->cfg);
+    for (const CycleLaunch &cl : P.launches) {
+        uint32_t pm;
+        MarkerSet ms;
+        if (launch_state(L, items, cl, pm, ms)) return MBL_E_INVALID;
+        int rc = 0;
+        if (cl.kind == 0) {
+            int tiles = 0;
+            for (int l = cl.lo; l <= cl.hi; l++) tiles += L->tiles[l];
+            rc = launch_stream_collide(h->stream, mode, L->vx[cl.lo], h->tpb, L->periodic, true, true, L->db, L->dtile,
+                                       L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo]);
+        } else if (cl.kind == 1) {
+            double xm = 0.0;
+            for (int j : cl.items)
+                if (items[j].temporal) xm = items[j].xm;
+            rc = launch_line_ops(h->stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_n, L->db, pm, ms, xm);
+        } else {
+            rc = launch_spline(h->stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_nc, cl.max_n, L->db, pm, ms);
+        }
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2);
@@ -1266,39 +1580,38 @@ static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblo
         sh += 6;
     }
     key |= (uint64_t)level << 60;
-    CycleState cs{pm, true, 0};
+    // the cycle in the reference order (markers rotate here), then its scheduled launches
+    std::vector<Ff_Store> saved(line_t3, line_t3 + nline);
+    CycleState cs{pm, 0, {}};
+    cycle(L, level, line_t3, cs);
+    auto fail = [&](int code) {
+        std::copy(saved.begin(), saved.end(), line_t3);
+        set_err(h, code);
+    };
+    if (cs.rc) return fail(cs.rc);
+    CycleProgram &P = L->prog[level];
+    if (!P.built && schedule_cycle(h, L, cs.items, P)) return fail(h->last_error ? h->last_error : MBL_E_HIP);
     if (cfg.use_graph) {
         auto it = L->graphs.find(key);
         if (it == L->graphs.end()) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
-            std::vector<Ff_Store> saved(line_t3, line_t3 + nline);
-            if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-                set_err(h, MBL_E_HIP);
-                return;
-            }
-            cycle(h, L, level, line_t3, cs);
+            if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(MBL_E_HIP);
+            int rc = run_cycle(h, L, cs.items, P);
             hipError_t e = hipStreamEndCapture(h->stream, &g);
-            if (cs.rc || e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (rc || e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
                 if (g) hipGraphDestroy(g);
-                std::copy(saved.begin(), saved.end(), line_t3);
-                set_err(h, cs.rc ? cs.rc : MBL_E_HIP);
-                return;
+                return fail(rc ? rc : MBL_E_HIP);
             }
             hipGraphDestroy(g);
             L->graphs[key] = ge;
             if (hipGraphLaunch(ge, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
         } else {
-            cs.launch = false;
-            cycle(h, L, level, line_t3, cs);
             if (hipGraphLaunch(it->second, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
         }
     } else {
-        cycle(h, L, level, line_t3, cs);
-    }
-    if (cs.rc) {
-        set_err(h, cs.rc);
-        return;
+        int rc = run_cycle(h, L, cs.items, P);
+        if (rc) return fail(rc);
     }
     // hand the new state back: F <-> f for every level that took an odd
     // number of substeps in this call
@@ -1362,7 +1675,7 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     const bool macros = !S.periodic || S.flags;
     for (int t = 0; t < nsteps; t++) {
         int rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, nullptr, 1,
-                                       S.tiles, S.rows, (uint32_t)par, h->groups);
+                                       S.tiles, S.rows, (uint32_t)par, h->groups, 0);
         if (rc) return set_err(h, rc);
         par ^= 1;
         for (int k = 0; k < 9; k++) std::swap(pp->F[k], pp->f[k]);
@@ -1850,6 +2163,19 @@ extern "C" void mbl_diagnostics_clear(mbl_handle_t h)
     h->ser_drag_t.clear();
     h->ser_cd.clear();
     h->ser_cl.clear();
+}
+
+extern "C" int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int *interface_ops, int *splines)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!h->layout || level < 0 || level >= MBL_MAX_LEVEL || !h->layout->prog[level].built) return MBL_E_INVALID;
+    int c[3] = {0, 0, 0};
+    for (const CycleLaunch &cl : h->layout->prog[level].launches) c[cl.kind]++;
+    if (collides) *collides = c[0];
+    if (interface_ops) *interface_ops = c[1];
+    if (splines) *splines = c[2];
+    return c[0] + c[1] + c[2];
 }
 
 extern "C" int mbl_divergence_step(mbl_handle_t h)

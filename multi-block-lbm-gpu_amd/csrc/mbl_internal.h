@@ -122,16 +122,21 @@ struct RasterCase {
 // ---- launchers (mbl_kernels.hip), all asynchronous on `stream` ----
 // tile_rows rows per tile (at least tpb / 64; the waves take them in turn);
 // d_tile_blk[t] = descriptor of tile t, or null to search tile_begin.
+// tile_base: the launch covers tiles [tile_base, tile_base + total_tiles) of the numbering of
+// the descriptors' tile_begin (and of d_tile_blk).
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
-                          uint32_t parity_mask, int groups);
+                          uint32_t parity_mask, int groups, int tile_base);
 int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &area, Lbd *x, Lbd *y,
                         Lbd *forcex, Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed,
                         double sigma, double LX, double LY, double R, double CX, double CY, const Consts &c);
 int launch_line_ops(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
                     const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms, double xm);
+// 0: lines of up to max_nc coarse points take the one-workgroup spline kernel; 1: the two-kernel path
+int launch_spline_path(int max_nc);
+// each op writes ring slot ms.m[op.mgroup][0]
 int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
-                  const BlockDesc *d_blocks, uint32_t parity_mask, int slot);
+                  const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms);
 int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
                      const BlockDesc *d_blocks, uint32_t parity_mask);
 int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N);

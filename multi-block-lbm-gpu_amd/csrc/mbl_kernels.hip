@@ -742,7 +742,7 @@ __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, con
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
 __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs,
                                                         const int *__restrict__ tile_blk, int ndesc, int tile_rows,
-                                                        uint32_t parity_mask, int total_tiles)
+                                                        uint32_t parity_mask, int total_tiles, int tile_base)
 {
     const int G = (int)gridDim.x;
     const int b = xcd_tile((int)blockIdx.x, G);
@@ -752,6 +752,9 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
         t0 = b * per;
         t1 = min(total_tiles, t0 + per);
     }
+    // the launch covers tiles [tile_base, tile_base + total_tiles) of the descriptors' numbering
+    t0 += tile_base;
+    t1 += tile_base;
     // one call site, inlined: the descriptor and plane bases stay in scalar registers
     for (int t = t0; t < t1; t++)
         sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, t);
@@ -1076,10 +1079,11 @@ __device__ inline double spline3_s(double xx, double x_1, double h, double fi_1,
 // write the fine edge's rho, u and the rescaled F into ring slot `slot`.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
-                                                     uint32_t parity_mask, int slot)
+                                                     uint32_t parity_mask, MarkerSet ms)
 {
     typedef typename Codec<MODE>::T T;
     const LineOp &op = ops[blockIdx.y];
+    const int slot = ms.m[op.mgroup][0]; // the oldest slot of the line's ring receives t_{n+1}
     const int ip = blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= op.n) return;
     const int N = op.nc;
@@ -1123,10 +1127,11 @@ constexpr int SPL_FUSED_MAX_NC = 303; // 27 * 8 * nc bytes of LDS within 64 KiB
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict__ ops,
                                                        const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
-                                                       int slot)
+                                                       MarkerSet ms)
 {
     typedef typename Codec<MODE>::T T;
     const LineOp &op = ops[blockIdx.y];
+    const int slot = ms.m[op.mgroup][0]; // the oldest slot of the line's ring receives t_{n+1}
     const int N = op.nc;
     const BlockDesc &sb = blocks[op.src_block];
     const int ps = (parity_mask >> sb.level) & 1;
@@ -1585,26 +1590,26 @@ __global__ __launch_bounds__(256) void k_plane_copy(const uint4 *__restrict__ sr
 
 template <int MODE, int VX>
 static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d,
-                       const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups)
+                       const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups, int tbase)
 {
     dim3 g(groups > 0 && groups < tiles ? groups : tiles), b(tpb);
     if (periodic) {
         if (flags) {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
         } else {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
         }
     } else {
         if (macros)
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
         else
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles);
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
     }
     MBL_LAUNCH_CHECK();
     return 0;
@@ -1612,13 +1617,13 @@ static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool m
 
 template <int MODE>
 static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d,
-                   const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups)
+                   const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups, int tbase)
 {
     switch (vx) {
-    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
-    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
-    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
-    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups);
+    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
+    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
+    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
+    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
     default: return MBL_E_INVALID;
     }
 }
@@ -1666,13 +1671,13 @@ static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bo
 
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
-                          uint32_t parity_mask, int groups)
+                          uint32_t parity_mask, int groups, int tile_base)
 {
     if (total_tiles <= 0) return 0;
-    if (tpb % 64 || tpb < 64 || tpb > 256 || tile_rows < tpb / 64) return MBL_E_INVALID;
+    if (tpb % 64 || tpb < 64 || tpb > 256 || tile_rows < tpb / 64 || tile_base < 0) return MBL_E_INVALID;
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, return sc_mode<M_>(s, vx, tpb, periodic, flags, macros, d_descs, d_tile_blk, ndesc,
-                                             total_tiles, tile_rows, parity_mask, groups));
+                                             total_tiles, tile_rows, parity_mask, groups, tile_base));
     return 0;
 }
 
@@ -1742,8 +1747,10 @@ static int spline_solve_r(hipStream_t s, int R, const LineOp *ops, int nops, siz
     return 0;
 }
 
+int launch_spline_path(int max_nc) { return max_nc <= SPL_FUSED_MAX_NC ? 0 : 1; }
+
 int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
-                  const BlockDesc *d_blocks, uint32_t parity_mask, int slot)
+                  const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms)
 {
     if (nops <= 0) return 0;
     if (max_nc > SPL_MAX_NC) return MBL_E_UNSUPPORTED;
@@ -1753,7 +1760,7 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
         const size_t lds = 27 * sizeof(double) * (size_t)max_nc;
         hipStream_t s = (hipStream_t)stream;
         MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_>), dim3(1, nops), dim3(threads), lds, s, d_ops,
-                                                 d_blocks, parity_mask, slot));
+                                                 d_blocks, parity_mask, ms));
         MBL_LAUNCH_CHECK();
         return 0;
     }
@@ -1766,7 +1773,7 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
     if (rc) return rc;
     MBL_LAUNCH_CHECK();
     dim3 g((max_nf + 255) / 256, nops), b(256);
-    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_eval<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, slot));
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_eval<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, ms));
     MBL_LAUNCH_CHECK();
     return 0;
 }
