@@ -447,6 +447,12 @@ double mbl_batch_node_updates(mbl_batch_t b);
  * recorded steps, case-major: cd[c*nt + s]; returns the number of steps
  * available (at most the history capacity, 4096). */
 int mbl_batch_forces(mbl_batch_t b, int nt, double *cd, double *cl);
+/* error_L2 (lbm/error_L2.h:57-91) of every case on its own: out[c] = the
+ * relative change sqrt(sum|u - u_prev|^2/m) / sqrt(sum|u|^2/m) over the case's
+ * blocks since the previous call, or since creation / the last
+ * mbl_batch_init_from_raster, whose state is the first u_prev; u_prev <- u.
+ * Synchronous. No reference counterpart (the reference has one case). */
+int mbl_batch_residual(mbl_batch_t b, double *out);
 /* Times `trial_steps` steps for every (nodes per thread, threads per group,
  * persistent groups) candidate and keeps the fastest; the trial steps advance
  * the cases. Returns 0 and the choice and its time per step. */

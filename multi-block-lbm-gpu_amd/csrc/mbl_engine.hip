@@ -2426,7 +2426,27 @@ struct mbl_batch {
     size_t max_nodes = 0;
     float *d_raster = nullptr;
     size_t raster_bytes = 0;
+    // per-case residual (error_L2 of each case): previous velocities of every block, and the
+    // residual tables in case-major block order (built on first use)
+    std::vector<Lbd *> upx, upy;
+    BlockDesc *d_rdesc = nullptr;
+    int *d_rtblk = nullptr, *d_case_tiles = nullptr;
+    double *d_rpart = nullptr, *d_rout = nullptr;
+    int rtotal = 0;
 };
+
+// u_prev <- u for every block of a batch (after creation and after every restart), so the
+// first per-case residual measures the change since that state
+static int batch_seed_uprev(mbl_batch *B)
+{
+    mbl_context *h = B->h;
+    for (size_t a = 0; a < B->areas.size(); a++) {
+        const size_t bytes = sizeof(Lbd) * (size_t)B->areas[a].nx * B->areas[a].ny;
+        HIPCHK(h, hipMemcpyAsync(B->upx[a], B->pps[a].ux, bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(B->upy[a], B->pps[a].uy, bytes, hipMemcpyDeviceToDevice, h->stream));
+    }
+    return 0;
+}
 
 // The three ring slots of every coarse/fine line <- the fine edge's current
 // F (init_Fstore, multi/init_multi.h:3-28, for all lines of all cases at once).
@@ -2556,7 +2576,7 @@ static int batch_build(mbl_batch *B)
     size_t total = 0;
     for (const AREA &a : B->areas) {
         size_t nn = (size_t)a.nx * a.ny;
-        total += al(nn * sizeof(int)) + 7 * al(nn * sizeof(Lbd)) + 18 * al(nn * eb);
+        total += al(nn * sizeof(int)) + 9 * al(nn * sizeof(Lbd)) + 18 * al(nn * eb);
     }
     for (const Inf_Ff_store &L : B->lc) total += 27 * al(L.Nf * sizeof(Lbd)) + 28 * al(L.Nc * sizeof(Lbd));
     HIPCHK(h, hipMalloc(&B->pool, total));
@@ -2577,6 +2597,13 @@ static int batch_build(mbl_batch *B)
             pp.f[k] = (Lbd *)take(nn * eb);
             pp.F[k] = (Lbd *)take(nn * eb);
         }
+    }
+    B->upx.resize(5 * n);
+    B->upy.resize(5 * n);
+    for (size_t a = 0; a < B->areas.size(); a++) {
+        size_t nn = (size_t)B->areas[a].nx * B->areas[a].ny;
+        B->upx[a] = (Lbd *)take(nn * sizeof(Lbd));
+        B->upy[a] = (Lbd *)take(nn * sizeof(Lbd));
     }
     B->t3.resize(4 * n);
     B->s3.resize(4 * n);
@@ -2616,6 +2643,7 @@ static int batch_build(mbl_batch *B)
         if (rc) return rc;
     }
     if (int rc = batch_ring_fill(B)) return rc;
+    if (int rc = batch_seed_uprev(B)) return rc;
     // drag links of every refined block
     std::vector<Int3> links;
     std::vector<int> off(n + 1, 0);
@@ -2689,7 +2717,58 @@ extern "C" void mbl_batch_destroy(mbl_batch_t B)
     if (B->d_rblk) hipFree(B->d_rblk);
     if (B->d_rcase) hipFree(B->d_rcase);
     if (B->d_raster) hipFree(B->d_raster);
+    for (void *q : {(void *)B->d_rdesc, (void *)B->d_rtblk, (void *)B->d_case_tiles, (void *)B->d_rpart,
+                    (void *)B->d_rout})
+        if (q) hipFree(q);
     delete B;
+}
+
+// error_L2 (lbm/error_L2.h:57-91) of every case of a batch on its own: case c's
+// sqrt(sum|u - u_prev|^2/m) / sqrt(sum|u|^2/m) over its five blocks, u_prev <- u. Tables in
+// case-major block order are built on the first call.
+extern "C" int mbl_batch_residual(mbl_batch_t B, double *out)
+{
+    if (!B || !out) return MBL_E_INVALID;
+    mbl_context *h = B->h;
+    const int n = B->n;
+    if (!B->d_rdesc) {
+        std::vector<BlockDesc> hd;
+        std::vector<int> tblk, case_tiles(n + 1, 0);
+        int total = 0;
+        for (int c = 0; c < n; c++) {
+            for (int q = 0; q < 5; q++) {
+                const int a = q < 4 ? 4 * c + q : 4 * n + c;
+                BlockDesc d = make_desc(h->cfg, B->areas[a], B->pps[a], 0, B->upx[a], B->upy[a]);
+                d.rtiles_x = (B->areas[a].nx + 63) / 64;
+                d.rtile_begin = total;
+                total += d.rtiles_x * ((B->areas[a].ny + 15) / 16);
+                tblk.resize(total, (int)hd.size());
+                hd.push_back(d);
+            }
+            case_tiles[c + 1] = total;
+        }
+        B->rtotal = total;
+        if (hipMalloc(&B->d_rdesc, sizeof(BlockDesc) * hd.size()) != hipSuccess ||
+            hipMalloc(&B->d_rtblk, sizeof(int) * tblk.size()) != hipSuccess ||
+            hipMalloc(&B->d_case_tiles, sizeof(int) * case_tiles.size()) != hipSuccess ||
+            hipMalloc(&B->d_rpart, sizeof(double) * 3 * (size_t)total) != hipSuccess ||
+            hipMalloc(&B->d_rout, sizeof(double) * 4 * (size_t)n) != hipSuccess)
+            return set_err(h, MBL_E_NOMEM);
+        HIPCHK(h, hipMemcpy(B->d_rdesc, hd.data(), sizeof(BlockDesc) * hd.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(B->d_rtblk, tblk.data(), sizeof(int) * tblk.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(B->d_case_tiles, case_tiles.data(), sizeof(int) * case_tiles.size(),
+                            hipMemcpyHostToDevice));
+    }
+    // the macros the descriptors point at are the caller-visible planes: the state of the last
+    // step (the F/f exchange of a batch does not move them)
+    int rc = launch_residual_cases(h->stream, B->d_rdesc, B->d_rtblk, 5 * n, B->rtotal, B->d_rpart, n,
+                                   B->d_case_tiles, B->d_rout);
+    if (rc) return set_err(h, rc);
+    std::vector<double> rec(4 * (size_t)n);
+    HIPCHK(h, hipMemcpyAsync(rec.data(), B->d_rout, sizeof(double) * rec.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int c = 0; c < n; c++) out[c] = rec[4 * (size_t)c];
+    return 0;
 }
 
 extern "C" double mbl_batch_node_updates(mbl_batch_t B)
@@ -2822,6 +2901,7 @@ extern "C" int mbl_batch_init_from_raster(mbl_batch_t B, int H, int W, const flo
                                 B->d_raster);
     if (rc) return set_err(h, rc);
     if ((rc = batch_ring_fill(B))) return set_err(h, rc);
+    if ((rc = batch_seed_uprev(B))) return set_err(h, rc);
     // a restart: the force history starts again
     B->nhist = 0;
     B->head = 0;

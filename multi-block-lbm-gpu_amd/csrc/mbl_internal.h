@@ -143,6 +143,10 @@ int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N);
 // d_tblk[t]: block of residual tile t (null: found from the descriptors' rtile_begin).
 int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, int ndesc, int total_rtiles,
                     double *partials, int mode, double umax, double LY, double *d_out);
+// error_L2 per case of a batch: the residual tiles (u_prev <- u), then case c's tiles
+// [d_case_tiles[c], d_case_tiles[c+1]) summed into d_out[4c..4c+3].
+int launch_residual_cases(void *stream, const BlockDesc *d_descs, const int *d_tblk, int ndesc, int total_rtiles,
+                          double *partials, int ncases, const int *d_case_tiles, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
 int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes);

@@ -1421,6 +1421,30 @@ __global__ __launch_bounds__(1024) void k_residual_final(int total, const double
     }
 }
 
+// Per-case residual of a batch: case c's residual tiles are [case_tiles[c], case_tiles[c+1]),
+// summed in order; out[4c..4c+3] = residual, s1, s2, smallest density (k_residual_final's record).
+__global__ __launch_bounds__(256) void k_residual_cases(const int *__restrict__ case_tiles,
+                                                        const double *__restrict__ partials, double *out)
+{
+    __shared__ double sh[16];
+    const int c = blockIdx.x;
+    double g1 = 0.0, g2 = 0.0, mn = INFINITY;
+    for (int q = case_tiles[c] + threadIdx.x; q < case_tiles[c + 1]; q += blockDim.x) {
+        g1 += partials[3 * (size_t)q];
+        g2 += partials[3 * (size_t)q + 1];
+        mn = fmin(mn, partials[3 * (size_t)q + 2]);
+    }
+    const double all1 = group_sum(g1, sh);
+    const double all2 = group_sum(g2, sh);
+    const double allmn = group_min(mn, sh);
+    if (threadIdx.x == 0) {
+        out[4 * (size_t)c] = sqrt(all1) / sqrt(all2);
+        out[4 * (size_t)c + 1] = all1;
+        out[4 * (size_t)c + 2] = all2;
+        out[4 * (size_t)c + 3] = allmn;
+    }
+}
+
 template <int MODE> __global__ void k_decode(const void *plane, int k, size_t n, double *out)
 {
     typedef typename Codec<MODE>::T T;
@@ -1795,6 +1819,19 @@ int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, i
                        umax, LY);
     MBL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, total_rtiles, (const double *)partials, d_out);
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_residual_cases(void *stream, const BlockDesc *d_descs, const int *d_tblk, int ndesc, int total_rtiles,
+                          double *partials, int ncases, const int *d_case_tiles, double *d_out)
+{
+    if (total_rtiles <= 0 || ncases <= 0) return MBL_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_residual_tiles, dim3(total_rtiles), dim3(256), 0, s, d_descs, d_tblk, ndesc, partials, 0, 0.0,
+                       0.0);
+    MBL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_residual_cases, dim3(ncases), dim3(256), 0, s, d_case_tiles, (const double *)partials, d_out);
     MBL_LAUNCH_CHECK();
     return 0;
 }

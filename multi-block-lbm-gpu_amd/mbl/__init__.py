@@ -97,6 +97,7 @@ def load_library(fp64=False):
         "mbl_batch_step": (ip, [vp, ip]),
         "mbl_batch_node_updates": (C.c_double, [vp]),
         "mbl_batch_forces": (ip, [vp, ip, dp, dp]),
+        "mbl_batch_residual": (ip, [vp, dp]),
         "mbl_batch_autotune": (ip, [vp, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), dp]),
         "mbl_batch_block": (ip, [vp, ip, ip, C.POINTER(AREA), C.POINTER(D2Q9)]),
         "mbl_batch_raster": (ip, [vp, ip, ip, fp]),
@@ -533,6 +534,13 @@ class Batch:
 
     def sync(self):
         check(self.lib, self.lib.mbl_synchronize(self.h), "synchronize")
+
+    def residual(self):
+        """error_L2 of every case since the previous call (or since creation / the last restart)."""
+        out = np.zeros(self.n)
+        check(self.lib, self.lib.mbl_batch_residual(self.b, out.ctypes.data_as(C.POINTER(C.c_double))),
+              "batch residual")
+        return out
 
     def node_updates(self):
         return self.lib.mbl_batch_node_updates(self.b)

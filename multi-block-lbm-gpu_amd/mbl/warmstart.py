@@ -45,12 +45,12 @@ from .net import Net
 IN_CH, OUT_CH = 7, 3
 
 
-def c5_specs(n, seed, offset=0.25):
+def c5_specs(n, seed, offset=0.25, re_min=80.0, re_max=200.0):
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         Dc = int(rng.choice([8, 12]))
-        sp = case_spec(Dc, float(rng.uniform(0.1, 0.3)), float(rng.uniform(80.0, 200.0)))
+        sp = case_spec(Dc, float(rng.uniform(0.1, 0.3)), float(rng.uniform(re_min, re_max)))
         sp.CY += offset
         out.append(sp)
     return out
@@ -237,9 +237,12 @@ def time_to_amplitude(cl, T, a_ref, tol=0.01):
     return int(bad[-1] + 1 + T)
 
 
-def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None):
+def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None, probe=0, res_out=None):
     """Runs a batch for `steps` steps; returns (cd, cl, wall seconds, snapshots).
-    snap = (start, every, count, H, W) records rasters at those steps."""
+    snap = (start, every, count, H, W) records rasters at those steps. With probe > 0 the
+    per-case residual (error_L2 of each case, mbl_batch_residual) after the first `probe`
+    steps is appended to the list res_out: the relative change of the velocity field since the
+    initial state, which decides the warm start's fallback."""
     b = Batch(lib, specs, storage=storage)
     if init is not None:
         b.init_from_raster(init)
@@ -248,10 +251,11 @@ def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None):
     cd = np.zeros((n, steps))
     cl = np.zeros((n, steps))
     shots, shot_t = [], []
-    marks = set()
+    snap_marks = set()
     if snap:
         s0, every, count, H, W = snap
-        marks = {s0 + k * every for k in range(count)}
+        snap_marks = {s0 + k * every for k in range(count)}
+    marks = snap_marks | ({probe} if probe > 0 else set())
     t0 = time.perf_counter()
     done = 0
     while done < steps:
@@ -264,7 +268,9 @@ def run(lib, specs, steps, storage, init=None, snap=None, chunk=4096, log=None):
         cd[:, done:done + m] = d
         cl[:, done:done + m] = l
         done += m
-        if done in marks:
+        if probe > 0 and done == probe and res_out is not None:
+            res_out.append(b.residual())
+        if done in snap_marks:
             shots.append(b.raster(snap[3], snap[4]))
             shot_t.append(done)
         if log and done % (4 * chunk) < m:
@@ -285,26 +291,62 @@ def predict(net, model, specs, H, W, phase):
     return np.stack([harmonic_eval(coef[i], ph[i]) for i in range(len(specs))])
 
 
-def convergence(cl_c, cl_w, cd_w):
-    """Per case: steps to a converged Cl amplitude, cold and warm, and the fallback count."""
+def convergence(cl_c, cl_w, cd_w, cd_c=None, res0=None, threshold=None, probe=0):
+    """Per case: steps to convergence, cold and warm, and the number of fallbacks.
+
+    A case that sheds in its cold run (a period of 8 steps or more and a Cl amplitude above
+    1e-3) has converged when its Cl amplitude stays within 1 % of the cold run's converged
+    amplitude; a steady case when its Cd stays within 0.5 % of the cold run's final Cd. A warm
+    case falls back to a cold start, and is charged the steps it used plus the cold steps, when
+    its residual after the first `probe` steps exceeds `threshold` (res0: per-case error_L2 of
+    the warm run), or when its forces turn non-finite or exceed |Cd| > 50."""
+    from .postproc import time_to_converged
     tc, tw, fallback = [], [], 0
     for c in range(cl_c.shape[0]):
         T_c = shedding_period(cl_c[c])
-        if T_c < 8:
+        sheds = T_c >= 8 and float(np.mean(amplitude(cl_c[c], T_c)[-2 * T_c:])) > 1e-3
+        if sheds:
+            a_ref = float(np.mean(amplitude(cl_c[c], T_c)[-2 * T_c:]))
+            t_cold = time_to_amplitude(cl_c[c], T_c, a_ref)
+        elif cd_c is not None:
+            t_cold = time_to_converged(cd_c[c], rel=0.005)
+        else:
             tc.append(-1)
             tw.append(-1)
             continue
-        a_ref = float(np.mean(amplitude(cl_c[c], T_c)[-2 * T_c:]))
-        t_cold = time_to_amplitude(cl_c[c], T_c, a_ref)
         bad = ~np.isfinite(cd_w[c]) | (np.abs(cd_w[c]) > 50)
-        if bad.any():
+        resid_bad = res0 is not None and threshold is not None and not (res0[c] <= threshold)
+        if resid_bad:
+            fallback += 1
+            t_warm = probe + t_cold if t_cold >= 0 else -1
+        elif bad.any():
             fallback += 1
             t_warm = int(np.argmax(bad)) + 1 + t_cold if t_cold >= 0 else -1
-        else:
+        elif sheds:
             t_warm = time_to_amplitude(cl_w[c], T_c, a_ref)
+        else:
+            # the warm run converges to the same steady state; judged against the cold run's
+            t_warm = time_to_converged_to(cd_w[c], float(cd_c[c][-1]), 0.005)
         tc.append(t_cold)
         tw.append(t_warm)
     return np.array(tc), np.array(tw), fallback
+
+
+def time_to_converged_to(x, target, rel):
+    """First step after which x stays within rel of target; -1 if never."""
+    x = np.asarray(x, np.float64)
+    bad = np.nonzero(np.abs(x - target) > rel * abs(target))[0]
+    if bad.size == 0:
+        return 0
+    if bad[-1] == x.size - 1:
+        return -1
+    return int(bad[-1] + 1)
+
+
+def fallback_cases(res0, threshold):
+    """Cases whose initial residual exceeds the threshold (or is not finite): restarted cold."""
+    r = np.asarray(res0, np.float64)
+    return ~(r <= threshold)
 
 
 def early_overshoot(cl_c, cl_w, periods=(1, 2)):
@@ -419,6 +461,19 @@ def main(argv=None):
                          "the mean being the network's average over 8 phases")
     ap.add_argument("--rho", default="net", choices=["net", "poisson"],
                     help="warm-start density: the network's or the pressure Poisson solution of its velocity")
+    ap.add_argument("--train-re-min", type=float, default=40.0, help="training cases: Re ~ U[this, 200]")
+    ap.add_argument("--test-re-min", type=float, default=40.0, help="held-out cases: Re ~ U[this, 200]")
+    ap.add_argument("--probe", type=int, default=20,
+                    help="coarse steps after a restart at which the per-case residual decides the fallback")
+    ap.add_argument("--fallback-residual", type=float, default=0.15,
+                    help="a warm case whose residual after --probe steps exceeds this restarts cold (measured "
+                         "after 20 steps: warm starts from the network 0.04-0.10, cold starts about 0.22, a "
+                         "corrupted field 0.52)")
+    ap.add_argument("--ensemble", type=int, default=1,
+                    help="train this many networks (seeds net-seed, net-seed+1000, ...) and average their fields")
+    ap.add_argument("--load-params", default=None,
+                    help="comma-separated .npz files of trained networks (their 'params'): skip snapshots and "
+                         "training, evaluate the (averaged) networks on the held-out cases")
     args = ap.parse_args(argv)
     lib = load_library(False)
     storage = abi.STORE_NATIVE if args.storage == "f32" else abi.STORE_F16S
@@ -431,8 +486,10 @@ def main(argv=None):
     if args.self_init:
         return self_init(lib, args, storage, out)
 
+    if args.load_params:
+        return evaluate_saved(lib, args, storage, out)
     # 1. snapshots of the training cases
-    tr = c5_specs(args.train, seed=101)
+    tr = c5_specs(args.train, seed=101, re_min=args.train_re_min)
     # the run goes on `tail` steps past the last snapshot so that a later zero crossing brackets it
     steps_tr = args.spin + args.nsnap * args.every + args.tail
     _, cl_tr, wall_tr, (shots, shot_t), nu_tr = run(lib, tr, steps_tr, snap_storage,
@@ -446,8 +503,9 @@ def main(argv=None):
         for c in range(len(tr)):
             x = cl_tr[c, args.spin:]
             T = shedding_period(cl_tr[c])
-            if T < 8:
-                keep[c] = False
+            if T < 8 or float(np.mean(amplitude(x, T)[T - 1:])) <= 1e-3:
+                # steady: kept when its Cl no longer moves over the snapshot window
+                keep[c] = float(np.max(x) - np.min(x)) <= 1e-3
                 continue
             a = amplitude(x, T)[T - 1:]
             keep[c] = np.max(np.abs(a - a.mean())) <= args.settled * a.mean()
@@ -460,6 +518,10 @@ def main(argv=None):
     for c, sp in enumerate(tr):
         x = cl_tr[c, args.spin:]
         ph = np.array([crossing_phase(x, t - 1 - args.spin) for t in shot_t])
+        if not np.isfinite(ph).any() and float(np.max(x) - np.min(x)) <= 1e-3:
+            # a steady case: the field does not depend on the phase, so its snapshots are spread
+            # over all phases (the network learns that independence)
+            ph = np.linspace(0.0, 2 * np.pi, len(shot_t), endpoint=False)
         tg = np.stack([to_target(shots[k][c:c + 1], [sp])[0] for k in range(len(shot_t))])
         ok = np.isfinite(ph) & np.isfinite(tg).all(axis=(1, 2, 3))
         case_t.append(tg[ok])
@@ -481,26 +543,34 @@ def main(argv=None):
     Xv, Tv = samples(range(len(tr) - nval, len(tr)))
     out["samples"] = int(Xt.shape[0] + Xv.shape[0])
 
-    # 2. training; the learning rate steps down (lr, lr/3, lr/10 over 50%, 30%, 20% of the epochs)
+    # 2. training; the learning rate steps down (lr, lr/3, lr/10 over 50%, 30%, 20% of the epochs);
+    # with --ensemble K, K networks from different seeds, whose predicted fields are averaged
     batch = min(args.batch, Xt.shape[0])
-    net = Net(lib, H, W, in_ch, out_ch, base=args.base, max_batch=max(batch, args.batch), seed=args.net_seed)
     t0 = time.perf_counter()
-    hist = []
-    e1 = max(1, int(0.5 * args.epochs))
-    e2 = max(1, int(0.3 * args.epochs))
-    for ep, lr in ((e1, args.lr), (e2, args.lr / 3), (max(1, args.epochs - e1 - e2), args.lr / 10)):
-        done_ep = 0
-        while done_ep < ep:  # calls of a few epochs: Adam state carries over, progress is printed
-            k = min(ep - done_ep, 5 if args.model == "phase" else 25)
-            hist.extend(net.train(Xt, Tt, batch=batch, epochs=k, lr=lr, seed=99 + len(hist) + args.net_seed - 2024))
-            done_ep += k
-            print(f"# epoch {len(hist)} loss {hist[-1]:.4e} {time.perf_counter() - t0:.1f} s", file=sys.stderr,
-                  flush=True)
-    hist = np.array(hist)
+    nets, hists = [], []
+    for e in range(args.ensemble):
+        seed = args.net_seed + 1000 * e
+        net = Net(lib, H, W, in_ch, out_ch, base=args.base, max_batch=max(batch, args.batch), seed=seed)
+        hist = []
+        e1 = max(1, int(0.5 * args.epochs))
+        e2 = max(1, int(0.3 * args.epochs))
+        for ep, lr in ((e1, args.lr), (e2, args.lr / 3), (max(1, args.epochs - e1 - e2), args.lr / 10)):
+            done_ep = 0
+            while done_ep < ep:  # calls of a few epochs: Adam state carries over, progress is printed
+                k = min(ep - done_ep, 5 if args.model == "phase" else 25)
+                hist.extend(net.train(Xt, Tt, batch=batch, epochs=k, lr=lr, seed=99 + len(hist) + seed - 2024))
+                done_ep += k
+                print(f"# net {e} epoch {len(hist)} loss {hist[-1]:.4e} {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
+        nets.append(net)
+        hists.append(np.array(hist))
+    net = Ensemble(nets)
+    hist = hists[0]
     out["train_s"] = round(time.perf_counter() - t0, 2)
     out["epochs"] = int(hist.size)
     out["net_seed"] = args.net_seed
-    out["epoch_loss_first_last"] = [float(hist[0]), float(hist[-1])]
+    out["ensemble"] = args.ensemble
+    out["epoch_loss_first_last"] = [[float(x[0]), float(x[-1])] for x in hists]
     # validation on the held-out cases' snapshots, in the target units of the phase model
     vcases = range(len(tr) - nval, len(tr))
     Tsnap = np.concatenate([case_t[c] for c in vcases])
@@ -520,22 +590,7 @@ def main(argv=None):
           file=sys.stderr, flush=True)
 
     # 3. held-out cases, cold and warm
-    te = c5_specs(args.test, seed=202)
-    cd_c, cl_c, wall_c, (fin, _), nu = run(lib, te, args.steps, storage, log="cold",
-                                           snap=(args.steps, 1, 1, H, W) if args.diagnose else None)
-    t0 = time.perf_counter()
-    pred = predict(net, args.model, te, H, W, 0.0)
-    init = from_target(pred, te).astype(np.float32)
-    if args.rho == "poisson":
-        init = poisson_density(init, te)
-    out["rho"] = args.rho
-    t_inf = time.perf_counter() - t0
-    cd_w, cl_w, wall_w, _, _ = run(lib, te, args.steps, storage, init=init, log="warm")
-    ms_step = 0.5 * (wall_c + wall_w) / args.steps * 1e3
-    tc, tw, fallback = convergence(cl_c, cl_w, cd_w)
-    out["steps_run"] = args.steps
-    out["mlups"] = round(nu * args.steps / wall_c / 1e6, 1)
-    summary(out, tc, tw, fallback, ms_step, t_inf)
+    te, cd_c, cl_c, cd_w, cl_w, tc, tw, fallback, pred, fin = held_out(lib, args, storage, net, out)
     out.update(early_overshoot(cl_c, cl_w))
     if args.fluct_scale:
         phs = np.linspace(0.0, 2 * np.pi, 8, endpoint=False)
@@ -605,7 +660,79 @@ def main(argv=None):
         out["diagnose"] = diag
     if args.save:
         np.savez(args.save, tc=tc, tw=tw, cl_cold=cl_c[:, ::10].astype(np.float32),
-                 cl_warm=cl_w[:, ::10].astype(np.float32), loss=hist, params=net.get_params())
+                 cl_warm=cl_w[:, ::10].astype(np.float32), loss=hist,
+                 **{("params" if i == 0 else f"params{i}"): n_.get_params() for i, n_ in enumerate(net.nets)})
+    net.close()
+    print(json.dumps(out), flush=True)
+    return out
+
+
+class Ensemble:
+    """Networks whose predictions are averaged (one network: that network)."""
+
+    def __init__(self, nets):
+        self.nets = nets
+
+    def forward(self, x):
+        return np.mean([n.forward(x) for n in self.nets], axis=0)
+
+    def close(self):
+        for n in self.nets:
+            n.close()
+
+
+def held_out(lib, args, storage, net, out):
+    """The held-out cases from step 0, cold and warm; per-case residual after --probe steps of
+    both (the warm one decides the fallback); convergence and the summary into `out`."""
+    H, W = args.H, args.W
+    te = c5_specs(args.test, seed=202, re_min=args.test_re_min)
+    res_c, res_w = [], []
+    cd_c, cl_c, wall_c, (fin, _), nu = run(lib, te, args.steps, storage, log="cold",
+                                           snap=(args.steps, 1, 1, H, W) if args.diagnose else None,
+                                           probe=args.probe, res_out=res_c)
+    t0 = time.perf_counter()
+    pred = predict(net, args.model, te, H, W, 0.0)
+    init = from_target(pred, te).astype(np.float32)
+    if args.rho == "poisson":
+        init = poisson_density(init, te)
+    out["rho"] = args.rho
+    t_inf = time.perf_counter() - t0
+    cd_w, cl_w, wall_w, _, _ = run(lib, te, args.steps, storage, init=init, log="warm", probe=args.probe,
+                                   res_out=res_w)
+    ms_step = 0.5 * (wall_c + wall_w) / args.steps * 1e3
+    tc, tw, fallback = convergence(cl_c, cl_w, cd_w, cd_c=cd_c, res0=res_w[0], threshold=args.fallback_residual,
+                                   probe=args.probe)
+    out["steps_run"] = args.steps
+    out["mlups"] = round(nu * args.steps / wall_c / 1e6, 1)
+    out["test_re_min"] = args.test_re_min
+    out["steady_cases"] = int(sum(1 for c in range(len(te)) if shedding_period(cl_c[c]) < 8 or
+                                  float(np.mean(amplitude(cl_c[c], max(shedding_period(cl_c[c]), 8))[-16:])) <= 1e-3))
+    out["probe_steps"] = args.probe
+    out["fallback_residual"] = args.fallback_residual
+    out["residual_after_probe"] = {
+        "cold_median": float(np.median(res_c[0])), "cold_max": float(np.max(res_c[0])),
+        "warm_median": float(np.median(res_w[0])), "warm_max": float(np.max(res_w[0]))}
+    summary(out, tc, tw, fallback, ms_step, t_inf)
+    return te, cd_c, cl_c, cd_w, cl_w, tc, tw, fallback, pred, fin
+
+
+def evaluate_saved(lib, args, storage, out):
+    """Held-out evaluation of saved networks (their predictions averaged): no snapshots, no training."""
+    nets = []
+    in_ch = IN_CH if args.model == "phase" else STATIC_CH
+    out_ch = OUT_CH if args.model == "phase" else NHARM * OUT_CH
+    for path in args.load_params.split(","):
+        d = np.load(path, allow_pickle=False)
+        keys = sorted(k for k in d.files if k.startswith("params"))
+        for k in keys:
+            n_ = Net(lib, args.H, args.W, in_ch, out_ch, base=args.base, max_batch=args.batch, seed=1)
+            n_.set_params(d[k])
+            nets.append(n_)
+    net = Ensemble(nets)
+    out["model"] = args.model
+    out["ensemble"] = len(nets)
+    out["loaded"] = args.load_params
+    held_out(lib, args, storage, net, out)
     net.close()
     print(json.dumps(out), flush=True)
     return out

@@ -59,3 +59,43 @@ def test_driver_matches_python_host_fp64(tmp_path, lib64):
     np.testing.assert_allclose(l2[:, 1], res["residual"], rtol=1e-5)
     np.testing.assert_allclose(fd[:, 1], drag["Cd"], rtol=1e-5, atol=1e-9)
     np.testing.assert_allclose(fd[:, 2], drag["Cl"], rtol=1e-5, atol=1e-9)
+
+
+def oracle_series(geom, steps, **cfg_kw):
+    """error_L2 residual, Cd and Cl of the FP64 oracle's restatement of Multi-block-LBM.cu:main."""
+    import oracle
+    lib = mbl.load_library(False)
+    g = geom.g
+    cfg = mbl.default_config(lib, LEVEL=g.LEVEL, number_line2=g.number_line2, LX=g.LX, LY=g.LY, R=g.R,
+                             CX=geom.CX, CY=geom.CY, bc_mode=geom.bc_mode, init_mode=geom.init_mode,
+                             write_files=0, number_area=g.number_area, **cfg_kw)
+    for i in range(g.LEVEL):
+        cfg.mi[i] = g.mi[i]
+    o = oracle.OracleSim(g, cfg)
+    o.run(steps)
+    out = o.series(0), o.series(3), o.series(4)
+    o.close()
+    return out
+
+
+@pytest.mark.parametrize("exe,layout,steps", [("mbl_driver64", "GEO_2_5", 12), ("mbl_driver64", "C1", 400),
+                                              ("mbl_driver", "C1", 400)])
+def test_driver_files_match_oracle(tmp_path, lib32, exe, layout, steps):
+    """The C driver's error_L2.dat and fdrag_t.dat (written with the reference's "%g", six
+    significant digits) against the FP64 oracle's series of the same layout: every line
+    within the format's rounding (5e-6 relative), the FP32 build's forces too (its residual, a
+    difference of consecutive velocity fields, carries FP32 rounding of u and is compared at
+    2e-3)."""
+    r = subprocess.run([os.path.join(BIN, exe), layout, str(steps), str(steps)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    l2 = read_dat(tmp_path / "error_L2.dat", 2)
+    fd = read_dat(tmp_path / "fdrag_t.dat", 3)
+    geom = mbl.c1_geometry(lib32) if layout == "C1" else mbl.Geometry.preset(lib32, layout)
+    res, cd, cl = oracle_series(geom, steps)
+    assert l2.shape == (steps, 2) and fd.shape == (steps, 3)
+    rtol_res = 5e-6 if exe == "mbl_driver64" else 2e-3
+    np.testing.assert_allclose(l2[:, 1], res, rtol=rtol_res, atol=0)
+    scale = max(1.0, np.max(np.abs(cd)))
+    np.testing.assert_allclose(fd[:, 1], cd, rtol=5e-6, atol=5e-6 * scale)
+    np.testing.assert_allclose(fd[:, 2], cl, rtol=5e-6, atol=5e-6 * scale)

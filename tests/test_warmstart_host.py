@@ -160,3 +160,34 @@ def test_case_set_configuration():
     assert {int(round(4.0 * s.R)) for s in tr} == {16, 24}
     key = lambda s: (s.LX, s.LY, round(s.Re, 9))
     assert not ({key(s) for s in tr} & {key(s) for s in te})
+
+
+def test_case_sets_of_the_full_range():
+    """The pipeline's default sets (Re ~ U[40, 200], training and held-out) cover the steady
+    range below the onset of shedding and do not share a case."""
+    tr, te = ws.c5_specs(256, seed=101, re_min=40.0), ws.c5_specs(64, seed=202, re_min=40.0)
+    for s in tr + te:
+        assert 40.0 <= s.Re <= 200.0
+    assert sum(s.Re < 60.0 for s in te) >= 3
+    key = lambda s: (s.LX, s.LY, round(s.Re, 9))
+    assert not ({key(s) for s in tr} & {key(s) for s in te})
+
+
+def test_convergence_rules():
+    """Shedding cases are judged by the Cl amplitude, steady ones by Cd within 0.5 %; a warm
+    case whose residual after the probe exceeds the threshold is charged probe + cold steps."""
+    t = np.arange(6000, dtype=float)
+    shed = 0.3 * (1 - np.exp(-t / 800.0)) * np.sin(2 * np.pi * t / 100.0)
+    steady_cd = 1.8 + 0.4 * np.exp(-t / 500.0)
+    cl_c = np.stack([shed, np.zeros_like(t)])
+    cd_c = np.stack([np.full_like(t, 1.3), steady_cd])
+    cl_w = np.stack([0.3 * np.sin(2 * np.pi * t / 100.0), np.zeros_like(t)])
+    cd_w = np.stack([np.full_like(t, 1.3), 1.8 + 0.4 * np.exp(-t / 100.0)])
+    tc, tw, fb = ws.convergence(cl_c, cl_w, cd_w, cd_c=cd_c, res0=np.array([0.001, 0.001]), threshold=0.02,
+                                probe=20)
+    assert fb == 0 and tc[0] > tw[0] > 0 and tc[1] > tw[1] > 0
+    # steady: the first step of |Cd - final| <= 0.5 % of final
+    assert abs(tc[1] - 500.0 * np.log(0.4 / (0.005 * steady_cd[-1]))) <= 2
+    tc2, tw2, fb2 = ws.convergence(cl_c, cl_w, cd_w, cd_c=cd_c, res0=np.array([0.5, np.nan]), threshold=0.02,
+                                   probe=20)
+    assert fb2 == 2 and list(tw2) == [20 + tc2[0], 20 + tc2[1]]
