@@ -33,13 +33,12 @@ struct WgradArgs {
     const bf16 *X;
     int ldx, xoff;
     int N, H, W, Cin, Cout, ksize;
-    float *partial;     // [nsplit][Cout][ksize*ksize][Cin]
+    float *partial;     // [nsplit][Cout*ksize*ksize*Cin + Cout]: weights, then the bias
 };
 
 void conv(hipStream_t s, const ConvArgs &a);
 void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit);
 void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int accumulate);
-void bias_grad(hipStream_t s, const bf16 *dY, int ldd, int doff, int P, int C, float *db, float *partial);
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb);
 void to_bf16(hipStream_t s, const float *src, size_t n, bf16 *dst);
 void maxpool(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg);

@@ -189,7 +189,7 @@ extern "C" int mbl_net_create(mbl_handle_t h, int H, int W, int in_ch, int out_c
         const Layer &l = n->L[i];
         int level = i < 8 ? (int)(i / 2) : (i < 14 ? 2 - (int)((i - 8) / 2) : 0);
         int nsplit = std::max(1, P_of(n, max_batch, level) / 4096);
-        n->partial_n = std::max(n->partial_n, (size_t)nsplit * l.nw());
+        n->partial_n = std::max(n->partial_n, (size_t)nsplit * (l.nw() + l.cout));
     }
     n->partial = (float *)dev(n, sizeof(float) * n->partial_n);
     for (void *p : n->allocs) ok = ok && p;
@@ -309,9 +309,8 @@ static void conv_bwd(mbl_net *n, int li, int N, const bf16 *dZ, int ldd, const b
     wa.Cout = l.cout;
     wa.ksize = l.ksize;
     wa.partial = n->partial;
-    conv_wgrad(n->s, wa, nsplit);
-    reduce_partials(n->s, n->partial, nsplit, l.nw(), n->g + l.woff, 0);
-    bias_grad(n->s, dZ, ldd, 0, P, l.cout, n->g + l.boff, n->partial);
+    conv_wgrad(n->s, wa, nsplit); // weight and bias partials (the bias follows the weights)
+    reduce_partials(n->s, n->partial, nsplit, l.nw() + l.cout, n->g + l.woff, 0);
     if (!dX) return;
     ConvArgs a;
     memset(&a, 0, sizeof(a));

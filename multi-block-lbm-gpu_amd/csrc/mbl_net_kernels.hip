@@ -146,7 +146,11 @@ __device__ inline int fdiv(int v, int d, float inv)
 // tap-shifted X[.][ci0..ci0+31] with one 16-byte load per thread and operand,
 // stores both transposed in LDS ([channel][pixel], so each lane's 8 K values
 // are one 16-byte LDS read), and each wave runs two MFMA k-steps (K = 32
-// pixels) on its 16 x 16 sub-tile.
+// pixels) on its 16 x 16 sub-tile. The workgroups of the first tap and input
+// channel tile also sum the dY values they load: the bias gradient
+// db[co] = sum_p dY[p][co], written after the weights' partials (the bias
+// follows the weights in the parameter vector), so one ordered reduction
+// yields both.
 // ---------------------------------------------------------------------------
 constexpr int WG_PX = 64;      // pixels per stage
 constexpr int WG_LD = WG_PX + 8; // LDS row length (bf16): rows stay 16-byte aligned, banks spread
@@ -155,6 +159,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
 {
     __shared__ __attribute__((aligned(16))) bf16 sA[32 * WG_LD]; // dY^T
     __shared__ __attribute__((aligned(16))) bf16 sB[32 * WG_LD]; // X^T (tap-shifted)
+    __shared__ float sBias[64][33];                              // per pixel lane: its 32 co sums
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int taps = a.ksize * a.ksize, half = a.ksize / 2;
@@ -172,6 +177,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
     // compute role: wave w owns co rows 16*(w&1).. and ci columns 16*(w>>1)..
     const int wco = (w & 1) * 16, wci = (w >> 1) * 16;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const bool bias = (blockIdx.y == 0); // tap 0, input channels 0..31
+    float bs[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int s0 = pb; s0 < pe; s0 += WG_PX) {
         const int p = s0 + lp;
         uint4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
@@ -186,6 +193,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
         }
         const bf16 *ea = reinterpret_cast<const bf16 *>(&va);
         const bf16 *eb = reinterpret_cast<const bf16 *>(&vb);
+        if (bias) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) bs[i] += bf2f(ea[i]);
+        }
         __syncthreads(); // the previous stage's reads are done
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -201,11 +212,22 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
         }
     }
     // acc[r] = D[co = co0 + wco + 4g + r][ci = ci0 + wci + r16]
-    float *out = a.partial + (size_t)blockIdx.z * a.Cout * taps * a.Cin;
+    const size_t nw = (size_t)a.Cout * taps * a.Cin;
+    float *out = a.partial + (size_t)blockIdx.z * (nw + a.Cout);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int co = co0 + wco + 4 * g + r;
         out[((size_t)co * taps + tap) * a.Cin + ci0 + wci + r16] = acc[r];
+    }
+    if (bias) { // the 64 pixel lanes' sums of each co, added in lane order
+#pragma unroll
+        for (int i = 0; i < 8; i++) sBias[lp][lc + i] = bs[i];
+        __syncthreads();
+        if (t < 32) {
+            float sum = 0.f;
+            for (int q = 0; q < 64; q++) sum += sBias[q][t];
+            out[nw + co0 + t] = sum;
+        }
     }
 }
 
@@ -217,25 +239,6 @@ __global__ void k_reduce_partials(const float *partial, int nsplit, size_t n, fl
     float s = 0.f;
     for (int k = 0; k < nsplit; k++) s += partial[(size_t)k * n + i];
     out[i] = accumulate ? out[i] + s : s;
-}
-
-// Partial bias gradients over one pixel chunk per blockIdx.y:
-// partial[chunk][c] = sum_{p in chunk} dY[p][c]. A workgroup covers 64 channels
-// (contiguous across lanes, so each pixel row is one coalesced read) times 4
-// pixel lanes; the chunk partials are then reduced in order (deterministic).
-__global__ __launch_bounds__(256) void k_bias_grad(const bf16 *dY, int ldd, int doff, int P, int C, int chunk,
-                                                   float *partial)
-{
-    __shared__ float sh[4][64];
-    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + lane;
-    const int p0 = blockIdx.y * chunk, p1 = min(P, p0 + chunk);
-    float s = 0.f;
-    if (c < C)
-        for (int p = p0 + q; p < p1; p += 4) s += bf2f(dY[(size_t)p * ldd + doff + c]);
-    sh[q][lane] = s;
-    __syncthreads();
-    if (q == 0 && c < C) partial[(size_t)blockIdx.y * C + c] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
 }
 
 // Backward-data weights: Wb[ci][tap][co] = W[co][taps-1-tap][ci] (BF16)
@@ -415,12 +418,6 @@ void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit)
 void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int acc)
 {
     hipLaunchKernelGGL(k_reduce_partials, g1(n), dim3(256), 0, s, partial, nsplit, n, out, acc);
-}
-void bias_grad(hipStream_t s, const bf16 *dY, int ldd, int doff, int P, int C, float *db, float *partial)
-{
-    const int chunk = 2048, nchunk = (P + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_bias_grad, dim3((C + 63) / 64, nchunk), dim3(256), 0, s, dY, ldd, doff, P, C, chunk, partial);
-    reduce_partials(s, partial, nchunk, (size_t)C, db, 0);
 }
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb)
 {
