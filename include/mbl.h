@@ -441,6 +441,14 @@ void mbl_batch_destroy(mbl_batch_t b);
 /* Advances every case by nsteps coarse steps; the momentum-exchange force on
  * every cylinder is reduced on the device after each step (FP64). */
 int mbl_batch_step(mbl_batch_t b, int nsteps);
+/* on != 0: each mbl_batch_step call is one persistent launch of one workgroup
+ * per case (largest first) that runs all nsteps coarse steps of its case -- the
+ * refinement cycle in the reference order and the drag, with workgroup
+ * barriers only -- and returns the same bits as the captured-graph path. Lines
+ * of up to 303 coarse points; other layouts return MBL_E_UNSUPPORTED from
+ * mbl_batch_step. on = 0 (default): the captured-graph path. No reference
+ * counterpart (the reference has one case). */
+int mbl_batch_set_persistent(mbl_batch_t b, int on);
 /* Node updates (all levels, all cases) per coarse step. */
 double mbl_batch_node_updates(mbl_batch_t b);
 /* Cd and Cl of every case (normalised as getDrag_block) for the last `nt`
@@ -455,7 +463,9 @@ int mbl_batch_forces(mbl_batch_t b, int nt, double *cd, double *cl);
 int mbl_batch_residual(mbl_batch_t b, double *out);
 /* Times `trial_steps` steps for every (nodes per thread, threads per group,
  * persistent groups) candidate and keeps the fastest; the trial steps advance
- * the cases. Returns 0 and the choice and its time per step. */
+ * the cases. Returns 0 and the choice and its time per step. With the
+ * one-launch path on (mbl_batch_set_persistent) the candidates are the nodes
+ * per thread and threads per tile of its stream-collide items. */
 int mbl_batch_autotune(mbl_batch_t b, int trial_steps, int *vx, int *tpb, int *groups, double *ms_per_step);
 
 /* Streaming-copy bandwidth (bytes read + written per second, GB/s) of a

@@ -156,6 +156,7 @@ struct mbl_context {
     bool vx_set = false;  // vx chosen by mbl_set_tuning or the autotuner, not by default
     int groups = 0; // persistent grid size of the stream-collide launches (0: one workgroup per tile)
     Layout *layout = nullptr;
+    uint64_t layout_gen = 0; // advances each time a layout is built
     Single single;
     Residual resid;
     // diagnostics: device record slots and pinned host mirrors
@@ -1533,11 +1534,9 @@ extern "C" void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lcc
     evolution_impl(h, A, level, nblocks, nline, nline2);
 }
 
-static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2)
+// The layout of a multi-block call: the handle's, rebuilt when the caller's tables changed.
+static Layout *layout_for(mbl_context *h, const EvoArgs &A, int nblocks, int nline, int nline2)
 {
-    const mbl_config &cfg = h->cfg;
-    D2Q9 *pps_d = A.pps;
-    Ff_Store *line_t3 = A.t3;
     const uint64_t sig = layout_sig(h, A, nblocks, nline, nline2);
     if (!h->layout || h->layout->sig != sig) {
         if (h->layout) {
@@ -1547,39 +1546,55 @@ static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblo
             h->layout = nullptr;
         }
         Layout *L = nullptr;
-        if (build_layout(h, A, nblocks, nline, nline2, sig, L)) return;
+        if (build_layout(h, A, nblocks, nline, nline2, sig, L)) return nullptr;
         h->layout = L;
+        h->layout_gen++;
     }
-    Layout *L = h->layout;
-    // current parity of every level from the caller's F pointers
-    uint32_t pm = 0;
-    for (int l = 0; l < cfg.LEVEL; l++) {
+    return h->layout;
+}
+
+// The parity of every level (which plane set holds its state, from the caller's F pointers)
+// and the phase key: parity bits, ring markers of every line group, starting level.
+static int phase_key(const Layout *L, const D2Q9 *pps_d, const Ff_Store *line_t3, int level, uint32_t &pm,
+                     uint64_t &key)
+{
+    pm = 0;
+    for (int l = 0; l < L->LEVEL; l++) {
         int par = -1;
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++) {
             int p = (pps_d[a].F[0] == (Lbd *)L->hb[a].P[0][0]) ? 0 : ((pps_d[a].F[0] == (Lbd *)L->hb[a].P[1][0]) ? 1 : -1);
-            if (p < 0 || (par >= 0 && p != par)) {
-                set_err(h, MBL_E_INVALID);
-                return;
-            }
+            if (p < 0 || (par >= 0 && p != par)) return MBL_E_INVALID;
             par = p;
         }
         if (par == 1) pm |= (1u << l);
     }
-    // phase key: parity bits + markers of every line group
-    uint64_t key = pm;
+    key = pm;
     int sh = 8;
-    for (int l = 0; l < cfg.LEVEL; l++) {
+    for (int l = 0; l < L->LEVEL; l++) {
         int m[3];
         if (L->LS3[l + 1] > L->LS3[l]) {
-            if (markers_of(L, line_t3, L->LS3[l], L->LS3[l + 1], m)) {
-                set_err(h, MBL_E_UNSUPPORTED);
-                return;
-            }
+            if (markers_of(L, line_t3, L->LS3[l], L->LS3[l + 1], m)) return MBL_E_UNSUPPORTED;
             key |= (uint64_t)(m[0] | (m[1] << 2) | (m[2] << 4)) << sh;
         }
         sh += 6;
     }
     key |= (uint64_t)level << 60;
+    return 0;
+}
+
+static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2)
+{
+    const mbl_config &cfg = h->cfg;
+    D2Q9 *pps_d = A.pps;
+    Ff_Store *line_t3 = A.t3;
+    Layout *L = layout_for(h, A, nblocks, nline, nline2);
+    if (!L) return;
+    uint32_t pm = 0;
+    uint64_t key = 0;
+    if (int rc = phase_key(L, pps_d, line_t3, level, pm, key)) {
+        set_err(h, rc);
+        return;
+    }
     // the cycle in the reference order (markers rotate here), then its scheduled launches
     std::vector<Ff_Store> saved(line_t3, line_t3 + nline);
     CycleState cs{pm, 0, {}};
@@ -2433,6 +2448,27 @@ struct mbl_batch {
     int *d_rtblk = nullptr, *d_case_tiles = nullptr;
     double *d_rpart = nullptr, *d_rout = nullptr;
     int rtotal = 0;
+    // one launch per mbl_batch_step call, one workgroup per case (mbl_batch_set_persistent):
+    // the item tables of every step of the phase period, built for one layout generation
+    bool persistent = false;
+    struct CaseProgram {
+        uint64_t gen = 0;                     // layout generation of the tables (0: none)
+        CaseItem *d_items = nullptr;          // [period][n][per_case]
+        LineOp *d_ops = nullptr;              // interface and spline ops, each case's contiguous
+        int *d_case = nullptr;                // workgroup -> case, largest case first
+        int per_case = 0, period = 0;
+        size_t lds = 0;
+        std::vector<uint64_t> key;            // phase key at the start of each step of the period
+        std::vector<uint32_t> flips;          // levels whose parity each step flips
+        std::vector<std::vector<int>> markers; // ring markers of every line after each step
+        void release()
+        {
+            if (d_items) hipFree(d_items);
+            if (d_ops) hipFree(d_ops);
+            if (d_case) hipFree(d_case);
+            *this = CaseProgram();
+        }
+    } cp;
 };
 
 // u_prev <- u for every block of a batch (after creation and after every restart), so the
@@ -2720,6 +2756,7 @@ extern "C" void mbl_batch_destroy(mbl_batch_t B)
     for (void *q : {(void *)B->d_rdesc, (void *)B->d_rtblk, (void *)B->d_case_tiles, (void *)B->d_rpart,
                     (void *)B->d_rout})
         if (q) hipFree(q);
+    B->cp.release();
     delete B;
 }
 
@@ -2779,11 +2816,243 @@ extern "C" double mbl_batch_node_updates(mbl_batch_t B)
     return u;
 }
 
-extern "C" int mbl_batch_step(mbl_batch_t B, int nsteps)
+// The item tables of the one-launch batch step. From the current state, the refinement
+// cycle is recorded in the reference order (cycle(), the same recording evolution() schedules)
+// step after step until the phase key (level parities and ring markers) comes back: the
+// period. Each case's items are the recorded ones restricted to its own blocks and lines --
+// the stream-collide of a level over its tiles, each sub-batch of interface ops and the
+// spline lines that touch its blocks -- followed by its drag; a workgroup runs them in that
+// order with a barrier between items, so every read sees what it sees in the reference order.
+static int batch_case_program(mbl_batch *B, Layout *L)
+{
+    mbl_context *h = B->h;
+    const int n = B->n;
+    mbl_batch::CaseProgram &cp = B->cp;
+    cp.release();
+    if (L->LEVEL != 2 || L->vx[0] != L->vx[1]) return MBL_E_UNSUPPORTED;
+    std::vector<int> case_of(5 * (size_t)n);
+    for (int c = 0; c < n; c++) {
+        for (int q = 0; q < 4; q++) case_of[4 * c + q] = c;
+        case_of[4 * n + c] = c;
+    }
+    // tiles of case c at level l: its blocks there are consecutive in the tile numbering
+    auto block_tiles = [&](int a) {
+        const int l = L->level_of[a];
+        const int end = (a + 1 < L->ASP[l + 1]) ? L->hb[a + 1].tile_begin : L->tile_off[l] + L->tiles[l];
+        return end - L->hb[a].tile_begin;
+    };
+    std::vector<int> t0(2 * (size_t)n), tn(2 * (size_t)n);
+    for (int c = 0; c < n; c++) {
+        t0[2 * c] = L->hb[4 * c].tile_begin;
+        tn[2 * c] = 0;
+        for (int q = 0; q < 4; q++) tn[2 * c] += block_tiles(4 * c + q);
+        if (L->hb[4 * c + 3].tile_begin + block_tiles(4 * c + 3) - t0[2 * c] != tn[2 * c]) return MBL_E_UNSUPPORTED;
+        t0[2 * c + 1] = L->hb[4 * n + c].tile_begin;
+        tn[2 * c + 1] = block_tiles(4 * n + c);
+    }
+    // the steps of one period
+    uint32_t pm0;
+    uint64_t key0;
+    if (int rc = phase_key(L, B->pps.data(), B->t3.data(), 0, pm0, key0)) return rc;
+    std::vector<Ff_Store> t3 = B->t3;
+    std::vector<D2Q9> pps = B->pps;
+    std::vector<std::vector<CycleItem>> steps;
+    std::vector<int> par;
+    for (int st = 0; st < 12; st++) {
+        uint32_t pm;
+        uint64_t key;
+        if (int rc = phase_key(L, pps.data(), t3.data(), 0, pm, key)) return rc;
+        if (st > 0 && key == key0) break;
+        CycleState cs{pm, 0, {}};
+        cycle(L, 0, t3.data(), cs);
+        if (cs.rc) return cs.rc;
+        const uint32_t fl = cs.pm ^ pm;
+        for (int l = 0; l < L->LEVEL; l++)
+            if (fl & (1u << l))
+                for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
+                    for (int k = 0; k < 9; k++) std::swap(pps[a].F[k], pps[a].f[k]);
+        std::vector<int> mk(3 * t3.size());
+        for (size_t b = 0; b < t3.size(); b++)
+            for (int q = 0; q < 3; q++) mk[3 * b + q] = t3[b].marker[q];
+        cp.key.push_back(key);
+        cp.flips.push_back(fl);
+        cp.markers.push_back(mk);
+        par.push_back(pps[4 * n].F[0] == (Lbd *)B->fine_F0 ? 0 : 1);
+        steps.push_back(std::move(cs.items));
+    }
+    {
+        uint32_t pm;
+        uint64_t key;
+        if (phase_key(L, pps.data(), t3.data(), 0, pm, key) || key != key0) return MBL_E_UNSUPPORTED;
+    }
+    const int period = (int)steps.size();
+    const std::vector<CycleItem> &it0 = steps[0];
+    for (const auto &v : steps) {
+        if (v.size() != it0.size()) return MBL_E_UNSUPPORTED;
+        for (size_t j = 0; j < v.size(); j++)
+            if (v[j].kind != it0[j].kind || v[j].level != it0[j].level || v[j].off != it0[j].off ||
+                v[j].count != it0[j].count)
+                return MBL_E_UNSUPPORTED;
+    }
+    // each case's ops of every recorded item, contiguous in the program's op table
+    const size_t ni = it0.size();
+    std::vector<LineOp> tab;
+    std::vector<int> ooff(ni * n, 0), ocnt(ni * n, 0), omax(ni * n, 0);
+    int max_nc = 0;
+    for (int c = 0; c < n; c++)
+        for (size_t j = 0; j < ni; j++) {
+            if (it0[j].kind == 0) continue;
+            ooff[j * n + c] = (int)tab.size();
+            for (int o = it0[j].off; o < it0[j].off + it0[j].count; o++) {
+                const LineOp &op = L->hops[o];
+                if (case_of[op.dst_block] != c) continue;
+                if (case_of[op.src_block] != c) return MBL_E_UNSUPPORTED;
+                tab.push_back(op);
+                ocnt[j * n + c]++;
+                omax[j * n + c] = std::max(omax[j * n + c], op.n);
+                if (op.type == OP_SPLINE) max_nc = std::max(max_nc, op.nc);
+            }
+        }
+    if (!cases_spline_ok(max_nc)) return MBL_E_UNSUPPORTED;
+    // items: [step][case][k]
+    std::vector<CaseItem> items;
+    int per_case = -1;
+    for (int st = 0; st < period; st++)
+        for (int c = 0; c < n; c++) {
+            const size_t before = items.size();
+            for (size_t j = 0; j < ni; j++) {
+                const CycleItem &ci = steps[st][j];
+                CaseItem x;
+                memset(&x, 0, sizeof(x));
+                x.pm = ci.pm;
+                x.ms = ci.ms;
+                x.xm = ci.xm;
+                if (ci.kind == 0) {
+                    x.kind = ITEM_COLLIDE;
+                    x.a = t0[2 * c + ci.level];
+                    x.b = tn[2 * c + ci.level];
+                    if (x.b == 0) continue;
+                } else {
+                    if (ocnt[j * n + c] == 0) continue;
+                    x.kind = ci.kind == 2 ? ITEM_SPLINE : ITEM_OPS;
+                    x.a = ooff[j * n + c];
+                    x.b = ocnt[j * n + c];
+                    x.max_n = omax[j * n + c];
+                }
+                items.push_back(x);
+            }
+            CaseItem d;
+            memset(&d, 0, sizeof(d));
+            d.kind = ITEM_DRAG;
+            d.a = par[st];
+            items.push_back(d);
+            const int cnt = (int)(items.size() - before);
+            if (per_case >= 0 && cnt != per_case) return MBL_E_UNSUPPORTED;
+            per_case = cnt;
+        }
+    // workgroups take the cases largest first
+    std::vector<int> order(n);
+    std::vector<double> work(n, 0.0);
+    for (int a = 0; a < 5 * n; a++) work[case_of[a]] += (double)B->areas[a].nx * B->areas[a].ny * B->areas[a].m;
+    for (int c = 0; c < n; c++) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work[x] > work[y]; });
+    if (hipMalloc(&cp.d_items, sizeof(CaseItem) * items.size()) != hipSuccess ||
+        hipMalloc(&cp.d_ops, sizeof(LineOp) * std::max<size_t>(1, tab.size())) != hipSuccess ||
+        hipMalloc(&cp.d_case, sizeof(int) * n) != hipSuccess) {
+        cp.release();
+        return MBL_E_NOMEM;
+    }
+    bool ok = hipMemcpy(cp.d_items, items.data(), sizeof(CaseItem) * items.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !tab.empty())
+        ok = hipMemcpy(cp.d_ops, tab.data(), sizeof(LineOp) * tab.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) ok = hipMemcpy(cp.d_case, order.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        cp.release();
+        return MBL_E_HIP;
+    }
+    cp.per_case = per_case;
+    cp.period = period;
+    cp.lds = cases_lds(max_nc);
+    cp.gen = h->layout_gen;
+    (void)pm0;
+    return 0;
+}
+
+// nsteps coarse steps of every case in one launch (one workgroup per case), then the host
+// state (plane pointer order, ring markers, force history) advanced as the steps leave it.
+static int batch_step_persistent(mbl_batch *B, int nsteps)
+{
+    mbl_context *h = B->h;
+    const int n = B->n;
+    EvoArgs A{B->asp, B->ls3, B->ltc, B->lccp, B->areas.data(), B->pps.data(), B->lc.data(), B->lf.data(),
+              B->lcc.data(), B->t3.data(), B->s3.data()};
+    A.bc = B->bc.data();
+    Layout *L = layout_for(h, A, 5 * n, 4 * n, 8 * n);
+    if (!L) return h->last_error ? h->last_error : set_err(h, MBL_E_INVALID);
+    mbl_batch::CaseProgram &cp = B->cp;
+    uint32_t pm;
+    uint64_t key;
+    if (int rc = phase_key(L, B->pps.data(), B->t3.data(), 0, pm, key)) return set_err(h, rc);
+    int q = -1;
+    if (cp.gen == h->layout_gen)
+        for (int i = 0; i < cp.period; i++)
+            if (cp.key[i] == key) q = i;
+    if (q < 0) {
+        hipStreamSynchronize(h->stream);
+        if (int rc = batch_case_program(B, L)) return set_err(h, rc);
+        q = 0;
+    }
+    CaseArgs a;
+    memset(&a, 0, sizeof(a));
+    a.items = cp.d_items;
+    a.per_case = cp.per_case;
+    a.period = cp.period;
+    a.step0 = q;
+    a.ncases = n;
+    a.nsteps = nsteps;
+    a.case_of_wg = cp.d_case;
+    a.descs = L->db;
+    a.tile_blk = L->dtile;
+    a.ndesc = L->nblocks;
+    a.rows = L->rows;
+    a.tpb = h->tpb;
+    a.ops = cp.d_ops;
+    a.drag[0] = B->d_drag[0];
+    a.drag[1] = B->d_drag[1];
+    a.hist = B->d_hist;
+    a.head0 = B->head;
+    a.cap = B->cap;
+    if (int rc = launch_cases(h->stream, storage_mode(h->cfg), L->vx[0], a, cp.lds)) return set_err(h, rc);
+    for (int st = 0; st < nsteps; st++) {
+        const uint32_t fl = cp.flips[(q + st) % cp.period];
+        for (int l = 0; l < L->LEVEL; l++)
+            if (fl & (1u << l))
+                for (int ar = L->ASP[l]; ar < L->ASP[l + 1]; ar++)
+                    for (int k = 0; k < 9; k++) std::swap(B->pps[ar].F[k], B->pps[ar].f[k]);
+    }
+    if (nsteps > 0) {
+        const std::vector<int> &mk = cp.markers[(q + nsteps - 1) % cp.period];
+        for (size_t b = 0; b < B->t3.size(); b++)
+            for (int p = 0; p < 3; p++) B->t3[b].marker[p] = mk[3 * b + p];
+    }
+    B->head = (int)((B->head + (long long)nsteps) % B->cap);
+    B->nhist = (int)std::min<long long>((long long)B->nhist + nsteps, B->cap);
+    return 0;
+}
+
+extern "C" int mbl_batch_set_persistent(mbl_batch_t B, int on)
 {
     if (!B) return MBL_E_INVALID;
+    B->persistent = on != 0;
+    return 0;
+}
+
+extern "C" int mbl_batch_step(mbl_batch_t B, int nsteps)
+{
+    if (!B || nsteps < 0) return MBL_E_INVALID;
     mbl_context *h = B->h;
     if (h->last_error) return h->last_error;
+    if (B->persistent) return batch_step_persistent(B, nsteps);
     EvoArgs A{B->asp, B->ls3, B->ltc, B->lccp, B->areas.data(), B->pps.data(), B->lc.data(), B->lf.data(),
               B->lcc.data(), B->t3.data(), B->s3.data()};
     A.bc = B->bc.data();
@@ -2934,6 +3203,7 @@ extern "C" int mbl_batch_autotune(mbl_batch_t B, int trial_steps, int *vx_out, i
     for (int v : vxs)
         for (int t : tpbs)
             for (int g : gs) {
+                if (B->persistent && g != 0) continue; // one workgroup per case: no grid size to pick
                 h->vx = v;
                 h->tpb = t;
                 h->groups = g;

@@ -119,6 +119,47 @@ struct RasterCase {
     double LX, LY;
 };
 
+// ---- batched cases in one persistent launch ----
+// One item of a case's coarse step, in the reference order: a stream-collide over the case's
+// tiles of one level, one sub-batch of its interface ops, its spline lines, or its drag.
+enum ItemKind { ITEM_COLLIDE = 0, ITEM_OPS = 1, ITEM_SPLINE = 2, ITEM_DRAG = 3 };
+struct CaseItem {
+    int kind;
+    int a, b;      // COLLIDE: first tile, tiles; OPS/SPLINE: first op, ops; DRAG: drag table
+    int max_n;     // OPS: entries of the longest op
+    uint32_t pm;   // level parities in force
+    MarkerSet ms;  // ring markers in force
+    double xm;     // OPS: temporal interpolation point
+};
+
+// Threads per workgroup (one case each; 8 waves, up to 256 VGPRs, so the inlined items do not
+// spill); the splines of a case run in one workgroup, so lines of up to CASE_TPB coarse points.
+constexpr int CASE_TPB = 512;
+
+struct CaseArgs {
+    const CaseItem *items; // [period][ncases][per_case]
+    int per_case;          // items of one case per coarse step
+    int period;            // coarse steps in the table (it repeats)
+    int step0;             // table step of the launch's first step
+    int ncases;
+    int nsteps;
+    const int *case_of_wg; // workgroup -> case
+    const BlockDesc *descs;
+    const int *tile_blk;
+    int ndesc;
+    int rows;              // stream-collide tile rows
+    int tpb;               // threads per stream-collide tile
+    const LineOp *ops;
+    const DragArgs *drag[2];
+    double *hist;          // Fx, Fy of case c after step s -> hist[2 (ncases ((head0 + s) % cap) + c)]
+    int head0, cap;
+};
+
+size_t cases_lds(int max_nc);
+// 1 when spline lines of max_nc coarse points run inside the case kernel
+int cases_spline_ok(int max_nc);
+int launch_cases(void *stream, int mode, int vx, const CaseArgs &a, size_t lds);
+
 // ---- launchers (mbl_kernels.hip), all asynchronous on `stream` ----
 // tile_rows rows per tile (at least tpb / 64; the waves take them in turn);
 // d_tile_blk[t] = descriptor of tile t, or null to search tile_begin.

@@ -687,9 +687,10 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
 // One tile: tw*VX columns x rows rows of one block; each wave takes 64/tw
 // rows at a time, the workgroup's waves the next ones in turn. The block comes from the per-tile table when
 // there is one (one scalar load), else from a search of tile_begin.
+// wv: the wave's index among the nwv waves that take the tile.
 template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
 __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, const int *__restrict__ tile_blk, int ndesc,
-                               int tile_rows, uint32_t parity_mask, int t)
+                               int tile_rows, uint32_t parity_mask, int t, int wv, int nwv)
 {
     int lo = 0;
     if (tile_blk) {
@@ -721,10 +722,10 @@ __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, con
     const int per_wave = two ? 2 : 1, sub = two ? (lane >> 5) : 0, col = lane & (tw - 1);
     const int base = ((int)txi * tw + col) * VX;
     const int rows = d.trows > 0 ? d.trows : tile_rows;
-    const int step = (int)(blockDim.x >> 6) * per_wave;
+    const int step = nwv * per_wave;
     const uint32_t unx = (uint32_t)d.nx;
     // the wave's first row rw and its row offset are scalar; a lane adds its segment's row (sub)
-    for (int rw = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * per_wave; rw < rows; rw += step) {
+    for (int rw = __builtin_amdgcn_readfirstlane(wv) * per_wave; rw < rows; rw += step) {
         if (rw + sub >= rows) break;
         const int jw = (int)tyi * rows + rw;
         const uint32_t r0 = (uint32_t)jw * unx + (sub ? unx : 0u);
@@ -757,7 +758,8 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
     t1 += tile_base;
     // one call site, inlined: the descriptor and plane bases stay in scalar registers
     for (int t = t0; t < t1; t++)
-        sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, t);
+        sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, t,
+                                                   (int)(threadIdx.x >> 6), (int)(blockDim.x >> 6));
 }
 
 // ---------------------------------------------------------------------------
@@ -872,15 +874,12 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
 // after the op table (two for CC/FC: index, then source node), not one more
 // for the mask.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
-                                                  uint32_t parity_mask, MarkerSet ms, double xm)
+__device__ __forceinline__ void line_op_entry(const LineOp &op, int e, const BlockDesc *__restrict__ blocks,
+                                              uint32_t parity_mask, const MarkerSet &ms, double xm)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
-    const LineOp &op = ops[blockIdx.y];
     const int m0 = ms.m[op.mgroup][0], m1 = ms.m[op.mgroup][1], m2 = ms.m[op.mgroup][2];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= op.n) return;
     const BlockDesc &db = blocks[op.dst_block];
     const int pd = (parity_mask >> db.level) & 1;
     const uint8_t mask = op.mask[e];
@@ -935,6 +934,16 @@ __global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops
             D[k][didx] = enc64<MODE>(fe + op.para * (ft - fe), k);
         }
     }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_line_ops(const LineOp *__restrict__ ops, const BlockDesc *__restrict__ blocks,
+                                                  uint32_t parity_mask, MarkerSet ms, double xm)
+{
+    const LineOp &op = ops[blockIdx.y];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= op.n) return;
+    line_op_entry<MODE>(op, e, blocks, parity_mask, ms, xm);
 }
 
 // init_Fstore: every ring slot <- the fine edge's F (init_multi.h:3-28)
@@ -1125,17 +1134,15 @@ __global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ 
 constexpr int SPL_FUSED_MAX_NC = 303; // 27 * 8 * nc bytes of LDS within 64 KiB
 
 template <int MODE>
-__global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict__ ops,
-                                                       const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
-                                                       MarkerSet ms)
+__device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *__restrict__ blocks,
+                                            uint32_t parity_mask, const MarkerSet &ms, double *lds)
 {
     typedef typename Codec<MODE>::T T;
-    const LineOp &op = ops[blockIdx.y];
     const int slot = ms.m[op.mgroup][0]; // the oldest slot of the line's ring receives t_{n+1}
     const int N = op.nc;
     const BlockDesc &sb = blocks[op.src_block];
     const int ps = (parity_mask >> sb.level) & 1;
-    extern __shared__ double lds[]; // G[12][N] | a[N] | b[N] | c[N] | d[12][N]
+    // lds: G[12][N] | a[N] | b[N] | c[N] | d[12][N]
     double *G = lds, *La = lds + 12 * N, *Lb = La + N, *Lc = Lb + N, *Ld = Lc + N;
     const int ip = threadIdx.x;
     const bool row = ip < N;
@@ -1235,6 +1242,15 @@ __global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict_
     }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict__ ops,
+                                                       const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
+                                                       MarkerSet ms)
+{
+    extern __shared__ double lds[];
+    spline_line<MODE>(ops[blockIdx.y], blocks, parity_mask, ms, lds);
+}
+
 // init_tri_A (multi/init_multi.h:30-51), final race-free state
 __global__ void k_tri_A(Lbd *l, Lbd *m, Lbd *u, int N)
 {
@@ -1311,11 +1327,9 @@ template <int MODE> __global__ __launch_bounds__(1024) void k_drag(DragArgs a)
 
 // Momentum exchange of many cases at once: workgroup c reduces case c's
 // links into out[2c], out[2c+1] (same per-link terms as k_drag).
-template <int MODE> __global__ __launch_bounds__(256) void k_drag_batch(const DragArgs *__restrict__ args, double *out)
+template <int MODE> __device__ __forceinline__ void drag_case(const DragArgs &a, double *out2, double *sh)
 {
     typedef typename Codec<MODE>::T T;
-    __shared__ double sh[16];
-    const DragArgs &a = args[blockIdx.x];
     double fx = 0.0, fy = 0.0;
     for (int e = threadIdx.x; e < a.n; e += blockDim.x) {
         const Int3 L = a.links[e];
@@ -1330,9 +1344,15 @@ template <int MODE> __global__ __launch_bounds__(256) void k_drag_batch(const Dr
     const double sx = group_sum(fx, sh);
     const double sy = group_sum(fy, sh);
     if (threadIdx.x == 0) {
-        out[2 * blockIdx.x] = sx;
-        out[2 * blockIdx.x + 1] = sy;
+        out2[0] = sx;
+        out2[1] = sy;
     }
+}
+
+template <int MODE> __global__ __launch_bounds__(256) void k_drag_batch(const DragArgs *__restrict__ args, double *out)
+{
+    __shared__ double sh[16];
+    drag_case<MODE>(args[blockIdx.x], out + 2 * blockIdx.x, sh);
 }
 
 // error_L2_area (error_L2.h:3-55): workgroup t takes tile t of 64 x 16 nodes
@@ -1605,6 +1625,77 @@ __global__ __launch_bounds__(256) void k_plane_copy(const uint4 *__restrict__ sr
 }
 
 // ---------------------------------------------------------------------------
+// Batched cases in one persistent launch, one workgroup per case: the workgroup
+// runs its case's coarse steps from a table of items in the reference order
+// (stream-collide of a level, a sub-batch of interface ops, the spline lines, the
+// drag), with a workgroup barrier between items. Cases are independent, so no
+// workgroup waits for another. The device code of each item is the separate
+// launches' own: the same per-node and per-entry arithmetic.
+// ---------------------------------------------------------------------------
+// The items of the case kernel (inlined: the launch's constants stay in scalar registers).
+template <int MODE>
+__device__ __forceinline__ void case_ops(const CaseItem &p, const LineOp *__restrict__ ops,
+                                      const BlockDesc *__restrict__ descs)
+{
+    const int total = p.b * p.max_n;
+    for (int g = (int)threadIdx.x; g < total; g += CASE_TPB) {
+        const int o = g / p.max_n, e = g - o * p.max_n;
+        const LineOp &op = ops[p.a + o];
+        if (e < op.n) line_op_entry<MODE>(op, e, descs, p.pm, p.ms, p.xm);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void case_splines(const CaseItem &p, const LineOp *__restrict__ ops,
+                                          const BlockDesc *__restrict__ descs, double *lds)
+{
+    for (int o = 0; o < p.b; o++) {
+        spline_line<MODE>(ops[p.a + o], descs, p.pm, p.ms, lds);
+        __syncthreads(); // the next line reuses the LDS
+    }
+}
+
+template <int MODE, int VX>
+__device__ __forceinline__ void case_collide(const CaseItem &p, const CaseArgs &a, int grp, int ng, int wv, int nwv)
+{
+    for (int t = grp; t < p.b; t += ng)
+        sc_tile<MODE, VX, false, true, true>(a.descs, a.tile_blk, a.ndesc, a.rows, p.pm, p.a + t, wv, nwv);
+}
+
+template <int MODE> __device__ __forceinline__ void case_drag(const DragArgs &d, double *out, double *sh)
+{
+    drag_case<MODE>(d, out, sh);
+}
+
+template <int MODE, int VX> __global__ __launch_bounds__(CASE_TPB) void k_cases(CaseArgs a)
+{
+    extern __shared__ double lds[];
+    __shared__ double sh[16];
+    const int c = a.case_of_wg[blockIdx.x];
+    const int tid = (int)threadIdx.x;
+    // stream-collide tiles: CASE_TPB / tpb at once, one per group of tpb threads
+    const int ng = CASE_TPB / a.tpb, grp = tid / a.tpb;
+    const int wv = (tid - grp * a.tpb) >> 6, nwv = a.tpb >> 6;
+    for (int s = 0; s < a.nsteps; s++) {
+        const CaseItem *it = a.items + ((size_t)((a.step0 + s) % a.period) * a.ncases + c) * a.per_case;
+        for (int k = 0; k < a.per_case; k++) {
+            const CaseItem &p = it[k];
+            if (p.kind == ITEM_COLLIDE) {
+                case_collide<MODE, VX>(p, a, grp, ng, wv, nwv);
+            } else if (p.kind == ITEM_OPS) {
+                case_ops<MODE>(p, a.ops, a.descs);
+            } else if (p.kind == ITEM_SPLINE) {
+                case_splines<MODE>(p, a.ops, a.descs, lds);
+            } else { // ITEM_DRAG
+                double *out = a.hist + 2 * ((size_t)a.ncases * (size_t)((a.head0 + s) % a.cap) + (size_t)c);
+                case_drag<MODE>(a.drag[p.a][c], out, sh);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 #define MBL_LAUNCH_CHECK()                                                                                             \
@@ -1772,6 +1863,37 @@ static int spline_solve_r(hipStream_t s, int R, const LineOp *ops, int nops, siz
 }
 
 int launch_spline_path(int max_nc) { return max_nc <= SPL_FUSED_MAX_NC ? 0 : 1; }
+
+size_t cases_lds(int max_nc) { return 27 * sizeof(double) * (size_t)(max_nc > 0 ? max_nc : 0); }
+
+int cases_spline_ok(int max_nc) { return max_nc <= SPL_FUSED_MAX_NC && max_nc <= CASE_TPB; }
+
+template <int MODE> static int cases_vx(hipStream_t s, int vx, const CaseArgs &a, size_t lds, int grid)
+{
+    switch (vx) {
+    case 1: hipLaunchKernelGGL((k_cases<MODE, 1>), dim3(grid), dim3(CASE_TPB), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((k_cases<MODE, 2>), dim3(grid), dim3(CASE_TPB), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((k_cases<MODE, 4>), dim3(grid), dim3(CASE_TPB), lds, s, a); break;
+    case 8: hipLaunchKernelGGL((k_cases<MODE, 8>), dim3(grid), dim3(CASE_TPB), lds, s, a); break;
+    default: return MBL_E_UNSUPPORTED;
+    }
+    return 0;
+}
+
+int launch_cases(void *stream, int mode, int vx, const CaseArgs &a, size_t lds)
+{
+    if (a.ncases <= 0 || a.per_case <= 0 || a.period <= 0 || a.step0 < 0 || a.step0 >= a.period || a.nsteps < 0 ||
+        a.cap <= 0 || a.tpb < 64 ||
+        a.tpb % 64 || CASE_TPB % a.tpb || a.rows < a.tpb / 64 || lds > 64 * 1024)
+        return MBL_E_INVALID;
+    if (a.nsteps == 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = 0;
+    MBL_MODE_SWITCH(mode, rc = cases_vx<M_>(s, vx, a, lds, a.ncases));
+    if (rc) return rc;
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
 
 int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
                   const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms)

@@ -95,6 +95,7 @@ def load_library(fp64=False):
         "mbl_batch_create": (ip, [vp, ip, C.POINTER(mbl_case_spec), C.POINTER(vp)]),
         "mbl_batch_destroy": (None, [vp]),
         "mbl_batch_step": (ip, [vp, ip]),
+        "mbl_batch_set_persistent": (ip, [vp, ip]),
         "mbl_batch_node_updates": (C.c_double, [vp]),
         "mbl_batch_forces": (ip, [vp, ip, dp, dp]),
         "mbl_batch_residual": (ip, [vp, dp]),
@@ -512,9 +513,10 @@ def c4_specs(n=256, seed=2024):
 
 
 class Batch:
-    """Many independent two-level cases advanced by shared launches (mbl_batch_*)."""
+    """Many independent two-level cases advanced by shared launches (mbl_batch_*), or with
+    persistent=True by one launch per step() call, one workgroup per case."""
 
-    def __init__(self, lib, specs, storage=abi.STORE_NATIVE, use_graph=1):
+    def __init__(self, lib, specs, storage=abi.STORE_NATIVE, use_graph=1, persistent=False):
         self.lib = lib
         cfg = default_config(lib, LEVEL=2, number_line2=8, bc_mode=abi.BC_CHANNEL, init_mode=abi.INIT_REST,
                              storage=storage, use_graph=use_graph, write_files=0)
@@ -528,6 +530,11 @@ class Batch:
         b = C.c_void_p()
         check(lib, lib.mbl_batch_create(h, self.n, self.specs, C.byref(b)), "batch create")
         self.b = b
+        if persistent:
+            self.set_persistent(True)
+
+    def set_persistent(self, on):
+        check(self.lib, self.lib.mbl_batch_set_persistent(self.b, 1 if on else 0), "batch persistent")
 
     def step(self, n):
         check(self.lib, self.lib.mbl_batch_step(self.b, n), "batch step")

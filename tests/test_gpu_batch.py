@@ -69,15 +69,22 @@ def oracle_for(lib, sp):
     return oracle.OracleSim(g, cfg)
 
 
-def test_batch_matches_oracle_fp64(lib64):
-    """Four uneven cases in one batch (FP64 build) against the FP64 oracle, each case on its
-    own: every plane of every block to 1e-12 and the drag series to 1e-10 after 40 coarse
-    steps (the batch's refinement cycle is the reference's evolution.h:2-65 per case)."""
+def uneven_specs():
     specs = [mbl.case_spec(5, 0.2, 60.0), mbl.case_spec(8, 0.25, 150.0), mbl.case_spec(6, 0.12, 45.0),
              mbl.case_spec(4, 0.3, 100.0)]
     specs[1].CY += 0.25
+    return specs
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_matches_oracle_fp64(lib64, persistent):
+    """Four uneven cases in one batch (FP64 build) against the FP64 oracle, each case on its
+    own: every plane of every block to 1e-12 and the drag series to 1e-10 after 40 coarse
+    steps (the batch's refinement cycle is the reference's evolution.h:2-65 per case), for the
+    captured-graph path and for the one-launch path (one workgroup per case)."""
+    specs = uneven_specs()
     steps = 40
-    b = mbl.Batch(lib64, specs)
+    b = mbl.Batch(lib64, specs, persistent=persistent)
     b.step(steps)
     cd_b, cl_b = b.forces(steps)
     for c, sp in enumerate(specs):
@@ -95,6 +102,50 @@ def test_batch_matches_oracle_fp64(lib64):
         assert np.max(np.abs(cl_b[c] - o.series(4))) / scale < 1e-10
         o.close()
     b.close()
+
+
+def batch_state(b, ncases):
+    out = []
+    for c in range(ncases):
+        for q in range(5):
+            for fld in ["rho", "ux", "uy"] + [f"F{k}" for k in range(9)]:
+                out.append(b.read(c, q, fld))
+    return out
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_persistent_equals_graph(lib64, lib32, fp64):
+    """The one-launch path (mbl_batch_set_persistent: one workgroup per case runs its refinement
+    cycles in the reference order) against the captured-graph path on the same uneven cases:
+    every plane and macro field and every force, bit for bit, after step() calls of 1, 7, 5 and
+    13 coarse steps (launches that start at different steps of the phase period), then with the
+    two batches' paths swapped for 4 more steps."""
+    lib = lib64 if fp64 else lib32
+    specs = uneven_specs()
+    g = mbl.Batch(lib, specs)
+    p = mbl.Batch(lib, specs, persistent=True)
+    total = 0
+    for n in (1, 7, 5, 13):
+        g.step(n)
+        p.step(n)
+        total += n
+    for x, y in zip(batch_state(g, len(specs)), batch_state(p, len(specs))):
+        assert np.array_equal(x, y)
+    cg, lg = g.forces(total)
+    cp, lp = p.forces(total)
+    assert cg.shape == (len(specs), total)
+    assert np.array_equal(cg, cp) and np.array_equal(lg, lp)
+    g.set_persistent(True)
+    p.set_persistent(False)
+    g.step(4)
+    p.step(4)
+    for x, y in zip(batch_state(g, len(specs)), batch_state(p, len(specs))):
+        assert np.array_equal(x, y)
+    cg, lg = g.forces(total + 4)
+    cp, lp = p.forces(total + 4)
+    assert np.array_equal(cg, cp) and np.array_equal(lg, lp)
+    g.close()
+    p.close()
 
 
 def test_batch_residual_matches_error_l2(lib64):
@@ -116,16 +167,17 @@ def test_batch_residual_matches_error_l2(lib64):
         assert np.max(np.abs(res[:, c] - l2["residual"]) / np.abs(l2["residual"])) < 1e-12
 
 
-def test_batch_256_cases(lib32):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_256_cases(lib32, persistent):
     """The 256-case sweep (seeded Re ~ U[40, 200], D in {10 .. 40} finest cells, blockage
     ~ U[0.1, 0.3]) at reduced steps: every case's forces finite; its mass (mean density over
     its channel, each node weighted by its cell area 1/m^2, the overlaps of the blocks counted
     once per block) within 2 % of the initial 1 through the inlet/outlet exchange (measured
     1.0 %); no node's
     density more than 15 % from 1 (measured 7.1 %: the pressure waves of the start from rest);
-    and its residual finite."""
+    and its residual finite. Captured-graph and one-launch paths."""
     specs = mbl.c4_specs(256)
-    b = mbl.Batch(lib32, specs)
+    b = mbl.Batch(lib32, specs, persistent=persistent)
     b.step(300)
     cd, cl = b.forces(300)
     assert cd.shape == (256, 300) and np.all(np.isfinite(cd)) and np.all(np.isfinite(cl))
