@@ -391,6 +391,15 @@ int mbl_divergence_step(mbl_handle_t h);
  * Returns the total and the count per kind (pointers may be NULL), or
  * MBL_E_INVALID before the first evolution() call from that level. */
 int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int *interface_ops, int *splines);
+/* Positions of the same schedule: launches at one position have no dependence among them.
+ * No reference counterpart. */
+int mbl_cycle_positions(mbl_handle_t h, int level);
+/* on != 0: the launches of one position run concurrently, the first on the handle's stream and
+ * the others on side streams forked from it and joined before the next position (parallel
+ * branches of the captured graph). Default off: one after another on the handle's stream,
+ * measured faster on MI355X (the cross-stream joins cost more than the overlap saves). Drops
+ * the captured graphs. No reference counterpart. */
+int mbl_set_cycle_fork(mbl_handle_t h, int on);
 
 /* Velocity L2 distance from the analytic plane Poiseuille profile
  * u_a(y) = 4 U0[0] (LY - 1.5 - y)(y - 0.5)/(LY - 2)^2 (the profile left

@@ -57,6 +57,7 @@ struct CycleItem {
 // A merged launch of the scheduled cycle: items of one kind with no dependence among them.
 struct CycleLaunch {
     int kind;
+    int pos = 0;             // position in the schedule (launches at one position are independent)
     std::vector<int> items;  // indices into the recorded item sequence
     int lo = 0, hi = 0;      // collide: levels lo..hi (adjacent, same nodes per thread)
     int op_off = 0, op_count = 0, max_n = 0, max_nc = 0; // interface/spline: range of the program's op table
@@ -172,6 +173,11 @@ struct mbl_context {
     void *scratch = nullptr;  // device address of this device's allocation-free scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timing = false;
+    // side streams of a cycle's concurrent launches (several launches at one position of the
+    // schedule) and the fork/join events; created on first use, default priority
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+    bool fork = false; // launches at one position run concurrently (mbl_set_cycle_fork)
 };
 
 static mbl_context *g_current = nullptr;
@@ -360,6 +366,11 @@ extern "C" void mbl_handle_destroy(mbl_handle_t h)
     if (h->h_drag) hipHostFree(h->h_drag);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
+    for (int i = 0; i < 3; i++) {
+        if (h->side[i]) hipStreamDestroy(h->side[i]);
+        if (h->ev_join[i]) hipEventDestroy(h->ev_join[i]);
+    }
+    if (h->ev_fork) hipEventDestroy(h->ev_fork);
     if (h->own_stream) hipStreamDestroy(h->own_stream);
     if (g_current == h) g_current = nullptr;
     delete h;
@@ -1362,6 +1373,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         for (size_t c = 0; c < col.size();) {
             CycleLaunch cl;
             cl.kind = 0;
+            cl.pos = q;
             cl.lo = cl.hi = items[col[c]].level;
             cl.items.push_back(col[c]);
             size_t d = c + 1;
@@ -1395,6 +1407,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
                 ifl.push_back(CycleLaunch());
                 tgt = &ifl.back();
                 tgt->kind = 1;
+                tgt->pos = q;
             }
             tgt->items.push_back(j);
         }
@@ -1407,6 +1420,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             for (int o = items[j].off; o < items[j].off + items[j].count; o++) mx = std::max(mx, L->hops[o].nc);
             CycleLaunch &t = sp[launch_spline_path(mx)];
             t.kind = 2;
+            t.pos = q;
             t.items.push_back(j);
         }
         for (auto &t : sp)
@@ -1476,29 +1490,68 @@ static int launch_state(const Layout *L, const std::vector<CycleItem> &items, co
     return 0;
 }
 
+static int launch_one(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P,
+                      const CycleLaunch &cl, hipStream_t stream)
+{
+    const int mode = storage_mode(h->cfg);
+    uint32_t pm;
+    MarkerSet ms;
+    if (launch_state(L, items, cl, pm, ms)) return MBL_E_INVALID;
+    if (cl.kind == 0) {
+        int tiles = 0;
+        for (int l = cl.lo; l <= cl.hi; l++) tiles += L->tiles[l];
+        return launch_stream_collide(stream, mode, L->vx[cl.lo], h->tpb, L->periodic, true, true, L->db, L->dtile,
+                                     L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo]);
+    }
+    if (cl.kind == 1) {
+        double xm = 0.0;
+        for (int j : cl.items)
+            if (items[j].temporal) xm = items[j].xm;
+        return launch_line_ops(stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_n, L->db, pm, ms, xm);
+    }
+    return launch_spline(stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_nc, cl.max_n, L->db, pm, ms);
+}
+
+static int side_streams(mbl_context *h)
+{
+    if (h->ev_fork) return 0;
+    for (int i = 0; i < 3; i++) {
+        if (hipStreamCreateWithFlags(&h->side[i], hipStreamNonBlocking) != hipSuccess) return MBL_E_HIP;
+        if (hipEventCreateWithFlags(&h->ev_join[i], hipEventDisableTiming) != hipSuccess) return MBL_E_HIP;
+    }
+    if (hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess) return MBL_E_HIP;
+    return 0;
+}
+
+// The scheduled launches in order. Launches at one position of the schedule have no
+// dependence among them: the first runs on the handle's stream, the others on side streams
+// forked from it and joined back before the next position (in a captured graph: parallel
+// branches).
 static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P)
 {
     if ((int)items.size() != P.nitems) return MBL_E_INVALID;
-    const int mode = storage_mode(h->cfg);
-    for (const CycleLaunch &cl : P.launches) {
-        uint32_t pm;
-        MarkerSet ms;
-        if (launch_state(L, items, cl, pm, ms)) return MBL_E_INVALID;
-        int rc = 0;
-        if (cl.kind == 0) {
-            int tiles = 0;
-            for (int l = cl.lo; l <= cl.hi; l++) tiles += L->tiles[l];
-            rc = launch_stream_collide(h->stream, mode, L->vx[cl.lo], h->tpb, L->periodic, true, true, L->db, L->dtile,
-                                       L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo]);
-        } else if (cl.kind == 1) {
-            double xm = 0.0;
-            for (int j : cl.items)
-                if (items[j].temporal) xm = items[j].xm;
-            rc = launch_line_ops(h->stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_n, L->db, pm, ms, xm);
-        } else {
-            rc = launch_spline(h->stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_nc, cl.max_n, L->db, pm, ms);
+    const size_t nl = P.launches.size();
+    for (size_t i = 0; i < nl;) {
+        size_t j = i + 1;
+        while (j < nl && P.launches[j].pos == P.launches[i].pos) j++;
+        const int k = h->fork ? (int)(j - i) : 1;
+        if (k > 1) {
+            if (int rc = side_streams(h)) return rc;
+            if (hipEventRecord(h->ev_fork, h->stream) != hipSuccess) return MBL_E_HIP;
+            for (int s = 0; s < std::min(k - 1, 3); s++)
+                if (hipStreamWaitEvent(h->side[s], h->ev_fork, 0) != hipSuccess) return MBL_E_HIP;
         }
-        if (rc) return rc;
+        for (size_t m = i; m < j; m++) {
+            const int r = (int)(m - i);
+            hipStream_t st = (k > 1 && r > 0) ? h->side[(r - 1) % 3] : h->stream;
+            if (int rc = launch_one(h, L, items, P, P.launches[m], st)) return rc;
+        }
+        if (k > 1)
+            for (int s = 0; s < std::min(k - 1, 3); s++) {
+                if (hipEventRecord(h->ev_join[s], h->side[s]) != hipSuccess) return MBL_E_HIP;
+                if (hipStreamWaitEvent(h->stream, h->ev_join[s], 0) != hipSuccess) return MBL_E_HIP;
+            }
+        i = j;
     }
     return 0;
 }
@@ -2191,6 +2244,28 @@ extern "C" int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int 
     if (interface_ops) *interface_ops = c[1];
     if (splines) *splines = c[2];
     return c[0] + c[1] + c[2];
+}
+
+extern "C" int mbl_cycle_positions(mbl_handle_t h, int level)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!h->layout || level < 0 || level >= MBL_MAX_LEVEL || !h->layout->prog[level].built) return MBL_E_INVALID;
+    const auto &v = h->layout->prog[level].launches;
+    return v.empty() ? 0 : v.back().pos + 1;
+}
+
+extern "C" int mbl_set_cycle_fork(mbl_handle_t h, int on)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->fork = on != 0;
+    if (h->layout) { // captured graphs hold the other launch order
+        for (auto &g : h->layout->graphs) hipGraphExecDestroy(g.second);
+        h->layout->graphs.clear();
+    }
+    return 0;
 }
 
 extern "C" int mbl_divergence_step(mbl_handle_t h)

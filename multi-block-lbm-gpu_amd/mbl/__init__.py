@@ -85,6 +85,8 @@ def load_library(fp64=False):
         "mbl_diagnostics_clear": (None, [vp]),
         "mbl_divergence_step": (ip, [vp]),
         "mbl_cycle_launches": (ip, [vp, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]),
+        "mbl_cycle_positions": (ip, [vp, ip]),
+        "mbl_set_cycle_fork": (ip, [vp, ip]),
         "mbl_error_L2_noalloc": (ip, [vp, ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(vp), C.POINTER(vp), vp]),
         "mbl_step_block": (ip, [vp, C.POINTER(AREA), C.POINTER(D2Q9), ip]),
         "mbl_timer_begin": (ip, [vp]),

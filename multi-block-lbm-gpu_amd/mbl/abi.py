@@ -118,7 +118,7 @@ EXPORTS = [
     "mbl_block_free", "mbl_read_distribution", "mbl_write_distribution", "mbl_geo_init",
     "mbl_geometry_preset", "mbl_geometry_channel", "init_General", "init_Fstore", "init_tri_A",
     "evolution", "sum_N_drag", "fdrag_store", "getDrag_block", "error_L2", "mbl_last_diagnostics",
-    "mbl_diagnostics_series", "mbl_diagnostics_clear", "mbl_divergence_step", "mbl_error_L2_noalloc", "mbl_cycle_launches", "mbl_step_block", "mbl_timer_begin",
+    "mbl_diagnostics_series", "mbl_diagnostics_clear", "mbl_divergence_step", "mbl_error_L2_noalloc", "mbl_cycle_launches", "mbl_cycle_positions", "mbl_set_cycle_fork", "mbl_step_block", "mbl_timer_begin",
     "mbl_timer_end", "mbl_set_tuning", "mbl_poiseuille_l2", "mbl_set_persistent",
     "mbl_batch_create", "mbl_batch_destroy", "mbl_batch_step", "mbl_batch_set_persistent", "mbl_batch_node_updates", "mbl_batch_forces", "mbl_batch_residual",
     "mbl_batch_autotune", "mbl_batch_block", "mbl_batch_raster", "mbl_batch_init_from_raster",

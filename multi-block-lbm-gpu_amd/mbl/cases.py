@@ -30,10 +30,13 @@ def node_updates_per_step(sim):
     return sum(sim.areas[a].nx * sim.areas[a].ny * sim.areas[a].m for a in range(sim.na))
 
 
-def run(lib, geom, steps, storage=abi.STORE_NATIVE, chunk=1000, D=None, U=0.1, progress=False, tuning=None, **kw):
+def run(lib, geom, steps, storage=abi.STORE_NATIVE, chunk=1000, D=None, U=0.1, progress=False, tuning=None, fork=False,
+        **kw):
     sim = MultiBlockSim(lib, geom, storage=storage, flush_interval=min(chunk, 4096), **kw)
     if tuning:
         check(lib, lib.mbl_set_tuning(sim.h, tuning[0], tuning[1]), "tuning")
+    if fork:
+        check(lib, lib.mbl_set_cycle_fork(sim.h, 1), "fork")
     t0 = time.perf_counter()
     done = 0
     while done < steps:
@@ -74,6 +77,8 @@ def main(argv=None):
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--vx", type=int, default=0, help="nodes per thread of the fused kernel (0: library default)")
     ap.add_argument("--tpb", type=int, default=0, help="threads per workgroup (0: library default)")
+    ap.add_argument("--fork", action="store_true",
+                    help="launches at one position of the scheduled cycle on forked side streams")
     args = ap.parse_args(argv)
     lib = load_library(args.fp64)
     if args.case == "c1":
@@ -81,7 +86,8 @@ def main(argv=None):
     else:
         geom, steps, D = c2b_geometry(lib), args.steps or 50000, 16.0
     out, series = run(lib, geom, steps, storage=STORAGE[args.storage] if not args.fp64 else abi.STORE_NATIVE,
-                      D=D, progress=args.progress, tuning=(args.vx, args.tpb) if (args.vx or args.tpb) else None)
+                      D=D, progress=args.progress, tuning=(args.vx, args.tpb) if (args.vx or args.tpb) else None,
+                      fork=args.fork)
     out.update({"case": args.case, "precision": "f64" if args.fp64 else args.storage})
     if args.save:
         np.savez(args.save, **series)

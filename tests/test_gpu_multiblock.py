@@ -180,6 +180,32 @@ def test_graph_replay_bitwise_equal(lib32):
     b.close()
 
 
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_forked_positions_bitwise_equal(lib32, use_graph):
+    """Launches at one position of the scheduled cycle run on forked side streams (parallel
+    graph branches); the three-level layout has such positions. Against the same cycle run
+    one launch after another: the same bits after 30 coarse steps, with fewer launches in
+    sequence than launches."""
+    geom = tiny3(lib32)
+    runs = []
+    for fork in (1, 0):
+        sim = mbl.MultiBlockSim(lib32, geom, use_graph=use_graph, U0=(0.1, 0.0), init_mode=abi.INIT_PROFILE)
+        mbl.check(lib32, lib32.mbl_set_cycle_fork(sim.h, fork), "fork")
+        sim.step(30)
+        if fork:
+            total = lib32.mbl_cycle_launches(sim.h, 0, None, None, None)
+            depth = lib32.mbl_cycle_positions(sim.h, 0)
+            assert 0 < depth < total, (depth, total)
+        runs.append(([sim.read(a, ("F", k)) for a in range(sim.na) for k in range(9)],
+                     [sim.read(a, f) for a in range(sim.na) for f in ("rho", "ux", "uy")], sim.series()))
+        sim.close()
+    a, b = runs
+    for x, y in zip(a[0] + a[1], b[0] + b[1]):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a[2][0]["residual"], b[2][0]["residual"])
+    assert np.array_equal(a[2][1]["Cd"], b[2][1]["Cd"])
+
+
 @pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F16D])
 def test_launch_shapes_bitwise_equal(lib32, storage):
     """Every launch shape (nodes per thread, workgroup size) gives the same bits.
