@@ -350,6 +350,29 @@ void error_L2(Lbi t, const Lbi N_areas, mbl_dim3 block, mbl_dim3 *grid, AREA *ar
 
 /* ---- additions (no counterpart in the reference) ---- */
 
+/* `nsteps` iterations of the reference driver's loop body (Multi-block-LBM.cu:144-158):
+ *     evolution(t, ..., 0, ...); error_L2(t, asp[LEVEL], ..., upsx, upsy, ...);
+ *     getDrag_block(t, d_pdrag, areas[last], pps_d[last], force, N_drag);
+ * for t = n, ..., n + nsteps - 1, with last = asp[LEVEL] - 1 (the reference's
+ * number_area - 1) and no drag when N_drag is 0. Same state, records, files and
+ * force output as those calls; the residual's sums may differ from error_L2's in
+ * the last bits (another summation order). Where the layout allows it (not
+ * periodic, one node per thread, no interface op rewrites the macros of a fluid,
+ * inlet or outlet node off a block edge), the last stream-collide of each level also sums
+ * error_L2's terms of the nodes it updates (u_prev <- u), and one launch after
+ * the cycle takes the other nodes, the drag and the final sums: two launches
+ * fewer per step than the three calls, and no pass over the nodes for the
+ * residual. Otherwise it makes the three calls. Returns the handle's error. */
+int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lccp,
+                       mbl_dim3 block, mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d,
+                       Inf_Ff_store *line_Inf_c, Inf_Ff_store *line_Inf_f, Inf_Ff_store *line_Inf_cc,
+                       Ff_Store *line_t3, FM_spline3 *line_S3, Lbd **upsx, Lbd **upsy, Int3 *d_pdrag,
+                       Lbi N_drag, double *force);
+/* 1 when mbl_evolution_step takes the fused path on the handle's current layout, 0 when it
+ * makes the three calls; MBL_E_INVALID before the first multi-block step. No reference
+ * counterpart. */
+int mbl_evolution_step_fused(mbl_handle_t h);
+
 /* Allocation-free variant of error_L2. No counterpart in the reference; it is
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). It computes

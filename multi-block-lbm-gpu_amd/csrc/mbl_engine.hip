@@ -60,6 +60,7 @@ struct CycleLaunch {
     int pos = 0;             // position in the schedule (launches at one position are independent)
     std::vector<int> items;  // indices into the recorded item sequence
     int lo = 0, hi = 0;      // collide: levels lo..hi (adjacent, same nodes per thread)
+    uint32_t rmask = 0;      // collide: levels whose last collide of the cycle this launch holds
     int op_off = 0, op_count = 0, max_n = 0, max_nc = 0; // interface/spline: range of the program's op table
 };
 
@@ -91,6 +92,15 @@ struct Layout {
     int *didx = nullptr;
     uint8_t *dmask = nullptr;
     double *dM64 = nullptr;
+    double *dpcr = nullptr; // PCR factor tables of the spline lines (one per distinct matrix)
+    // fused coarse step (mbl_evolution_step): residual sums from the last collide of each level
+    // and one diagnostics launch (the nodes no collide writes the macros of, the drag)
+    bool step_ok = false;   // non-periodic, one node per thread, macros of collide nodes not rewritten
+    int total_tiles = 0;    // stream-collide tiles of all levels
+    int2 *dlist = nullptr;  // (block, node) of the nodes whose macros no collide writes
+    int nlist = 0;
+    double *dres = nullptr; // 3 (total_tiles + list workgroups) partial sums
+    std::vector<Lbd *> up_x, up_y; // u_prev arrays installed in the descriptors
     Batch bT[MBL_MAX_LEVEL], bCC[MBL_MAX_LEVEL], bS[MBL_MAX_LEVEL], bCF[MBL_MAX_LEVEL];
     std::vector<std::vector<int>> batch_lines_T, batch_lines_S, batch_lines_CF; // line ids per level
     std::map<uint64_t, hipGraphExec_t> graphs;
@@ -110,6 +120,9 @@ struct Layout {
         if (didx) hipFree(didx);
         if (dmask) hipFree(dmask);
         if (dM64) hipFree(dM64);
+        if (dpcr) hipFree(dpcr);
+        if (dlist) hipFree(dlist);
+        if (dres) hipFree(dres);
         if (dtile) hipFree(dtile);
         dtile = nullptr;
         db = nullptr;
@@ -117,6 +130,9 @@ struct Layout {
         didx = nullptr;
         dmask = nullptr;
         dM64 = nullptr;
+        dpcr = nullptr;
+        dlist = nullptr;
+        dres = nullptr;
     }
 };
 
@@ -703,6 +719,54 @@ static void cc_nodes(const Inf_Ff_store &L, int ip, int &i1, int &j1, int &i2, i
 }
 
 // ---------------------------------------------------------------------------
+// PCR elimination factors of a spline matrix. Level q pairs row i with rows i -/+ 2^q:
+// alpha = -a_i / b_{i-s}, gamma = -c_i / b_{i+s}, and the row's new couplings and diagonal
+// follow from them; the right-hand sides are then updated on the device with the stored
+// factors. The multiply-adds are spelled as fused operations, as the device forms them.
+// ---------------------------------------------------------------------------
+namespace mbl {
+int pcr_factor_table(const double *l, const double *m, const double *u, int N, std::vector<double> &tab)
+{
+    std::vector<double> a(l, l + N), b(m, m + N), c(u, u + N), na(N), nb(N), nc(N);
+    std::vector<double> lev;
+    int levels = 0;
+    for (int s = 1; s < N; s <<= 1) {
+        const size_t base = lev.size();
+        lev.resize(base + 2 * (size_t)N, 0.0);
+        bool decoupled = true;
+        for (int i = 0; i < N; i++) {
+            const int im = i - s, iq = i + s;
+            double xa = 0.0, xc = 0.0, xb = b[i];
+            if (im >= 0) {
+                const double alpha = -a[i] / b[im];
+                lev[base + i] = alpha;
+                xa = alpha * a[im];
+                xb = std::fma(alpha, c[im], xb);
+            }
+            if (iq < N) {
+                const double gamma = -c[i] / b[iq];
+                lev[base + N + i] = gamma;
+                xc = gamma * c[iq];
+                xb = std::fma(gamma, a[iq], xb);
+            }
+            na[i] = xa;
+            nb[i] = xb;
+            nc[i] = xc;
+            decoupled = decoupled && std::fabs(xa) <= 0x1p-60 * std::fabs(xb) && std::fabs(xc) <= 0x1p-60 * std::fabs(xb);
+        }
+        a.swap(na);
+        b.swap(nb);
+        c.swap(nc);
+        levels++;
+        if (decoupled) break;
+    }
+    tab = lev;
+    tab.insert(tab.end(), b.begin(), b.end());
+    return levels;
+}
+} // namespace mbl
+
+// ---------------------------------------------------------------------------
 // Hazard scheduler for one batch of interface ops issued in reference order.
 // Units are (block, node, field) with field 0 = rho/u, 1 = F. A write that is
 // overwritten later in the batch before any read is dead and its entry bit is
@@ -1140,6 +1204,41 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
             }
         if (!add_batch(std::move(ops), L->bCF[l])) return fail("coarse/fine transfer line outside its blocks");
     }
+    // PCR factor tables of the spline matrices, read once from the caller's l, m, u (init_tri_A
+    // writes them before the first evolution(); the spline reads them, never writes them)
+    std::vector<double> pcr_all;
+    std::vector<std::pair<size_t, int>> pcr_of(all.size(), {0, 0}); // per op: offset, levels
+    {
+        std::map<std::vector<double>, std::pair<size_t, int>> seen;
+        for (size_t j = 0; j < all.size(); j++) {
+            const LineOp &op = all[j].op;
+            if (op.type != OP_SPLINE) continue;
+            const int N = op.nc;
+            std::vector<Lbd> hl(N), hm(N), hu(N);
+            if (hipMemcpy(hl.data(), op.tl, sizeof(Lbd) * N, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(hm.data(), op.tm, sizeof(Lbd) * N, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(hu.data(), op.tu, sizeof(Lbd) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+                L->release();
+                delete L;
+                return set_err(h, MBL_E_HIP);
+            }
+            std::vector<double> key(3 * (size_t)N);
+            for (int i = 0; i < N; i++) {
+                key[i] = (double)hl[i];
+                key[N + i] = (double)hm[i];
+                key[2 * N + i] = (double)hu[i];
+            }
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                std::vector<double> tab;
+                const int lv = pcr_factor_table(key.data(), key.data() + N, key.data() + 2 * N, N, tab);
+                if (lv > PCR_MAX_LEVELS && launch_spline_path(N) == 0) return fail("spline matrix does not decouple");
+                it = seen.emplace(key, std::make_pair(pcr_all.size(), lv)).first;
+                pcr_all.insert(pcr_all.end(), tab.begin(), tab.end());
+            }
+            pcr_of[j] = it->second;
+        }
+    }
     // upload tables
     size_t nidx = 0, nmask = 0, nM = 0;
     for (auto &o : all) {
@@ -1152,7 +1251,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         hipMalloc(&L->didx, sizeof(int) * std::max<size_t>(1, nidx)) != hipSuccess ||
         hipMalloc(&L->dmask, std::max<size_t>(1, nmask)) != hipSuccess ||
         hipMalloc(&L->dM64, sizeof(double) * std::max<size_t>(1, nM)) != hipSuccess ||
-        hipMalloc(&L->dtile, sizeof(int) * std::max<size_t>(1, tile_blk.size())) != hipSuccess) {
+        hipMalloc(&L->dtile, sizeof(int) * std::max<size_t>(1, tile_blk.size())) != hipSuccess ||
+        hipMalloc(&L->dpcr, sizeof(double) * std::max<size_t>(1, pcr_all.size())) != hipSuccess) {
         L->release();
         delete L;
         return set_err(h, MBL_E_NOMEM);
@@ -1162,7 +1262,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     std::vector<uint8_t> hmask;
     hmask.reserve(nmask);
     size_t mo = 0;
-    for (auto &o : all) {
+    for (size_t j = 0; j < all.size(); j++) {
+        const OpBuild &o = all[j];
         LineOp op = o.op;
         op.src_idx = o.src.empty() ? nullptr : L->didx + hidx.size();
         hidx.insert(hidx.end(), o.src.begin(), o.src.end());
@@ -1173,6 +1274,8 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         if (op.type == OP_SPLINE) {
             op.M64 = L->dM64 + mo;
             mo += 12 * (size_t)op.nc;
+            op.pcr = L->dpcr + pcr_of[j].first;
+            op.pcr_levels = pcr_of[j].second;
         }
         L->hops.push_back(op);
     }
@@ -1184,11 +1287,48 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     if (ok && !hidx.empty())
         ok = hipMemcpy(L->didx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !hmask.empty()) ok = hipMemcpy(L->dmask, hmask.data(), hmask.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !pcr_all.empty())
+        ok = hipMemcpy(L->dpcr, pcr_all.data(), sizeof(double) * pcr_all.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         L->release();
         delete L;
         return set_err(h, MBL_E_HIP);
     }
+    // The fused coarse step: the last collide of each level sums the residual terms of the
+    // nodes off the block edges it writes the macros of (fluid, inlet, outlet), which no
+    // interface op may rewrite; the other nodes (block edges, where same-level copies end on
+    // the channel's inlet and outlet; solid and ghost nodes) are summed after the cycle, from
+    // a list.
+    L->total_tiles = (int)tile_blk.size();
+    bool step_ok = !L->periodic;
+    for (int l = 0; l < cfg.LEVEL; l++) step_ok = step_ok && L->vx[l] == 1;
+    auto summed_by_collide = [&](int a, size_t node) {
+        const int f = hflags[a][node];
+        const int nx = A.areas[a].nx, ny = A.areas[a].ny, i = (int)(node % nx), j = (int)(node / nx);
+        const bool edge = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+        return !edge && (f == FLUID || f == BOUNDARY_INLET || f == BOUNDARY_OUTLET);
+    };
+    for (const OpBuild &o : all)
+        for (size_t e = 0; e < o.mask.size() && step_ok; e++)
+            if ((o.mask[e] & 1) && summed_by_collide(o.op.dst_block, (size_t)o.dst[e])) step_ok = false;
+    if (step_ok) {
+        std::vector<int2> list;
+        for (int a = 0; a < nblocks; a++)
+            for (size_t i = 0; i < hflags[a].size(); i++)
+                if (!summed_by_collide(a, i)) list.push_back(make_int2(a, (int)i));
+        L->nlist = (int)list.size();
+        const int wgs = std::max(1, (L->nlist + 255) / 256);
+        ok = hipMalloc(&L->dlist, sizeof(int2) * std::max<size_t>(1, list.size())) == hipSuccess &&
+             hipMalloc(&L->dres, sizeof(double) * 3 * ((size_t)L->total_tiles + wgs)) == hipSuccess &&
+             (list.empty() ||
+              hipMemcpy(L->dlist, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice) == hipSuccess);
+        if (!ok) {
+            L->release();
+            delete L;
+            return set_err(h, MBL_E_NOMEM);
+        }
+    }
+    L->step_ok = step_ok;
     out = L;
     return 0;
 }
@@ -1426,6 +1566,16 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         for (auto &t : sp)
             if (!t.items.empty()) P.launches.push_back(t);
     }
+    // the launch that holds each level's last collide of the cycle (its tiles' state and macros
+    // are final for the step: the fused step's residual is summed there)
+    for (int l = 0; l < L->LEVEL; l++) {
+        int lastj = -1;
+        for (int j = 0; j < n; j++)
+            if (items[j].kind == 0 && items[j].level == l) lastj = j;
+        for (auto &cl : P.launches)
+            if (cl.kind == 0 && std::find(cl.items.begin(), cl.items.end(), lastj) != cl.items.end())
+                cl.rmask |= 1u << l;
+    }
     // the program's op table: each interface/spline launch's ops contiguous
     std::vector<LineOp> tab;
     for (auto &cl : P.launches) {
@@ -1490,8 +1640,9 @@ static int launch_state(const Layout *L, const std::vector<CycleItem> &items, co
     return 0;
 }
 
+// fused: the collide launches add the residual sums of the levels whose last collide they hold.
 static int launch_one(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P,
-                      const CycleLaunch &cl, hipStream_t stream)
+                      const CycleLaunch &cl, hipStream_t stream, bool fused)
 {
     const int mode = storage_mode(h->cfg);
     uint32_t pm;
@@ -1500,8 +1651,10 @@ static int launch_one(mbl_context *h, Layout *L, const std::vector<CycleItem> &i
     if (cl.kind == 0) {
         int tiles = 0;
         for (int l = cl.lo; l <= cl.hi; l++) tiles += L->tiles[l];
+        const uint32_t rmask = fused ? cl.rmask : 0u;
         return launch_stream_collide(stream, mode, L->vx[cl.lo], h->tpb, L->periodic, true, true, L->db, L->dtile,
-                                     L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo]);
+                                     L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo], rmask,
+                                     rmask ? L->dres : nullptr);
     }
     if (cl.kind == 1) {
         double xm = 0.0;
@@ -1527,7 +1680,8 @@ static int side_streams(mbl_context *h)
 // dependence among them: the first runs on the handle's stream, the others on side streams
 // forked from it and joined back before the next position (in a captured graph: parallel
 // branches).
-static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P)
+static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P,
+                     bool fused)
 {
     if ((int)items.size() != P.nitems) return MBL_E_INVALID;
     const size_t nl = P.launches.size();
@@ -1544,7 +1698,7 @@ static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &it
         for (size_t m = i; m < j; m++) {
             const int r = (int)(m - i);
             hipStream_t st = (k > 1 && r > 0) ? h->side[(r - 1) % 3] : h->stream;
-            if (int rc = launch_one(h, L, items, P, P.launches[m], st)) return rc;
+            if (int rc = launch_one(h, L, items, P, P.launches[m], st, fused)) return rc;
         }
         if (k > 1)
             for (int s = 0; s < std::min(k - 1, 3); s++) {
@@ -1556,7 +1710,8 @@ static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &it
     return 0;
 }
 
-static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2);
+static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2,
+                           bool fused = false);
 
 extern "C" void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lccp, Lbi level, mbl_dim3 block,
                           mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d, Inf_Ff_store *line_Inf_c,
@@ -1635,7 +1790,10 @@ static int phase_key(const Layout *L, const D2Q9 *pps_d, const Ff_Store *line_t3
     return 0;
 }
 
-static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2)
+// fused (level 0 of a layout with step_ok): the collides also write the residual sums
+// (mbl_evolution_step); its graphs are kept apart from the plain cycle's.
+static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2,
+                           bool fused)
 {
     const mbl_config &cfg = h->cfg;
     D2Q9 *pps_d = A.pps;
@@ -1659,13 +1817,14 @@ static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblo
     if (cs.rc) return fail(cs.rc);
     CycleProgram &P = L->prog[level];
     if (!P.built && schedule_cycle(h, L, cs.items, P)) return fail(h->last_error ? h->last_error : MBL_E_HIP);
+    if (fused) key |= 1ull << 57;
     if (cfg.use_graph) {
         auto it = L->graphs.find(key);
         if (it == L->graphs.end()) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
             if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(MBL_E_HIP);
-            int rc = run_cycle(h, L, cs.items, P);
+            int rc = run_cycle(h, L, cs.items, P, fused);
             hipError_t e = hipStreamEndCapture(h->stream, &g);
             if (rc || e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
                 if (g) hipGraphDestroy(g);
@@ -1678,7 +1837,7 @@ static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblo
             if (hipGraphLaunch(it->second, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
         }
     } else {
-        int rc = run_cycle(h, L, cs.items, P);
+        int rc = run_cycle(h, L, cs.items, P, fused);
         if (rc) return fail(rc);
     }
     // hand the new state back: F <-> f for every level that took an odd
@@ -2191,6 +2350,104 @@ extern "C" void getDrag_block(Lbi t, Int3 *d_pdrag, AREA area, D2Q9 d_pp, double
         force[0] = h->last[1];
         force[1] = h->last[2];
     }
+}
+
+// One or more iterations of the reference driver's loop (Multi-block-LBM.cu:144-158):
+// evolution -> error_L2 -> getDrag_block, fused where the layout allows (the residual sums
+// come from the last collide of each level, the rest in one diagnostics launch), otherwise
+// the three calls one after another.
+extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp,
+                                  Lbi *lccp, mbl_dim3 block, mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d,
+                                  Inf_Ff_store *line_Inf_c, Inf_Ff_store *line_Inf_f, Inf_Ff_store *line_Inf_cc,
+                                  Ff_Store *line_t3, FM_spline3 *line_S3, Lbd **upsx, Lbd **upsy, Int3 *d_pdrag,
+                                  Lbi N_drag, double *force)
+{
+    mbl_context *h = handle ? handle : g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (h->last_error) return h->last_error;
+    const mbl_config &cfg = h->cfg;
+    if (nsteps < 0 || !asp || !ls3cffcp || !ltcffcp || !lccp || !areas || !pps_d || !upsx || !upsy ||
+        N_drag < 0 || (N_drag > 0 && !d_pdrag))
+        return set_err(h, MBL_E_INVALID);
+    const int nblocks = asp[cfg.LEVEL];
+    const int nline = ltcffcp[cfg.LEVEL];
+    const int nline2 = lccp[cfg.LEVEL];
+    if (nblocks <= 0 || nline != ls3cffcp[cfg.LEVEL] || (nline > 0 && (!line_Inf_c || !line_Inf_f || !line_t3 || !line_S3)) ||
+        (nline2 > 0 && !line_Inf_cc))
+        return set_err(h, MBL_E_INVALID);
+    for (int a = 0; a < nblocks; a++)
+        if (!upsx[a] || !upsy[a]) return set_err(h, MBL_E_INVALID);
+    mbl_context *prev = g_current;
+    g_current = h;
+    const int last = nblocks - 1;
+    EvoArgs A{asp, ls3cffcp, ltcffcp, lccp, areas, pps_d, line_Inf_c, line_Inf_f, line_Inf_cc, line_t3, line_S3};
+    for (Lbi s = 0; s < nsteps && !h->last_error; s++) {
+        const Lbi t = n + s;
+        Layout *L = layout_for(h, A, nblocks, nline, nline2);
+        if (!L) break;
+        if (!L->step_ok) { // the three calls
+            evolution(t, asp, ls3cffcp, ltcffcp, lccp, 0, block, grid, areas, pps_h, pps_d, line_Inf_c, line_Inf_f,
+                      line_Inf_cc, line_t3, line_S3, h);
+            error_L2(t, nblocks, block, grid, areas, pps_d, upsx, upsy, nullptr, nullptr, nullptr, nullptr);
+            if (N_drag > 0) getDrag_block(t, d_pdrag, areas[last], pps_d[last], force, N_drag);
+            continue;
+        }
+        // the caller's u_prev arrays in the descriptors the collides read
+        if (L->up_x.size() != (size_t)nblocks || !std::equal(upsx, upsx + nblocks, L->up_x.begin()) ||
+            !std::equal(upsy, upsy + nblocks, L->up_y.begin())) {
+            L->up_x.assign(upsx, upsx + nblocks);
+            L->up_y.assign(upsy, upsy + nblocks);
+            for (int a = 0; a < nblocks; a++) {
+                L->hb[a].upx = upsx[a];
+                L->hb[a].upy = upsy[a];
+            }
+            if (hipMemcpyAsync(L->db, L->hb.data(), sizeof(BlockDesc) * nblocks, hipMemcpyHostToDevice, h->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) {
+                set_err(h, MBL_E_HIP);
+                break;
+            }
+        }
+        evolution_impl(h, A, 0, nblocks, nline, nline2, true);
+        if (h->last_error) break;
+        StepDiagArgs d;
+        memset(&d, 0, sizeof(d));
+        d.descs = L->db;
+        d.list = L->dlist;
+        d.nlist = L->nlist;
+        d.list_wgs = std::max(1, (L->nlist + 255) / 256);
+        d.ntiles = L->total_tiles;
+        d.rparts = L->dres;
+        d.out = h->d_l2 + L2_REC * h->pend_l2.size();
+        if (N_drag > 0) {
+            for (int k = 0; k < 9; k++) d.drag.Fold[k] = (void *)pps_d[last].f[k];
+            d.drag.links = d_pdrag;
+            d.drag.n = N_drag;
+            d.drag.nx = areas[last].nx;
+            d.drag.out = h->d_drag + 2 * h->pend_drag.size();
+        }
+        if (int rc = launch_step_diag(h->stream, storage_mode(cfg), d)) {
+            set_err(h, rc);
+            break;
+        }
+        h->pend_l2.push_back(t);
+        if (N_drag > 0) h->pend_drag.push_back(PendingDrag{t, areas[last].m});
+        if (cfg.sync_diag || (int)h->pend_l2.size() >= h->cap || (int)h->pend_drag.size() >= h->cap)
+            mbl_flush_diagnostics(h);
+        if (force && N_drag > 0) {
+            force[0] = h->last[1];
+            force[1] = h->last[2];
+        }
+    }
+    g_current = prev;
+    return h->last_error;
+}
+
+extern "C" int mbl_evolution_step_fused(mbl_handle_t h)
+{
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!h->layout) return MBL_E_INVALID;
+    return h->layout->step_ok ? 1 : 0;
 }
 
 extern "C" int mbl_last_diagnostics(mbl_handle_t h, double out[5])

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/mbl.h"
 
 #ifdef MBL_FP64
@@ -90,7 +92,23 @@ struct LineOp {
     Lbd *varibs;
     Lbd *gcol[12];
     double *M64;
+    // SPLINE: the elimination factors of parallel cyclic reduction on tridiag(tl, tm, tu),
+    // which depend on the matrix only (built once per layout from the caller's l, m, u):
+    // level q's factors of row i at pcr[(2q) nc + i] (alpha, the row i - 2^q) and
+    // pcr[(2q + 1) nc + i] (gamma, the row i + 2^q), then the reduced diagonal at
+    // pcr[2 pcr_levels nc + i]. pcr_levels: the levels after which every row is decoupled.
+    const double *pcr;
+    int pcr_levels;
 };
+
+// Most PCR levels the one-workgroup spline keeps in registers (rows decouple after 6 levels for
+// the spline matrix of any length; 10 levels solve any system of up to 1024 rows exactly).
+constexpr int PCR_MAX_LEVELS = 10;
+
+// The PCR factor table of tridiag(l, m, u) of N rows (layout of LineOp::pcr): the levels are run
+// until every row's couplings are below 2^-60 of its diagonal or the system is solved.
+// Returns the number of levels; `tab` receives (2 levels + 1) N values.
+int pcr_factor_table(const double *l, const double *m, const double *u, int N, std::vector<double> &tab);
 
 struct DragArgs {
     void *Fold[MBL_Q]; // planes holding the state one substep older (the caller's f)
@@ -98,6 +116,19 @@ struct DragArgs {
     int n;
     int nx;
     double *out;       // [0] Fx, [1] Fy
+};
+
+// The diagnostics launches of a fused coarse step (launch_step_diag: the nodes the collides do
+// not sum and the drag, then the final sums).
+struct StepDiagArgs {
+    const BlockDesc *descs;
+    const int2 *list;  // (block, node) of every node the stream-collides do not sum
+    int nlist;
+    int list_wgs;      // workgroups over the list (256 entries each, at least one)
+    int ntiles;        // stream-collide tiles of all levels: their sums at rparts[0, 3 ntiles)
+    double *rparts;    // 3 (ntiles + list_wgs) partial sums
+    double *out;       // error_L2 record: residual, s1, s2, smallest density
+    DragArgs drag;     // drag.n == 0: no drag
 };
 
 // One block of a batched case for the raster resampler and the raster
@@ -165,9 +196,13 @@ int launch_cases(void *stream, int mode, int vx, const CaseArgs &a, size_t lds);
 // d_tile_blk[t] = descriptor of tile t, or null to search tile_begin.
 // tile_base: the launch covers tiles [tile_base, tile_base + total_tiles) of the numbering of
 // the descriptors' tile_begin (and of d_tile_blk).
+// rmask != 0 (flagged, macros, one node per thread): the tiles of the levels in rmask also add
+// error_L2's terms of their fluid, inlet and outlet nodes off the block edges (u_prev <- u) and
+// write the tile's weighted sums to rparts[3 t .. 3 t + 2], t the tile's number across all levels.
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
-                          uint32_t parity_mask, int groups, int tile_base);
+                          uint32_t parity_mask, int groups, int tile_base, uint32_t rmask = 0,
+                          double *rparts = nullptr);
 int launch_init_general(void *stream, int mode, const BlockDesc &d, const AREA &area, Lbd *x, Lbd *y,
                         Lbd *forcex, Lbd *forcey, int bc_mode, int init_mode, unsigned long long seed,
                         double sigma, double LX, double LY, double R, double CX, double CY, const Consts &c);
@@ -189,6 +224,7 @@ int launch_residual(void *stream, const BlockDesc *d_descs, const int *d_tblk, i
 int launch_residual_cases(void *stream, const BlockDesc *d_descs, const int *d_tblk, int ndesc, int total_rtiles,
                           double *partials, int ncases, const int *d_case_tiles, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
+int launch_step_diag(void *stream, int mode, const StepDiagArgs &a);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
 int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes);
 // 9 planes of plane_bytes (consecutive in src and in dst) copied plane by plane

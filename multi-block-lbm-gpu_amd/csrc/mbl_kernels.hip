@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "mbl_internal.h"
 
 namespace mbl {
@@ -380,10 +383,28 @@ __device__ inline void fluidlike(const BlockDesc &d, const typename Codec<MODE>:
     }
 }
 
+// error_L2's terms (error_L2.h:3-55) of one node from the macros just stored, and u_prev <- u:
+// the partial sums of |u - u_prev|^2 and |u|^2 and the smallest density, in FP64.
+struct ResAcc {
+    double s1 = 0.0, s2 = 0.0, mn = INFINITY;
+};
+
+__device__ inline void resid_node(const BlockDesc &d, uint32_t o, Lbd r, Lbd ux, Lbd uy, Lbd px, Lbd py, ResAcc &a)
+{
+#pragma clang fp contract(off)
+    const double x = (double)ux, y = (double)uy, x0 = (double)px, y0 = (double)py;
+    a.mn = fmin(a.mn, (double)r);
+    a.s1 += (x - x0) * (x - x0) + (y - y0) * (y - y0);
+    a.s2 += x * x + y * y;
+    *off32((GLOBAL_AS Lbd *)d.upx, o) = ux;
+    *off32((GLOBAL_AS Lbd *)d.upy, o) = uy;
+}
+
+// mout (may be null): receives the macros the update wrote (rho, ux, uy); returns whether it wrote them.
 template <int MODE, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ inline void general_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
+__device__ inline bool general_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
                                     typename Codec<MODE>::GT *const (&D)[9], int i, int j, const Consts &c,
-                                    typename Codec<MODE>::R omega)
+                                    typename Codec<MODE>::R omega, Lbd *mout = nullptr)
 {
     typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
@@ -409,7 +430,7 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
         }
 #pragma unroll
         for (int k = 0; k < 9; k++) D[k][ixy] = v[OPP[k]];
-        return;
+        return false;
     }
     if (flag == FLUID) {
         R Q[9], rho, ux, uy;
@@ -420,8 +441,13 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
             d.rho[ixy] = (Lbd)rho;
             d.ux[ixy] = (Lbd)ux;
             d.uy[ixy] = (Lbd)uy;
+            if (mout) {
+                mout[0] = (Lbd)rho;
+                mout[1] = (Lbd)ux;
+                mout[2] = (Lbd)uy;
+            }
         }
-        return;
+        return MACROS;
     }
     if (flag == BOUNDARY_INLET || flag == BOUNDARY_OUTLET) {
         // channel ends by non-equilibrium extrapolation (Guo, Zheng & Shi
@@ -462,105 +488,140 @@ __device__ inline void general_node(const BlockDesc &d, const typename Codec<MOD
             d.rho[ixy] = (Lbd)rs;
             d.ux[ixy] = (Lbd)us;
             d.uy[ixy] = (Lbd)vs;
+            if (mout) {
+                mout[0] = (Lbd)rs;
+                mout[1] = (Lbd)us;
+                mout[2] = (Lbd)vs;
+            }
         }
-        return;
+        return MACROS;
     }
     // C_BOUNDARY (and any other flag): F carried over unchanged
 #pragma unroll
     for (int k = 0; k < 9; k++) D[k][ixy] = S[k][ixy];
+    return false;
 }
 
-// NR nodes of one column of a flagged block (rows jr[0..NR-1], those >= ny
-// absent), each in one instruction stream for every flag a block edge or body
-// can carry, so narrow blocks (whose waves all hold edge lanes) do not run a
-// second, scalar path: a lane pulls the 9 sources of each node (its own node
-// for a ghost copy or a pull from outside the block), collides, and keeps the
-// collided, the bounced-back or the copied values by select. The loads of all
-// NR nodes are issued before any of them is used, so a lane has NR nodes' worth
-// of memory traffic in flight per round trip. Only inlet/outlet nodes and fluid
-// nodes on the block edge (reference edge rule) branch to the general update.
-template <int MODE, bool MACROS, int NR>
-__device__ inline void flagged_rows(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
-                                    typename Codec<MODE>::GT *const (&D)[9], int i, const int (&jr)[NR],
-                                    const uint32_t (&ro)[NR], const Consts &c, typename Codec<MODE>::R omega)
+// One node of a flagged block in one instruction stream for every flag a block
+// edge or body can carry, so narrow blocks (whose waves all hold edge lanes) do
+// not run a second path: a lane pulls the 9 sources of a node, collides, and
+// keeps the collided, the bounced-back or the copied values by select.
+//  * ghost (C_BOUNDARY) nodes pull their own values (carried over);
+//  * an inlet (outlet) node pulls for its x+1 (x-1) neighbour, whose
+//    post-collision state its non-equilibrium extrapolation starts from (the
+//    neighbour's own values when it is a ghost node);
+//  * fluid nodes on the block edge, and an inlet/outlet whose neighbour is on
+//    the edge, take the equilibrium of (rho0, U0) (evolution_block.h:47-57)
+//    in place of the collision.
+// Only waves holding an inlet/outlet or edge node run those parts, and no part
+// holds more registers than the collision itself. Contraction is off in the
+// whole update (the collision spells out its fused operations), so every
+// instantiation rounds alike. RESID: with `res`, the nodes
+// off the block's outer edge whose macros the update writes (fluid, inlet,
+// outlet) also add their error_L2 terms to `acc` and set u_prev <- u (their
+// previous velocity is loaded with the flags, off the dependent path); an
+// interface op may rewrite the macros of an edge node later in the cycle, so
+// edge nodes are summed after the cycle.
+template <int MODE, bool MACROS, bool RESID = false>
+__device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
+                                    typename Codec<MODE>::GT *const (&D)[9], int i, int j, uint32_t r0,
+                                    const Consts &c, typename Codec<MODE>::R omega, bool res = false,
+                                    ResAcc *acc = nullptr)
 {
+#pragma clang fp contract(off)
     typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
     typedef typename C::T T;
     const int nx = d.nx, ny = d.ny;
     const uint32_t unx = (uint32_t)nx;
-    int flag[NR];
-    uint32_t o[NR];
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-        o[r] = ro[r] + (uint32_t)i;
-        flag[r] = (jr[r] < ny) ? d.flag[o[r]] : C_BOUNDARY;
+    const uint32_t o = r0 + (uint32_t)i;
+    // the node's flag and its x-neighbours' (same cache lines), in one round trip
+    const int flag = d.flag[o];
+    const int flm = d.flag[i > 0 ? o - 1 : o];
+    const int flp = d.flag[i < nx - 1 ? o + 1 : o];
+    const bool rnode = RESID && res && !(i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
+    Lbd px = (Lbd)0, py = (Lbd)0;
+    if (rnode) {
+        px = *off32((const GLOBAL_AS Lbd *)d.upx, o);
+        py = *off32((const GLOBAL_AS Lbd *)d.upy, o);
     }
-    T raw[NR][9];
+    const bool inlet = flag == BOUNDARY_INLET, outlet = flag == BOUNDARY_OUTLET, nee = inlet || outlet;
+    const bool solid = flag == SOLID_SBB || flag == MBL_SOLID_WALL;
+    // the node the pulls are for, and its flag
+    const int xs = inlet ? i + 1 : (outlet ? i - 1 : i);
+    const uint32_t os = r0 + (uint32_t)xs;
+    const int sflag = inlet ? flp : (outlet ? flm : flag);
+    const bool copy = nee ? (sflag == C_BOUNDARY) : !(flag == FLUID || solid);
+    const bool edge_s = (xs == 0 || j == 0 || xs == nx - 1 || j == ny - 1);
+    const bool eq = !copy && (nee || flag == FLUID) && edge_s;
+    T raw[9];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int j = jr[r];
-        const bool copy = !(flag[r] == FLUID || flag[r] == SOLID_SBB || flag[r] == MBL_SOLID_WALL);
+    for (int k = 0; k < 9; k++) {
+        const int id = xs - EX[k], jd = j - EY[k];
+        const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
+        // row jd's offset from row j's (wraps only where own holds)
+        const uint32_t rk = EY[k] == 1 ? r0 - unx : (EY[k] == -1 ? r0 + unx : r0);
+        raw[k] = *off32(S[k], own ? os : rk + (uint32_t)id);
+    }
+    R q[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const int id = i - EX[k], jd = j - EY[k];
-            const bool own = copy || id < 0 || id >= nx || jd < 0 || jd >= ny;
-            // row jd's offset from row j's (wraps only where own holds)
-            const uint32_t rk = EY[k] == 1 ? ro[r] - unx : (EY[k] == -1 ? ro[r] + unx : ro[r]);
-            raw[r][k] = (j < ny) ? *off32(S[k], own ? o[r] : rk + (uint32_t)id) : T{};
+    for (int k = 0; k < 9; k++) q[k] = C::ld(raw[k]);
+    R Q[9], rho, ux, uy;
+    collide_c<C>(q, omega, Q, rho, ux, uy);
+    if (__builtin_amdgcn_ballot_w64(eq)) {
+        if (eq) {
+            rho = (R)c.rho0;
+            ux = (R)c.U0x;
+            uy = (R)c.U0y;
+            equilibrium<C::SHIFT, R>(rho, ux, uy, Q);
         }
     }
-    bool special[NR];
+    if (__builtin_amdgcn_ballot_w64(nee)) {
+        if (nee) {
+            // channel ends by non-equilibrium extrapolation (Guo, Zheng & Shi 2002) from the
+            // post-collision state of the neighbour: the inlet imposes u_in(y) with the
+            // neighbour's density, the outlet rho0 with the neighbour's velocity
+            R rN = rho, uN = ux, vN = uy;
+            if (copy) {
+                rN = (R)d.rho[os];
+                uN = (R)d.ux[os];
+                vN = (R)d.uy[os];
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        const int j = jr[r];
-        const bool edge = (i == 0 || j == 0 || i == nx - 1 || j == ny - 1);
-        special[r] = j < ny && (flag[r] == BOUNDARY_INLET || flag[r] == BOUNDARY_OUTLET || (flag[r] == FLUID && edge));
-    }
+                for (int k = 0; k < 9; k++) Q[k] = q[k];
+            }
+            R rs, us, vs;
+            if (inlet) {
+                const double y = j * d.dx + d.base_y;
+                rs = rN;
+                us = (R)(4.0 * c.umax * (c.LY - 1.5 - y) * (y - 0.5) / (c.LY - 2.0) / (c.LY - 2.0));
+                vs = (R)0.0;
+            } else {
+                rs = (R)c.rho0;
+                us = uN;
+                vs = vN;
+            }
+            const R uuN = uN * uN + vN * vN, uus = us * us + vs * vs;
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        if (jr[r] >= ny || special[r]) continue;
-        const bool solid = (flag[r] == SOLID_SBB || flag[r] == MBL_SOLID_WALL);
-        const bool fluid = (flag[r] == FLUID);
-        R q[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) q[k] = C::ld(raw[r][k]);
-        R Q[9], rho, ux, uy;
-        collide_c<C>(q, omega, Q, rho, ux, uy);
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const T v = fluid ? C::st(Q[k]) : (solid ? raw[r][OPP[k]] : raw[r][k]);
-            __builtin_nontemporal_store(v, off32(D[k], o[r]));
+            for (int k = 0; k < 9; k++)
+                Q[k] = qeq<C::SHIFT, R>(k, rs, rs - (R)1.0, us, vs, uus) +
+                       (Q[k] - qeq<C::SHIFT, R>(k, rN, rN - (R)1.0, uN, vN, uuN));
+            rho = rs;
+            ux = us;
+            uy = vs;
         }
-        if (MACROS && fluid) {
-            __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o[r]));
-            __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o[r]));
-            __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o[r]));
-        }
     }
-    // the general update after the others, with their loaded values dead; one
-    // call site (it reads the source planes and writes only its own node)
-    unsigned todo = 0;
+    const bool upd = flag == FLUID || nee;
 #pragma unroll
-    for (int r = 0; r < NR; r++) todo |= special[r] ? 1u << r : 0u;
-    while (todo) {
-        const int r = __builtin_ctz(todo);
-        todo &= todo - 1;
-        int j = jr[0];
-#pragma unroll
-        for (int q = 1; q < NR; q++) j = (r == q) ? jr[q] : j; // selects, not an indexed (scratch) array
-        general_node<MODE, false, true, MACROS>(d, S, D, i, j, c, omega);
+    for (int k = 0; k < 9; k++) {
+        const T v = upd ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
+        __builtin_nontemporal_store(v, off32(D[k], o));
     }
-}
-
-template <int MODE, bool MACROS>
-__device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MODE>::GT *const (&S)[9],
-                                    typename Codec<MODE>::GT *const (&D)[9], int i, int j, uint32_t r0,
-                                    const Consts &c, typename Codec<MODE>::R omega)
-{
-    const int jr[1] = {j};
-    const uint32_t ro[1] = {r0};
-    flagged_rows<MODE, MACROS, 1>(d, S, D, i, jr, ro, c, omega);
+    if (MACROS && upd) {
+        __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o));
+        __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o));
+        __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o));
+        if (rnode) resid_node(d, o, (Lbd)rho, (Lbd)ux, (Lbd)uy, px, py, *acc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -571,9 +632,11 @@ __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MOD
 // ---------------------------------------------------------------------------
 // One row segment of a tile: 64*VX consecutive nodes of row j (element offset
 // r0 = j * nx in the planes) starting at the wave's first node.
-template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
-__device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, uint32_t r0, int x0)
+template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS, bool RESID = false>
+__device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask, int j, uint32_t r0, int x0,
+                                       bool res = false, ResAcc *acc = nullptr)
 {
+    static_assert(!RESID || (FLAGS && MACROS && !PERIODIC && VX == 1), "residual terms: flagged path only");
     typedef typename Codec<MODE>::R R;
     typedef Codec<MODE> C;
     typedef typename C::T T;
@@ -592,7 +655,7 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
     const R omega = (R)d.omega;
 
     if constexpr (FLAGS && !PERIODIC && VX == 1) {
-        flagged_node<MODE, MACROS>(d, S, D, x0, j, r0, c, omega);
+        flagged_node<MODE, MACROS, RESID>(d, S, D, x0, j, r0, c, omega, res, acc);
         return;
     }
 
@@ -688,9 +751,47 @@ __device__ __forceinline__ void sc_row(const BlockDesc &d, uint32_t parity_mask,
 // rows at a time, the workgroup's waves the next ones in turn. The block comes from the per-tile table when
 // there is one (one scalar load), else from a search of tile_begin.
 // wv: the wave's index among the nwv waves that take the tile.
-template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
+// The tile's residual sums (weighted by the block's 1/m, as k_residual_tiles weights its
+// tiles) into out[0..2]: sum |u - u_prev|^2 / m, sum |u|^2 / m, smallest density. Every thread
+// of the workgroup calls it.
+__device__ inline double wave_sum(double v);
+
+__device__ inline double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline void tile_resid_store(const ResAcc &a, double inv_m, double *out)
+{
+    __shared__ double rsh[3 * 16];
+    const double s1 = wave_sum(a.s1), s2 = wave_sum(a.s2), mn = wave_min(a.mn);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads(); // the previous tile's reads of rsh are done
+    if ((threadIdx.x & 63) == 0) {
+        rsh[3 * w] = s1;
+        rsh[3 * w + 1] = s2;
+        rsh[3 * w + 2] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t1 = 0.0, t2 = 0.0, tm = INFINITY;
+        for (int q = 0; q < nw; q++) {
+            t1 += rsh[3 * q];
+            t2 += rsh[3 * q + 1];
+            tm = fmin(tm, rsh[3 * q + 2]);
+        }
+        out[0] = t1 * inv_m;
+        out[1] = t2 * inv_m;
+        out[2] = tm;
+    }
+}
+
+template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS, bool RESID = false>
 __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, const int *__restrict__ tile_blk, int ndesc,
-                               int tile_rows, uint32_t parity_mask, int t, int wv, int nwv)
+                               int tile_rows, uint32_t parity_mask, int t, int wv, int nwv, uint32_t rmask = 0,
+                               double *rparts = nullptr)
 {
     int lo = 0;
     if (tile_blk) {
@@ -724,6 +825,9 @@ __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, con
     const int rows = d.trows > 0 ? d.trows : tile_rows;
     const int step = nwv * per_wave;
     const uint32_t unx = (uint32_t)d.nx;
+    // residual terms of this tile's nodes when its level's last collide of the cycle is this one
+    const bool res = RESID && ((rmask >> d.level) & 1u);
+    ResAcc acc;
     // the wave's first row rw and its row offset are scalar; a lane adds its segment's row (sub)
     for (int rw = __builtin_amdgcn_readfirstlane(wv) * per_wave; rw < rows; rw += step) {
         if (rw + sub >= rows) break;
@@ -733,17 +837,23 @@ __device__ __forceinline__ void sc_tile(const BlockDesc *__restrict__ descs, con
         // length that is not a multiple of VX begins (j*nx mod VX) nodes into
         // its first segment; the host sizes tiles_x for that shift.
         const int x0 = PERIODIC ? base : base - (int)(r0 & (uint32_t)(VX - 1));
-        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS>(d, parity_mask, jw + sub, r0, x0);
+        sc_row<MODE, VX, PERIODIC, FLAGS, MACROS, RESID>(d, parity_mask, jw + sub, r0, x0, res, &acc);
+    }
+    if constexpr (RESID) {
+        if (res) tile_resid_store(acc, d.inv_m, rparts + 3 * (size_t)t);
     }
 }
 
 // One tile per workgroup when the grid covers every tile; otherwise a
 // persistent grid in which workgroup b (after the XCD-aware reordering) walks
 // one contiguous chunk of tiles, so x-neighbouring tiles stay on one XCD.
-template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS>
+// RESID: the tiles of the levels in rmask also write their residual sums to rparts[3 t ..]
+// (t: the tile's number across all levels).
+template <int MODE, int VX, bool PERIODIC, bool FLAGS, bool MACROS, bool RESID = false>
 __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restrict__ descs,
                                                         const int *__restrict__ tile_blk, int ndesc, int tile_rows,
-                                                        uint32_t parity_mask, int total_tiles, int tile_base)
+                                                        uint32_t parity_mask, int total_tiles, int tile_base,
+                                                        uint32_t rmask, double *rparts)
 {
     const int G = (int)gridDim.x;
     const int b = xcd_tile((int)blockIdx.x, G);
@@ -758,8 +868,9 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
     t1 += tile_base;
     // one call site, inlined: the descriptor and plane bases stay in scalar registers
     for (int t = t0; t < t1; t++)
-        sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS>(descs, tile_blk, ndesc, tile_rows, parity_mask, t,
-                                                   (int)(threadIdx.x >> 6), (int)(blockDim.x >> 6));
+        sc_tile<MODE, VX, PERIODIC, FLAGS, MACROS, RESID>(descs, tile_blk, ndesc, tile_rows, parity_mask, t,
+                                                          (int)(threadIdx.x >> 6), (int)(blockDim.x >> 6), rmask,
+                                                          rparts);
 }
 
 // ---------------------------------------------------------------------------
@@ -1126,12 +1237,15 @@ __global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ 
 // The whole spline step of one line in one workgroup (lines up to
 // SPL_FUSED_MAX_NC coarse points): gather the 12 columns (init_Spline3,
 // assit_multi.h:48-116), solve tridiag(l, m, u) M = rhs for all 12 by PCR at
-// once -- the matrix is shared, so each row's elimination factors are computed
-// once and applied to 12 right-hand sides -- and evaluate the fine points
-// (calculate_Spline3_gpu, :136-185) from the columns and M held in LDS. Same
-// operations, in the same order per column, as k_spline_solve + k_spline_eval.
-// One coarse row per thread; blockDim.x >= nc.
-constexpr int SPL_FUSED_MAX_NC = 303; // 27 * 8 * nc bytes of LDS within 64 KiB
+// once, and evaluate the fine points (calculate_Spline3_gpu, :136-185) from
+// the columns and M held in LDS. The matrix is the same at every step, so the
+// elimination factors of every PCR level are computed once per layout
+// (LineOp::pcr): a level is one LDS exchange of the 12 right-hand sides and two
+// multiply-adds per column, with no division and no vote. The fine points'
+// write masks and node indices are loaded before the gather, off the path of
+// the dependent loads. Same operations, in the same order per column, as
+// k_spline_solve + k_spline_eval. One coarse row per thread; blockDim.x >= nc.
+constexpr int SPL_FUSED_MAX_NC = 303; // 24 * 8 * nc bytes of LDS within 64 KiB (and the drag's)
 
 template <int MODE>
 __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *__restrict__ blocks,
@@ -1139,14 +1253,29 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
 {
     typedef typename Codec<MODE>::T T;
     const int slot = ms.m[op.mgroup][0]; // the oldest slot of the line's ring receives t_{n+1}
-    const int N = op.nc;
+    const int N = op.nc, NF = op.n, NL = op.pcr_levels;
     const BlockDesc &sb = blocks[op.src_block];
+    const BlockDesc &db = blocks[op.dst_block];
     const int ps = (parity_mask >> sb.level) & 1;
-    // lds: G[12][N] | a[N] | b[N] | c[N] | d[12][N]
-    double *G = lds, *La = lds + 12 * N, *Lb = La + N, *Lc = Lb + N, *Ld = Lc + N;
+    // lds: G[12][N] | d[12][N]
+    double *G = lds, *Ld = lds + 12 * N;
     const int ip = threadIdx.x;
     const bool row = ip < N;
-    double a = 0.0, b = 1.0, c = 0.0, d[12];
+    // the first fine point of this thread: its write mask and node index, loaded now
+    const int f0 = ip;
+    const uint8_t mk0 = f0 < NF ? op.mask[f0] : (uint8_t)0;
+    const int di0 = f0 < NF ? op.dst_idx[f0] : 0;
+    // this row's elimination factors of every level, and its reduced diagonal
+    double al[PCR_MAX_LEVELS], ga[PCR_MAX_LEVELS], bd = 1.0;
+    if (row) {
+#pragma unroll
+        for (int q = 0; q < PCR_MAX_LEVELS; q++) {
+            al[q] = q < NL ? op.pcr[(size_t)(2 * q) * N + ip] : 0.0;
+            ga[q] = q < NL ? op.pcr[(size_t)(2 * q + 1) * N + ip] : 0.0;
+        }
+        bd = op.pcr[(size_t)(2 * NL) * N + ip];
+    }
+    double d[12];
     if (row) {
         const int node = op.src_idx[ip];
         double g[12];
@@ -1160,9 +1289,6 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
             G[q * N + ip] = g[q];
             op.gcol[q][ip] = (Lbd)g[q];
         }
-        a = (double)op.tl[ip];
-        b = (double)op.tm[ip];
-        c = (double)op.tu[ip];
     }
     __syncthreads();
 #pragma unroll
@@ -1170,64 +1296,47 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
         d[q] = (!row || ip == 0 || ip == N - 1)
                    ? 0.0
                    : (6.0 * G[q * N + ip] - 3.0 * G[q * N + ip - 1] - 3.0 * G[q * N + ip + 1]);
-    __syncthreads(); // the right-hand sides are formed before the first level's writes
-    for (int s = 1; s < N; s <<= 1) {
+#pragma unroll
+    for (int lv = 0; lv < PCR_MAX_LEVELS; lv++) {
+        if (lv >= NL) continue; // NL is the same for the whole workgroup
+        const int s = 1 << lv;
         if (row) {
-            La[ip] = a;
-            Lb[ip] = b;
-            Lc[ip] = c;
 #pragma unroll
             for (int q = 0; q < 12; q++) Ld[q * N + ip] = d[q];
         }
         __syncthreads();
         if (row) {
             const int im = ip - s, iq = ip + s;
-            double na = 0.0, nc = 0.0, nb = b;
             if (im >= 0) {
-                const double alpha = -a / Lb[im];
-                na = alpha * La[im];
-                nb += alpha * Lc[im];
 #pragma unroll
-                for (int q = 0; q < 12; q++) d[q] += alpha * Ld[q * N + im];
+                for (int q = 0; q < 12; q++) d[q] += al[lv] * Ld[q * N + im];
             }
             if (iq < N) {
-                const double gamma = -c / Lb[iq];
-                nc = gamma * Lc[iq];
-                nb += gamma * La[iq];
 #pragma unroll
-                for (int q = 0; q < 12; q++) d[q] += gamma * Ld[q * N + iq];
+                for (int q = 0; q < 12; q++) d[q] += ga[lv] * Ld[q * N + iq];
             }
-            a = na;
-            b = nb;
-            c = nc;
         }
-        // Diagonally dominant systems decouple fast (the spline matrix tridiag(0.5, 2, 0.5): couplings
-        // 0.25, 0.071, 5e-3, 3e-5, 7e-10, 5e-19 of the diagonal over the levels). Once every row's
-        // couplings are below 2^-60 of its diagonal, further levels change no result beyond
-        // rounding: the workgroup votes and stops (the vote is also the barrier between this
-        // level's LDS reads and the next level's writes).
-        const bool decoupled = !row || (fabs(a) <= 0x1p-60 * fabs(b) && fabs(c) <= 0x1p-60 * fabs(b));
-        if (__syncthreads_and(decoupled)) break;
+        __syncthreads(); // this level's reads before the next level's writes
     }
     if (row) {
 #pragma unroll
         for (int q = 0; q < 12; q++) {
-            const double M = d[q] / b;
+            const double M = d[q] / bd;
             Ld[q * N + ip] = M;
             op.varibs[(size_t)q * N + ip] = (Lbd)M;
         }
     }
     __syncthreads();
     const double *M = Ld;
-    for (int f = threadIdx.x; f < op.n; f += blockDim.x) {
+    for (int f = f0; f < NF; f += blockDim.x) {
+        const uint8_t mk = f == f0 ? mk0 : op.mask[f];
         const double xx = (double)f / (double)op.ratio;
         const int I = (ceil(xx) == 0.0) ? (int)(ceil(xx) + 1.0) : (int)ceil(xx);
         const double rho = spline3_s(xx, I - 1.0, 1.0, G[I - 1], G[I], M[I - 1], M[I]);
         const double ux = spline3_s(xx, I - 1.0, 1.0, G[N + I - 1], G[N + I], M[N + I - 1], M[N + I]);
         const double uy = spline3_s(xx, I - 1.0, 1.0, G[2 * N + I - 1], G[2 * N + I], M[2 * N + I - 1], M[2 * N + I]);
-        if (op.mask[f] & 1) {
-            const BlockDesc &db = blocks[op.dst_block];
-            const int idx = op.dst_idx[f];
+        if (mk & 1) {
+            const int idx = f == f0 ? di0 : op.dst_idx[f];
             db.rho[idx] = (Lbd)rho;
             db.ux[idx] = (Lbd)ux;
             db.uy[idx] = (Lbd)uy;
@@ -1462,6 +1571,39 @@ __global__ __launch_bounds__(256) void k_residual_cases(const int *__restrict__ 
         out[4 * (size_t)c + 1] = all1;
         out[4 * (size_t)c + 2] = all2;
         out[4 * (size_t)c + 3] = allmn;
+    }
+}
+
+// The diagnostics of one coarse step after a cycle whose last stream-collide of each level
+// wrote its tiles' residual sums (error_L2.h:57-91 and force_block.h:180-216).
+// Workgroups [0, list_wgs) take error_L2's terms of the nodes the collides do not sum (block
+// edges, solid and ghost nodes: their macros are final once the cycle is done), u_prev <- u,
+// into their partial-sum slots after the tiles'; workgroup list_wgs (with a drag) sums the
+// momentum exchange. k_residual_final then sums every slot in a fixed order (a second launch:
+// within one launch, workgroups on different XCDs would need a write-back of the L2 to see
+// each other's partial sums).
+template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagArgs a)
+{
+    __shared__ double sh[16];
+    const int wg = (int)blockIdx.x;
+    if (wg < a.list_wgs) {
+        ResAcc acc;
+        const int e = wg * (int)blockDim.x + (int)threadIdx.x;
+        if (e < a.nlist) {
+            const int2 q = a.list[e];
+            const BlockDesc &d = a.descs[q.x];
+            const uint32_t o = (uint32_t)q.y;
+            const Lbd ux = d.ux[o], uy = d.uy[o], r = d.rho[o], px = d.upx[o], py = d.upy[o];
+            const double x = (double)ux, y = (double)uy, x0 = (double)px, y0 = (double)py;
+            acc.mn = (double)r;
+            acc.s1 = ((x - x0) * (x - x0) + (y - y0) * (y - y0)) * d.inv_m;
+            acc.s2 = (x * x + y * y) * d.inv_m;
+            d.upx[o] = ux;
+            d.upy[o] = uy;
+        }
+        tile_resid_store(acc, 1.0, a.rparts + 3 * ((size_t)a.ntiles + (size_t)wg));
+    } else {
+        drag_case<MODE>(a.drag, a.drag.out, sh);
     }
 }
 
@@ -1705,26 +1847,37 @@ template <int MODE, int VX> __global__ __launch_bounds__(CASE_TPB) void k_cases(
 
 template <int MODE, int VX>
 static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d,
-                       const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups, int tbase)
+                       const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups, int tbase,
+                       uint32_t rmask, double *rparts)
 {
     dim3 g(groups > 0 && groups < tiles ? groups : tiles), b(tpb);
     if (periodic) {
         if (flags) {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
         } else {
             if (macros)
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
             else
-                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, true, false, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
         }
     } else {
-        if (macros)
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
-        else
-            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase);
+        if (macros) {
+            if constexpr (VX == 1) {
+                if (rmask)
+                    hipLaunchKernelGGL((k_stream_collide<MODE, 1, false, true, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, rmask, rparts);
+                else
+                    hipLaunchKernelGGL((k_stream_collide<MODE, 1, false, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
+            } else {
+                if (rmask) return MBL_E_UNSUPPORTED;
+                hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, true>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
+            }
+        } else {
+            if (rmask) return MBL_E_UNSUPPORTED;
+            hipLaunchKernelGGL((k_stream_collide<MODE, VX, false, true, false>), g, b, 0, s, d, tb, nd, rows, pm, tiles, tbase, 0u, nullptr);
+        }
     }
     MBL_LAUNCH_CHECK();
     return 0;
@@ -1732,13 +1885,15 @@ static int sc_dispatch(hipStream_t s, int tpb, bool periodic, bool flags, bool m
 
 template <int MODE>
 static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bool macros, const BlockDesc *d,
-                   const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups, int tbase)
+                   const int *tb, int nd, int tiles, int rows, uint32_t pm, int groups, int tbase, uint32_t rmask,
+                   double *rparts)
 {
+    if (rmask && (periodic || !flags || !macros || vx != 1 || !rparts)) return MBL_E_UNSUPPORTED;
     switch (vx) {
-    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
-    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
-    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
-    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase);
+    case 1: return sc_dispatch<MODE, 1>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase, rmask, rparts);
+    case 2: return sc_dispatch<MODE, 2>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase, rmask, rparts);
+    case 4: return sc_dispatch<MODE, 4>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase, rmask, rparts);
+    case 8: return sc_dispatch<MODE, 8>(s, tpb, periodic, flags, macros, d, tb, nd, tiles, rows, pm, groups, tbase, rmask, rparts);
     default: return MBL_E_INVALID;
     }
 }
@@ -1786,13 +1941,13 @@ static int sc_mode(hipStream_t s, int vx, int tpb, bool periodic, bool flags, bo
 
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
-                          uint32_t parity_mask, int groups, int tile_base)
+                          uint32_t parity_mask, int groups, int tile_base, uint32_t rmask, double *rparts)
 {
     if (total_tiles <= 0) return 0;
     if (tpb % 64 || tpb < 64 || tpb > 256 || tile_rows < tpb / 64 || tile_base < 0) return MBL_E_INVALID;
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, return sc_mode<M_>(s, vx, tpb, periodic, flags, macros, d_descs, d_tile_blk, ndesc,
-                                             total_tiles, tile_rows, parity_mask, groups, tile_base));
+                                             total_tiles, tile_rows, parity_mask, groups, tile_base, rmask, rparts));
     return 0;
 }
 
@@ -1864,7 +2019,7 @@ static int spline_solve_r(hipStream_t s, int R, const LineOp *ops, int nops, siz
 
 int launch_spline_path(int max_nc) { return max_nc <= SPL_FUSED_MAX_NC ? 0 : 1; }
 
-size_t cases_lds(int max_nc) { return 27 * sizeof(double) * (size_t)(max_nc > 0 ? max_nc : 0); }
+size_t cases_lds(int max_nc) { return 24 * sizeof(double) * (size_t)(max_nc > 0 ? max_nc : 0); }
 
 int cases_spline_ok(int max_nc) { return max_nc <= SPL_FUSED_MAX_NC && max_nc <= CASE_TPB; }
 
@@ -1901,9 +2056,9 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
     if (nops <= 0) return 0;
     if (max_nc > SPL_MAX_NC) return MBL_E_UNSUPPORTED;
     if (max_nc <= SPL_FUSED_MAX_NC) {
-        int threads = 64;
-        while (threads < max_nc) threads <<= 1;
-        const size_t lds = 27 * sizeof(double) * (size_t)max_nc;
+        // one thread per coarse row, and per fine point where the workgroup allows (up to 1024)
+        const int threads = std::min(1024, (std::max(max_nc, max_nf) + 63) / 64 * 64);
+        const size_t lds = 24 * sizeof(double) * (size_t)max_nc;
         hipStream_t s = (hipStream_t)stream;
         MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_>), dim3(1, nops), dim3(threads), lds, s, d_ops,
                                                  d_blocks, parity_mask, ms));
@@ -1971,6 +2126,19 @@ int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases
     if (ncases <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_drag_batch<M_>), dim3(ncases), dim3(256), 0, s, d_args, out));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_step_diag(void *stream, int mode, const StepDiagArgs &a)
+{
+    if (a.list_wgs < 1 || a.ntiles < 0 || !a.rparts || !a.out) return MBL_E_INVALID;
+    const int grid = a.list_wgs + (a.drag.n > 0 ? 1 : 0);
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_step_diag<M_>), dim3(grid), dim3(256), 0, s, a));
+    MBL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, a.ntiles + a.list_wgs, (const double *)a.rparts,
+                       a.out);
     MBL_LAUNCH_CHECK();
     return 0;
 }

@@ -6,11 +6,14 @@
  * evolution -> error_L2 -> getDrag_block with the per-chunk timing line of
  * time_t.dat. It includes only include/mbl.h and links libmbl_hip.
  *
- *   mbl_driver [layout] [steps] [T] [output]
+ *   mbl_driver [layout] [steps] [T] [output] [fused]
  *     layout: GEO_1_1 | GEO_2_2 | GEO_2_5 (default) | GEO_4_9 | C1
  *     steps : total coarse steps (default 1000), T: steps per timing chunk
  *     output: 1 writes steady_<n>.dat at the start and after every chunk, as
  *             the reference does (Multi-block-LBM.cu:120, 170); default 0
+ *     fused : 1 runs each chunk as one mbl_evolution_step call (the same loop
+ *             body, with the residual summed inside the cycle); default 0,
+ *             the reference's three calls per step
  */
 #define _POSIX_C_SOURCE 199309L
 #include <math.h>
@@ -45,6 +48,7 @@ int main(int argc, char **argv)
     int T = argc > 3 ? atoi(argv[3]) : 1000;
     if (T <= 0) T = 1000;
     const int write_fields = argc > 4 ? atoi(argv[4]) : 0;
+    const int fused = argc > 5 ? atoi(argv[5]) : 0;
 
     mbl_geometry g;
     mbl_config cfg;
@@ -163,11 +167,16 @@ int main(int argc, char **argv)
     while (done < steps) {
         int chunk = steps - done < T ? steps - done : T;
         double t0 = wall_s();
-        for (int n = 0; n < chunk; n++) {
-            int t = done + n;
-            evolution(t, asp, ls3, ltc, lcc, 0, block, grid, Areas, Vh, Vd, LC, LF, LCC, t3, s3, h);
-            error_L2(t, na, block, grid, Areas, Vd, upsx, upsy, NULL, NULL, NULL, NULL);
-            if (N_drag > 0) getDrag_block(t, d_links, Areas[last], Vd[last], NULL, N_drag);
+        if (fused) {
+            mbl_evolution_step(h, done, chunk, asp, ls3, ltc, lcc, block, grid, Areas, Vh, Vd, LC, LF, LCC, t3, s3,
+                               upsx, upsy, d_links, N_drag, NULL);
+        } else {
+            for (int n = 0; n < chunk; n++) {
+                int t = done + n;
+                evolution(t, asp, ls3, ltc, lcc, 0, block, grid, Areas, Vh, Vd, LC, LF, LCC, t3, s3, h);
+                error_L2(t, na, block, grid, Areas, Vd, upsx, upsy, NULL, NULL, NULL, NULL);
+                if (N_drag > 0) getDrag_block(t, d_links, Areas[last], Vd[last], NULL, N_drag);
+            }
         }
         mbl_flush_diagnostics(h);
         if ((rc = mbl_synchronize(h))) {

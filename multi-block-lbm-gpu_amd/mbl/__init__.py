@@ -78,6 +78,12 @@ def load_library(fp64=False):
         "sum_N_drag": (ip, [mbl_dim3, mbl_dim3, AREA, D2Q9]),
         "fdrag_store": (None, [AREA, D2Q9, vp, D2Q9, ip]),
         "getDrag_block": (None, [ip, vp, AREA, D2Q9, dp, ip]),
+        "mbl_evolution_step_fused": (ip, [vp]),
+        "mbl_evolution_step": (ip, [vp, ip, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), mbl_dim3,
+                                    C.POINTER(mbl_dim3), C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(D2Q9),
+                                    C.POINTER(Inf_Ff_store), C.POINTER(Inf_Ff_store), C.POINTER(Inf_Ff_store),
+                                    C.POINTER(Ff_Store), C.POINTER(FM_spline3), C.POINTER(vp), C.POINTER(vp), vp, ip,
+                                    dp]),
         "error_L2": (None, [ip, ip, mbl_dim3, C.POINTER(mbl_dim3), C.POINTER(AREA), C.POINTER(D2Q9),
                             C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp]),
         "mbl_last_diagnostics": (ip, [vp, dp]),
@@ -224,8 +230,9 @@ def c2b_geometry(lib):
 class MultiBlockSim:
     def __init__(self, lib, geom, storage=abi.STORE_NATIVE, use_graph=1, sync_diag=0, write_files=0,
                  out_dir="", flush_interval=1000, rho0=1.0, U0=(0.1, 0.0), init_mode=None, bc_mode=None,
-                 tuning=None):
+                 tuning=None, fused=True):
         self.lib = lib
+        self.fused = fused
         self.geom = geom
         g = geom.g
         self.real = lib.real
@@ -313,10 +320,20 @@ class MultiBlockSim:
         self.lib.evolution(self.t, self.asp, self.ls3, self.ltc, self.lccp, 0, self.block, self.grid, self.areas,
                            self.pps_h, self.pps_d, self.lc, self.lf, self.lcc, self.t3, self.s3, self.h)
 
-    def step(self, n=1, diagnostics=True):
+    def step(self, n=1, diagnostics=True, fused=None):
+        """n coarse steps of the reference loop: evolution, then (diagnostics) error_L2 and
+        getDrag_block. fused (default: the sim's): the n steps in one mbl_evolution_step call."""
         lib = self.lib
         lib.mbl_handle_make_current(self.h)
         last = self.na - 1
+        if diagnostics and (self.fused if fused is None else fused):
+            rc = lib.mbl_evolution_step(self.h, self.t, n, self.asp, self.ls3, self.ltc, self.lccp, self.block,
+                                        self.grid, self.areas, self.pps_h, self.pps_d, self.lc, self.lf, self.lcc,
+                                        self.t3, self.s3, self.upx, self.upy, self.d_links, self.n_drag, None)
+            self.t += n
+            if rc:
+                raise MblError(f"step failed: {abi.E_NAMES.get(rc, rc)}")
+            return
         for _ in range(n):
             self.evolve()
             if diagnostics:

@@ -31,8 +31,8 @@ def node_updates_per_step(sim):
 
 
 def run(lib, geom, steps, storage=abi.STORE_NATIVE, chunk=1000, D=None, U=0.1, progress=False, tuning=None, fork=False,
-        **kw):
-    sim = MultiBlockSim(lib, geom, storage=storage, flush_interval=min(chunk, 4096), **kw)
+        fused=True, **kw):
+    sim = MultiBlockSim(lib, geom, storage=storage, flush_interval=min(chunk, 4096), fused=fused, **kw)
     if tuning:
         check(lib, lib.mbl_set_tuning(sim.h, tuning[0], tuning[1]), "tuning")
     if fork:
@@ -79,6 +79,9 @@ def main(argv=None):
     ap.add_argument("--tpb", type=int, default=0, help="threads per workgroup (0: library default)")
     ap.add_argument("--fork", action="store_true",
                     help="launches at one position of the scheduled cycle on forked side streams")
+    ap.add_argument("--three-calls", action="store_true",
+                    help="each step as evolution, error_L2 and getDrag_block (default: one mbl_evolution_step "
+                         "call per chunk)")
     args = ap.parse_args(argv)
     lib = load_library(args.fp64)
     if args.case == "c1":
@@ -87,7 +90,7 @@ def main(argv=None):
         geom, steps, D = c2b_geometry(lib), args.steps or 50000, 16.0
     out, series = run(lib, geom, steps, storage=STORAGE[args.storage] if not args.fp64 else abi.STORE_NATIVE,
                       D=D, progress=args.progress, tuning=(args.vx, args.tpb) if (args.vx or args.tpb) else None,
-                      fork=args.fork)
+                      fork=args.fork, fused=not args.three_calls)
     out.update({"case": args.case, "precision": "f64" if args.fp64 else args.storage})
     if args.save:
         np.savez(args.save, **series)

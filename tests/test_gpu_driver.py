@@ -78,15 +78,16 @@ def oracle_series(geom, steps, **cfg_kw):
     return out
 
 
-@pytest.mark.parametrize("exe,layout,steps", [("mbl_driver64", "GEO_2_5", 12), ("mbl_driver64", "C1", 400),
-                                              ("mbl_driver", "C1", 400)])
-def test_driver_files_match_oracle(tmp_path, lib32, exe, layout, steps):
+@pytest.mark.parametrize("exe,layout,steps,fused", [("mbl_driver64", "GEO_2_5", 12, 0), ("mbl_driver64", "C1", 400, 0),
+                                                    ("mbl_driver", "C1", 400, 0), ("mbl_driver64", "C1", 400, 1),
+                                                    ("mbl_driver", "GEO_2_5", 12, 1)])
+def test_driver_files_match_oracle(tmp_path, lib32, exe, layout, steps, fused):
     """The C driver's error_L2.dat and fdrag_t.dat (written with the reference's "%g", six
     significant digits) against the FP64 oracle's series of the same layout: every line
     within the format's rounding (5e-6 relative), the FP32 build's forces too (its residual, a
     difference of consecutive velocity fields, carries FP32 rounding of u and is compared at
-    2e-3)."""
-    r = subprocess.run([os.path.join(BIN, exe), layout, str(steps), str(steps)], cwd=tmp_path,
+    2e-3). fused: the driver's chunks as mbl_evolution_step calls."""
+    r = subprocess.run([os.path.join(BIN, exe), layout, str(steps), str(steps), "0", str(fused)], cwd=tmp_path,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     l2 = read_dat(tmp_path / "error_L2.dat", 2)

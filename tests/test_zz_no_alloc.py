@@ -23,7 +23,9 @@ def tiny3(lib):
 
 
 def compare(lib, geom, steps):
-    a = mbl.MultiBlockSim(lib, geom, flush_interval=4096)
+    # a steps through the reference calls (evolution, error_L2, getDrag_block): its residual is
+    # error_L2's, the one the allocation-free entry point reproduces bit for bit
+    a = mbl.MultiBlockSim(lib, geom, flush_interval=4096, fused=False)
     b = mbl.MultiBlockSim(lib, geom, flush_interval=4096)
     d_out = lib.mbl_malloc(4 * 8)
     rec = np.zeros(4)
