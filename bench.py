@@ -51,6 +51,7 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
     import mbl
 
     blk = mbl.PeriodicBlock(lib, n, n, tau=0.6, storage=storage, tuning=tuning)
+    m0 = blk.moments()  # mass, momentum and non-finite count of the initial field (device sums)
     # bring the GPU clocks up first: untimed steps for at least half a second, then the W warmup steps
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < 0.5:
@@ -61,10 +62,16 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
     t0 = time.perf_counter()
     ev_ms = blk.timed_steps(steps)  # events bracket the K launches; ends with a synchronize
     wall = time.perf_counter() - t0
+    m1 = blk.moments()
     blk.close()
     nodes = n * n
+    # a periodic BGK block conserves mass and momentum: their change over the run (relative to
+    # the mass) is rounding, and every entry stays finite
+    inv = {"mass_rel_change": float((m1[0] - m0[0]) / m0[0]),
+           "momentum_change_per_node": [float((m1[1] - m0[1]) / nodes), float((m1[2] - m0[2]) / nodes)],
+           "nonfinite_entries": int(m1[3])}
     return {"wall_s": wall, "event_ms": ev_ms, "per_launch_ms": ev_ms / steps,
-            "mlups": nodes * steps / wall / 1e6, "nodes": nodes}
+            "mlups": nodes * steps / wall / 1e6, "nodes": nodes, "invariants": inv}
 
 
 def stream_copy_gbs(lib, mib=4096, reps=10):
@@ -206,6 +213,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.storage),
                      "kernel": "k_stream_collide (periodic)", "per_launch_ms": round(r["per_launch_ms"], 5),
                      "bytes_per_node": bpn},
+        "invariants": r["invariants"],
     }
     copy, copy9 = stream_copy_gbs(lib)
     if copy:

@@ -437,6 +437,12 @@ int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 *pps_d, dou
  * path of the single-block periodic bandwidth benchmark. */
 int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
 
+/* Invariants of one block's state (the F planes): out[0] = sum of f over every node and
+ * direction (the mass), out[1], out[2] = sum of f e_x, f e_y (the momentum), out[3] = number
+ * of non-finite entries; FP64 sums on the device. Synchronous. No reference counterpart: a
+ * check of full-size runs whose planes are too large to copy back. */
+int mbl_block_moments(mbl_handle_t h, const AREA *area, const D2Q9 *pp, double out[4]);
+
 /* Per-launch time of the fused stream-collide kernel: wraps the next
  * mbl_step_block() launches in HIP events on the handle's stream. */
 int mbl_timer_begin(mbl_handle_t h);

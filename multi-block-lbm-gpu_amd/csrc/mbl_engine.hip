@@ -1910,6 +1910,33 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     return 0;
 }
 
+extern "C" int mbl_block_moments(mbl_handle_t h, const AREA *area, const D2Q9 *pp, double out[4])
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!area || !pp || !out || area->nx <= 0 || area->ny <= 0) return set_err(h, MBL_E_INVALID);
+    for (int k = 0; k < 9; k++)
+        if (!pp->F[k]) return set_err(h, MBL_E_INVALID);
+    const size_t n = (size_t)area->nx * area->ny;
+    const int nparts = (int)std::min<size_t>(2048, (n + 255) / 256);
+    void **dplanes = nullptr;
+    double *buf = nullptr;
+    void *hp[9];
+    for (int k = 0; k < 9; k++) hp[k] = (void *)pp->F[k];
+    int rc = 0;
+    if (hipMalloc(&dplanes, sizeof(hp)) != hipSuccess || hipMalloc(&buf, sizeof(double) * (4 * (size_t)nparts + 4)) != hipSuccess)
+        rc = MBL_E_NOMEM;
+    if (!rc && hipMemcpyAsync(dplanes, hp, sizeof(hp), hipMemcpyHostToDevice, h->stream) != hipSuccess) rc = MBL_E_HIP;
+    if (!rc) rc = launch_moments(h->stream, storage_mode(h->cfg), dplanes, n, nparts, buf, buf + 4 * (size_t)nparts);
+    if (!rc && (hipMemcpyAsync(out, buf + 4 * (size_t)nparts, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess))
+        rc = MBL_E_HIP;
+    if (dplanes) hipFree(dplanes);
+    if (buf) hipFree(buf);
+    return rc ? set_err(h, rc) : 0;
+}
+
 // Times `reps` launches of a copy between two fresh buffers; the copy is
 // `planes` planes of `plane_bytes` each (1: one stream), GB/s of bytes read + written.
 static int copy_probe(mbl_context *h, size_t plane_bytes, int planes, int reps, double *gbs)

@@ -234,6 +234,10 @@ int launch_raster(void *stream, const RasterBlock *d_blocks, const RasterCase *d
 int launch_init_raster(void *stream, int mode, const RasterBlock *d_blocks, int nblocks, size_t max_nodes,
                        const RasterCase *d_cases, int H, int W, const float *d_fields);
 int launch_decode(void *stream, int mode, const void *plane, int k, size_t n, double *out);
+// d_out[0..3]: sum of f, f e_x, f e_y over the 9 planes d_planes[0..8] of n entries, and the
+// number of non-finite entries; nparts workgroups write partials[4 nparts] first.
+int launch_moments(void *stream, int mode, const void *const *d_planes, size_t n, int nparts, double *partials,
+                   double *d_out);
 int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const double *in);
 
 } // namespace mbl

@@ -1607,6 +1607,49 @@ template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagA
     }
 }
 
+// Zeroth and first moments of one block's 9 planes (sum of f, f e_x, f e_y over every node)
+// and the count of non-finite entries, FP64: workgroup partials in a grid-stride order,
+// then one ordered sum (k_moments_final).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_moments(const void *const *planes, size_t n, double *partials)
+{
+    typedef typename Codec<MODE>::T T;
+    __shared__ double sh[16];
+    double m = 0.0, mx = 0.0, my = 0.0, bad = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const double f = dec64<MODE>(((const T *)planes[k])[i], k);
+            if (!isfinite(f)) {
+                bad += 1.0;
+                continue;
+            }
+            m += f;
+            mx += f * EX[k];
+            my += f * EY[k];
+        }
+    }
+    const double a = group_sum(m, sh), b = group_sum(mx, sh), c = group_sum(my, sh), d = group_sum(bad, sh);
+    if (threadIdx.x == 0) {
+        partials[4 * (size_t)blockIdx.x] = a;
+        partials[4 * (size_t)blockIdx.x + 1] = b;
+        partials[4 * (size_t)blockIdx.x + 2] = c;
+        partials[4 * (size_t)blockIdx.x + 3] = d;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_moments_final(int nparts, const double *partials, double *out)
+{
+    __shared__ double sh[16];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nparts; q += blockDim.x)
+        for (int c = 0; c < 4; c++) v[c] += partials[4 * (size_t)q + c];
+    for (int c = 0; c < 4; c++) {
+        const double s = group_sum(v[c], sh);
+        if (threadIdx.x == 0) out[c] = s;
+    }
+}
+
 template <int MODE> __global__ void k_decode(const void *plane, int k, size_t n, double *out)
 {
     typedef typename Codec<MODE>::T T;
@@ -2139,6 +2182,18 @@ int launch_step_diag(void *stream, int mode, const StepDiagArgs &a)
     MBL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, a.ntiles + a.list_wgs, (const double *)a.rparts,
                        a.out);
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_moments(void *stream, int mode, const void *const *d_planes, size_t n, int nparts, double *partials,
+                   double *d_out)
+{
+    if (n == 0 || nparts < 1) return MBL_E_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_moments<M_>), dim3(nparts), dim3(256), 0, s, d_planes, n, partials));
+    MBL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(256), 0, s, nparts, (const double *)partials, d_out);
     MBL_LAUNCH_CHECK();
     return 0;
 }

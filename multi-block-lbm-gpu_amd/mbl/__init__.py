@@ -79,6 +79,7 @@ def load_library(fp64=False):
         "fdrag_store": (None, [AREA, D2Q9, vp, D2Q9, ip]),
         "getDrag_block": (None, [ip, vp, AREA, D2Q9, dp, ip]),
         "mbl_evolution_step_fused": (ip, [vp]),
+        "mbl_block_moments": (ip, [vp, C.POINTER(AREA), C.POINTER(D2Q9), dp]),
         "mbl_evolution_step": (ip, [vp, ip, ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), mbl_dim3,
                                     C.POINTER(mbl_dim3), C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(D2Q9),
                                     C.POINTER(Inf_Ff_store), C.POINTER(Inf_Ff_store), C.POINTER(Inf_Ff_store),
@@ -484,6 +485,14 @@ class PeriodicBlock:
 
     def sync(self):
         check(self.lib, self.lib.mbl_synchronize(self.h), "synchronize")
+
+    def moments(self):
+        """Sum of f, f e_x, f e_y over every node and direction (mass, momentum), and the count
+        of non-finite entries, from a device reduction (FP64)."""
+        out = np.zeros(4)
+        check(self.lib, self.lib.mbl_block_moments(self.h, C.byref(self.area), C.byref(self.pp),
+                                                   out.ctypes.data_as(C.POINTER(C.c_double))), "moments")
+        return out
 
     def read_F(self, k, which="F"):
         n = self.area.nx * self.area.ny

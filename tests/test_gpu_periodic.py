@@ -108,6 +108,27 @@ def test_mass_and_momentum_conserved(lib64):
     assert abs((F1 * ey).sum() - (F0 * ey).sum()) < 1e-12 * NX * NY
 
 
+@pytest.mark.parametrize("fp64,storage", [(True, 0), (False, 0), (False, 2), (False, 5)])
+def test_device_moments_match_host_sums(lib32, lib64, fp64, storage):
+    """mbl_block_moments (the full-size invariant bench.py reports) against the same sums of
+    the planes read back: mass and momentum, before and after 20 steps, and no non-finite
+    entry."""
+    lib = lib64 if fp64 else lib32
+    b = mbl.PeriodicBlock(lib, NX, NY, tau=TAU, storage=storage)
+    ex = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])[:, None, None]
+    ey = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])[:, None, None]
+    for steps in (0, 20):
+        b.step(steps)
+        b.sync()
+        F = np.stack([b.read_F(k) for k in range(9)])
+        m = b.moments()
+        assert m[3] == 0
+        assert abs(m[0] - F.sum()) < 1e-12 * F.sum()
+        assert abs(m[1] - (F * ex).sum()) < 1e-12 * NX * NY
+        assert abs(m[2] - (F * ey).sum()) < 1e-12 * NX * NY
+    b.close()
+
+
 def test_odd_width_periodic(lib32, lib64):
     """nx not a multiple of the vector width falls back to one node per thread."""
     o = oracle.OraclePeriodic(37, 23, tau=TAU)
