@@ -410,9 +410,11 @@ int mbl_divergence_step(mbl_handle_t h);
  * `level` (no reference counterpart): evolution() records the cycle of
  * evolution.h:2-65 in its order, derives every dependence between its
  * operations and merges independent operations of one kind into shared
- * launches (stream-collides of adjacent levels, interface ops, splines).
- * Returns the total and the count per kind (pointers may be NULL), or
- * MBL_E_INVALID before the first evolution() call from that level. */
+ * launches (stream-collides of adjacent levels, interface ops, splines; the
+ * interface ops and one-workgroup splines of one position share a launch).
+ * Returns the number of launches and, per kind, the launches that carry it
+ * (pointers may be NULL; a shared launch counts for both), or MBL_E_INVALID
+ * before the first evolution() call from that level. */
 int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int *interface_ops, int *splines);
 /* Positions of the same schedule: launches at one position have no dependence among them.
  * No reference counterpart. */

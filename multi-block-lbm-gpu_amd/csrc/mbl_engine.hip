@@ -54,7 +54,8 @@ struct CycleItem {
     MarkerSet ms;
 };
 
-// A merged launch of the scheduled cycle: items of one kind with no dependence among them.
+// A merged launch of the scheduled cycle: items of one kind with no dependence among them
+// (kind 3: one-workgroup splines and interface ops of one position in one launch).
 struct CycleLaunch {
     int kind;
     int pos = 0;             // position in the schedule (launches at one position are independent)
@@ -62,6 +63,7 @@ struct CycleLaunch {
     int lo = 0, hi = 0;      // collide: levels lo..hi (adjacent, same nodes per thread)
     uint32_t rmask = 0;      // collide: levels whose last collide of the cycle this launch holds
     int op_off = 0, op_count = 0, max_n = 0, max_nc = 0; // interface/spline: range of the program's op table
+    int nspl = 0, max_nf = 0; // kind 3: its first nspl ops are splines (of up to max_nf fine points)
 };
 
 struct CycleProgram {
@@ -1343,9 +1345,9 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
 // operations touch (collide epochs per level, interface node units, ring slots), gives each
 // operation the earliest position its dependences allow, and merges operations of one kind
 // at one position into one launch (stream-collides of adjacent levels, interface ops of any
-// line, splines of any level). Every read still sees the value it sees in the reference
-// order, so results are bit-identical; the coarse step of a three-level layout needs 14
-// launches instead of 22.
+// line, splines of any level; interface ops and one-workgroup splines of one position in one
+// launch). Every read still sees the value it sees in the reference order, so results are
+// bit-identical; the coarse step of the three-level C2b layout takes 16 launches instead of 22.
 struct CycleState {
     uint32_t pm;
     int rc;
@@ -1551,7 +1553,6 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             }
             tgt->items.push_back(j);
         }
-        for (auto &x : ifl) P.launches.push_back(x);
         // splines: the one-workgroup path and the two-kernel path separately
         CycleLaunch sp[2];
         for (int j = 0; j < n; j++) {
@@ -1563,6 +1564,20 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             t.pos = q;
             t.items.push_back(j);
         }
+        // one interface launch and one-workgroup splines at one position: one launch whose first
+        // workgroups run the splines and the others the interface entries (no dependence among
+        // them, or the schedule would have put them at different positions); with forked
+        // positions they stay two launches, on two streams
+        if (!h->fork && ifl.size() == 1 && !sp[0].items.empty()) {
+            CycleLaunch m;
+            m.kind = 3;
+            m.pos = q;
+            m.items = sp[0].items;
+            m.items.insert(m.items.end(), ifl[0].items.begin(), ifl[0].items.end());
+            ifl.clear();
+            sp[0] = m;
+        }
+        for (auto &x : ifl) P.launches.push_back(x);
         for (auto &t : sp)
             if (!t.items.empty()) P.launches.push_back(t);
     }
@@ -1583,9 +1598,16 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         cl.op_off = (int)tab.size();
         for (int j : cl.items)
             for (int o = items[j].off; o < items[j].off + items[j].count; o++) {
-                tab.push_back(L->hops[o]);
-                cl.max_n = std::max(cl.max_n, L->hops[o].n);
-                cl.max_nc = std::max(cl.max_nc, L->hops[o].nc);
+                const LineOp &op = L->hops[o];
+                tab.push_back(op);
+                if (cl.kind == 3 && op.type == OP_SPLINE) { // the splines come first (items in that order)
+                    cl.nspl++;
+                    cl.max_nf = std::max(cl.max_nf, op.n);
+                    cl.max_nc = std::max(cl.max_nc, op.nc);
+                    continue;
+                }
+                cl.max_n = std::max(cl.max_n, op.n);
+                cl.max_nc = std::max(cl.max_nc, op.nc);
             }
         cl.op_count = (int)tab.size() - cl.op_off;
     }
@@ -1656,10 +1678,13 @@ static int launch_one(mbl_context *h, Layout *L, const std::vector<CycleItem> &i
                                      L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo], rmask,
                                      rmask ? L->dres : nullptr);
     }
-    if (cl.kind == 1) {
+    if (cl.kind == 1 || cl.kind == 3) {
         double xm = 0.0;
         for (int j : cl.items)
             if (items[j].temporal) xm = items[j].xm;
+        if (cl.kind == 3)
+            return launch_iface_mixed(stream, mode, P.dops + cl.op_off, cl.nspl, cl.op_count - cl.nspl, cl.max_nc,
+                                      cl.max_nf, cl.max_n, L->db, pm, ms, xm);
         return launch_line_ops(stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_n, L->db, pm, ms, xm);
     }
     return launch_spline(stream, mode, P.dops + cl.op_off, cl.op_count, cl.max_nc, cl.max_n, L->db, pm, ms);
@@ -2522,12 +2547,12 @@ extern "C" int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int 
     if (!h) h = g_current;
     if (!h) return MBL_E_NO_HANDLE;
     if (!h->layout || level < 0 || level >= MBL_MAX_LEVEL || !h->layout->prog[level].built) return MBL_E_INVALID;
-    int c[3] = {0, 0, 0};
+    int c[4] = {0, 0, 0, 0};
     for (const CycleLaunch &cl : h->layout->prog[level].launches) c[cl.kind]++;
     if (collides) *collides = c[0];
-    if (interface_ops) *interface_ops = c[1];
-    if (splines) *splines = c[2];
-    return c[0] + c[1] + c[2];
+    if (interface_ops) *interface_ops = c[1] + c[3];
+    if (splines) *splines = c[2] + c[3];
+    return c[0] + c[1] + c[2] + c[3];
 }
 
 extern "C" int mbl_cycle_positions(mbl_handle_t h, int level)
@@ -2545,9 +2570,10 @@ extern "C" int mbl_set_cycle_fork(mbl_handle_t h, int on)
     if (!h) return MBL_E_NO_HANDLE;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->fork = on != 0;
-    if (h->layout) { // captured graphs hold the other launch order
+    if (h->layout) { // captured graphs hold the other launch order; the schedule is rebuilt
         for (auto &g : h->layout->graphs) hipGraphExecDestroy(g.second);
         h->layout->graphs.clear();
+        for (auto &p : h->layout->prog) p.built = false;
     }
     return 0;
 }

@@ -213,6 +213,10 @@ int launch_spline_path(int max_nc);
 // each op writes ring slot ms.m[op.mgroup][0]
 int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max_nc, int max_nf,
                   const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms);
+// nspl one-workgroup spline ops (max_nc coarse, max_nf fine points) followed by nops interface ops
+// (up to max_n entries) in one launch
+int launch_iface_mixed(void *stream, int mode, const LineOp *d_ops, int nspl, int nops, int max_nc, int max_nf,
+                       int max_n, const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms, double xm);
 int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,
                      const BlockDesc *d_blocks, uint32_t parity_mask);
 int launch_tri_A(void *stream, Lbd *l, Lbd *m, Lbd *u, int N);

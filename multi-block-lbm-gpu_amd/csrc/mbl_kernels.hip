@@ -1360,6 +1360,26 @@ __global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict_
     spline_line<MODE>(ops[blockIdx.y], blocks, parity_mask, ms, lds);
 }
 
+// One-workgroup splines and interface ops of one position of the scheduled cycle in one
+// launch: workgroup row y < nspl runs spline y (its first workgroup; the line's whole spline
+// step), the rows after it the entries of interface op y (one entry per thread).
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__ ops, int nspl,
+                                                      const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
+                                                      MarkerSet ms, double xm)
+{
+    extern __shared__ double lds[];
+    const int y = (int)blockIdx.y;
+    if (y < nspl) {
+        if (blockIdx.x == 0) spline_line<MODE>(ops[y], blocks, parity_mask, ms, lds);
+        return;
+    }
+    const LineOp &op = ops[y];
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= op.n) return;
+    line_op_entry<MODE>(op, e, blocks, parity_mask, ms, xm);
+}
+
 // init_tri_A (multi/init_multi.h:30-51), final race-free state
 __global__ void k_tri_A(Lbd *l, Lbd *m, Lbd *u, int N)
 {
@@ -2118,6 +2138,20 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
     MBL_LAUNCH_CHECK();
     dim3 g((max_nf + 255) / 256, nops), b(256);
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_eval<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, ms));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_iface_mixed(void *stream, int mode, const LineOp *d_ops, int nspl, int nops, int max_nc, int max_nf,
+                       int max_n, const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms, double xm)
+{
+    if (nspl <= 0 || max_nc > SPL_FUSED_MAX_NC) return MBL_E_INVALID;
+    const int threads = std::min(1024, (std::max(max_nc, max_nf) + 63) / 64 * 64);
+    const size_t lds = 24 * sizeof(double) * (size_t)max_nc;
+    dim3 g(std::max(1, (max_n + threads - 1) / threads), nspl + std::max(0, nops)), b(threads);
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_iface_mixed<M_>), g, b, lds, s, d_ops, nspl, d_blocks, parity_mask, ms,
+                                             xm));
     MBL_LAUNCH_CHECK();
     return 0;
 }
