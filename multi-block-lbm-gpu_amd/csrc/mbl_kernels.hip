@@ -1598,10 +1598,10 @@ __global__ __launch_bounds__(256) void k_residual_cases(const int *__restrict__ 
 // wrote its tiles' residual sums (error_L2.h:57-91 and force_block.h:180-216).
 // Workgroups [0, list_wgs) take error_L2's terms of the nodes the collides do not sum (block
 // edges, solid and ghost nodes: their macros are final once the cycle is done), u_prev <- u,
-// into their partial-sum slots after the tiles'; workgroup list_wgs (with a drag) sums the
-// momentum exchange. k_residual_final then sums every slot in a fixed order (a second launch:
-// within one launch, workgroups on different XCDs would need a write-back of the L2 to see
-// each other's partial sums).
+// plus one contiguous share of the tiles' partial sums, into their slots after the tiles';
+// workgroup list_wgs (with a drag) sums the momentum exchange. k_residual_final then sums the
+// list_wgs slots in a fixed order (a second launch: within one launch, workgroups on
+// different XCDs would need a write-back of the L2 to see each other's partial sums).
 template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagArgs a)
 {
     __shared__ double sh[16];
@@ -1620,6 +1620,13 @@ template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagA
             acc.s2 = (x * x + y * y) * d.inv_m;
             d.upx[o] = ux;
             d.upy[o] = uy;
+        }
+        const int chunk = (a.ntiles + a.list_wgs - 1) / a.list_wgs;
+        const int t1 = min(a.ntiles, (wg + 1) * chunk);
+        for (int q = wg * chunk + (int)threadIdx.x; q < t1; q += (int)blockDim.x) {
+            acc.s1 += a.rparts[3 * (size_t)q];
+            acc.s2 += a.rparts[3 * (size_t)q + 1];
+            acc.mn = fmin(acc.mn, a.rparts[3 * (size_t)q + 2]);
         }
         tile_resid_store(acc, 1.0, a.rparts + 3 * ((size_t)a.ntiles + (size_t)wg));
     } else {
@@ -2214,8 +2221,8 @@ int launch_step_diag(void *stream, int mode, const StepDiagArgs &a)
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_step_diag<M_>), dim3(grid), dim3(256), 0, s, a));
     MBL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, a.ntiles + a.list_wgs, (const double *)a.rparts,
-                       a.out);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, a.list_wgs,
+                       (const double *)a.rparts + 3 * (size_t)a.ntiles, a.out);
     MBL_LAUNCH_CHECK();
     return 0;
 }
