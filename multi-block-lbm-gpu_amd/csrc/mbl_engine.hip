@@ -778,6 +778,8 @@ int pcr_factor_table(const double *l, const double *m, const double *u, int N, s
 // ---------------------------------------------------------------------------
 struct OpBuild {
     LineOp op;
+    int uid = -1;  // serial number (c_f ops named by the redirected line ends of f_c ops)
+    int red_uid[2] = {-1, -1};
     int line = -1; // coarse/fine line whose ring the op writes (spline) or reads (temporal, c_f)
     std::vector<int> src, dst;
     std::vector<uint8_t> mask;
@@ -1164,6 +1166,55 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         return ac;
     };
     // Batches, each scheduled on its own; ops appended to `all` grouped by sub-batch.
+    int next_uid = 0;
+    // A c_f / f_c batch (evolution.h:52-62, line by line): an f_c op reads fine nodes that the
+    // c_f op of another line of the batch writes where the lines meet (the ends of the f_c
+    // line on the fine block's ghost edges). Those reads would split the batch into two
+    // launches; instead each such entry reads the value the reference order gives it: what an
+    // earlier c_f op wrote (its ring slot), or, before a later c_f op writes the node, the
+    // ghost value the last collide carried over (the other plane set; ghost nodes only). At
+    // most two such entries per f_c op; otherwise the reads stay and the batch splits.
+    auto redirect_fc_ends = [&](std::vector<OpBuild> &ops) {
+        for (auto &o : ops) o.uid = next_uid++;
+        std::unordered_map<uint64_t, std::vector<std::pair<int, int>>> cfw; // unit -> (op, entry) writers
+        for (size_t j = 0; j < ops.size(); j++)
+            if (ops[j].op.type == OP_CF)
+                for (size_t e = 0; e < ops[j].dst.size(); e++)
+                    if (ops[j].mask[e] & 2) cfw[unit(ops[j].op.dst_block, (size_t)ops[j].dst[e], 1)].push_back({(int)j, (int)e});
+        for (size_t j = 0; j < ops.size(); j++) {
+            OpBuild &o = ops[j];
+            if (o.op.type != OP_FC) continue;
+            struct Red { int e, t, op, ce; uint64_t u; };
+            std::vector<Red> red;
+            bool ok = true;
+            for (size_t e = 0; e < o.src.size() && ok; e++) {
+                const uint64_t u = unit(o.op.src_block, (size_t)o.src[e], 1);
+                auto it = cfw.find(u);
+                if (it == cfw.end()) continue;
+                int before = -1, before_e = -1;
+                for (auto &w : it->second)
+                    if (w.first < (int)j) {
+                        before = w.first;
+                        before_e = w.second;
+                    }
+                if (before >= 0)
+                    red.push_back({(int)e, 1, before, before_e, u});
+                else if (hflags[o.op.src_block][o.src[e]] == C_BOUNDARY)
+                    red.push_back({(int)e, 2, -1, -1, u});
+                else
+                    ok = false;
+            }
+            if (!ok || red.size() > 2 || red.empty()) continue;
+            o.op.red_n = (int)red.size();
+            for (size_t r = 0; r < red.size(); r++) {
+                o.op.red_e[r] = red[r].e;
+                o.op.red_t[r] = red[r].t;
+                o.op.red_ce[r] = red[r].ce;
+                o.red_uid[r] = red[r].op >= 0 ? ops[red[r].op].uid : -1;
+                o.rd.erase(std::remove(o.rd.begin(), o.rd.end(), red[r].u), o.rd.end());
+            }
+        }
+    };
     auto add_batch = [&](std::vector<OpBuild> ops, Batch &batch) -> bool {
         for (auto &o : ops)
             if (o.op.n < 0) return false;
@@ -1204,6 +1255,9 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
                 ops.push_back(make_T_or_CF(b, OP_CF));
                 ops.push_back(make_FC(b));
             }
+        for (auto &o : ops)
+            if (o.op.n < 0) return fail("coarse/fine transfer line outside its blocks");
+        redirect_fc_ends(ops);
         if (!add_batch(std::move(ops), L->bCF[l])) return fail("coarse/fine transfer line outside its blocks");
     }
     // PCR factor tables of the spline matrices, read once from the caller's l, m, u (init_tri_A
@@ -1264,9 +1318,19 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     std::vector<uint8_t> hmask;
     hmask.reserve(nmask);
     size_t mo = 0;
+    std::unordered_map<int, size_t> uid_index;
+    for (size_t j = 0; j < all.size(); j++)
+        if (all[j].uid >= 0) uid_index[all[j].uid] = j;
     for (size_t j = 0; j < all.size(); j++) {
         const OpBuild &o = all[j];
         LineOp op = o.op;
+        for (int r = 0; r < op.red_n; r++)
+            if (op.red_t[r] == 1) {
+                const LineOp &cf = all[uid_index.at(o.red_uid[r])].op;
+                op.red_mg[r] = cf.mgroup;
+                for (int sl = 0; sl < 3; sl++)
+                    for (int k = 0; k < MBL_Q; k++) op.red_ring[r][sl][k] = cf.ring[sl][k];
+            }
         op.src_idx = o.src.empty() ? nullptr : L->didx + hidx.size();
         hidx.insert(hidx.end(), o.src.begin(), o.src.end());
         op.dst_idx = o.dst.empty() ? nullptr : L->didx + hidx.size();

@@ -99,6 +99,14 @@ struct LineOp {
     // pcr[2 pcr_levels nc + i]. pcr_levels: the levels after which every row is decoupled.
     const double *pcr;
     int pcr_levels;
+    // FC: entries whose fine node a c_f op of the same batch writes (line ends on the fine
+    // block's ghost edges), read without the node so that the batch is one launch: type 1, the
+    // value that c_f op wrote before (its ring slot, entry red_ce); type 2, the value before
+    // that c_f op writes it (the ghost as the last collide carried it over: the other plane set)
+    int red_n;
+    int red_e[2], red_t[2], red_ce[2];
+    int red_mg[2];                    // type 1: the c_f op's marker group
+    const Lbd *red_ring[2][3][MBL_Q]; // type 1: the c_f op's ring slots
 };
 
 // Most PCR levels the one-workgroup spline keeps in registers (rows decouple after 6 levels for

@@ -1025,8 +1025,29 @@ __device__ __forceinline__ void line_op_entry(const LineOp &op, int e, const Blo
     const int ps = (parity_mask >> sb.level) & 1;
     const int sidx = op.src_idx[e];
     T sv[9];
+    int rt = 0;
+    if (op.red_n > 0 && e == op.red_e[0]) rt = 1;
+    if (op.red_n > 1 && e == op.red_e[1]) rt = 2;
+    if (rt) { // a line end read elsewhere (LineOp::red_*): the same bits the node holds in the reference order
+        const bool r1 = (rt == 2);
+        const int t = r1 ? op.red_t[1] : op.red_t[0];
+        if (t == 1) {
+            // both redirects' ring pointers come with the op (uniform), one is selected per lane
+            const int mc0 = ms.m[op.red_mg[0]][2], mc1 = ms.m[op.red_mg[1]][2];
+            const int ce = r1 ? op.red_ce[1] : op.red_ce[0];
 #pragma unroll
-    for (int k = 0; k < 9; k++) sv[k] = ((const T *)sb.P[ps][k])[sidx];
+            for (int k = 0; k < 9; k++) {
+                const Lbd *p = r1 ? op.red_ring[1][mc1][k] : op.red_ring[0][mc0][k];
+                sv[k] = enc64<MODE>((double)p[ce], k);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) sv[k] = ((const T *)sb.P[ps ^ 1][k])[sidx];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) sv[k] = ((const T *)sb.P[ps][k])[sidx];
+    }
     const Lbd r = sb.rho[sidx], ux = sb.ux[sidx], uy = sb.uy[sidx];
     if (mask & 1) {
         db.rho[didx] = r;
