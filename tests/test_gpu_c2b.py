@@ -25,6 +25,28 @@ LONG = 50000
 START = LONG // 2
 
 
+def test_c2b_schedule_shape(lib32):
+    """The C2b coarse step as scheduled: 22 operations of the reference order (7 collides, 15
+    interface batches) in 14 positions of one launch each -- collides of adjacent levels share a
+    launch, the c_f / f_c batches are one launch each, interface ops and splines of one position
+    share one. With forked positions the launches that share a position stay apart."""
+    import ctypes as C
+    geom = mbl.c2b_geometry(lib32)
+    for fork, want_launches in ((0, 14), (1, 16)):
+        sim = mbl.MultiBlockSim(lib32, geom)
+        mbl.check(lib32, lib32.mbl_set_cycle_fork(sim.h, fork), "fork")
+        sim.step(2)
+        col, ifc, spl = C.c_int(), C.c_int(), C.c_int()
+        total = lib32.mbl_cycle_launches(sim.h, 0, C.byref(col), C.byref(ifc), C.byref(spl))
+        pos = lib32.mbl_cycle_positions(sim.h, 0)
+        print("fork", fork, "launches", total, "positions", pos, "collide/interface/spline", col.value, ifc.value,
+              spl.value)
+        assert pos == 14
+        assert total == want_launches
+        assert col.value == 4 and spl.value == 3
+        sim.close()
+
+
 def test_c2b_fp64_matches_oracle(lib64):
     geom = mbl.c2b_geometry(lib64)
     _, _, lc, _, _ = geom.init(lib64)
