@@ -112,9 +112,11 @@ def test_geo_4_9_reference_mode_fp64(lib64):
 
 
 def test_tiny2_fp32_close_to_oracle(lib32):
+    """FP32 planes (FP64 arithmetic) against the FP64 oracle over 40 coarse steps: macros within
+    1e-4 of their scale (measured 2.5e-5: the planes' FP32 rounding), F within 5e-6."""
     geom = tiny2(lib32)
     sim, o = run_pair(lib32, geom, 40)
-    worst = assert_fields_close(sim, o, 5e-4, 5e-6)
+    worst = assert_fields_close(sim, o, 1e-4, 5e-6)
     print("fp32 worst field rel err", worst)
     sim.close()
 
