@@ -184,6 +184,11 @@ struct mbl_context {
     double *h_l2 = nullptr, *h_drag = nullptr;
     std::vector<int> pend_l2;
     std::vector<PendingDrag> pend_drag;
+    // records of mbl_evolution_step whose final sum waits for the flush: (slot, partial sums);
+    // slot s keeps its partials at d_lparts + 3 lparts_stride s
+    std::vector<std::pair<int, int>> pend_final;
+    double *d_lparts = nullptr;
+    int lparts_stride = 0;
     std::vector<int> ser_l2_t, ser_drag_t;
     std::vector<double> ser_l2, ser_cd, ser_cl;
     double last[5] = {0, 0, 0, 0, 0};
@@ -380,6 +385,7 @@ extern "C" void mbl_handle_destroy(mbl_handle_t h)
     release_resid(h);
     if (h->d_l2) hipFree(h->d_l2);
     if (h->d_drag) hipFree(h->d_drag);
+    if (h->d_lparts) hipFree(h->d_lparts);
     if (h->h_l2) hipHostFree(h->h_l2);
     if (h->h_drag) hipHostFree(h->h_drag);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -2252,6 +2258,22 @@ extern "C" int mbl_flush_diagnostics(mbl_handle_t h)
     if (!h) return MBL_E_NO_HANDLE;
     const size_t nl = h->pend_l2.size(), nd = h->pend_drag.size();
     if (!nl && !nd) return 0;
+    // the deferred final sums, one launch per run of consecutive slots with the same partial count
+    for (size_t i = 0; i < h->pend_final.size();) {
+        size_t j = i + 1;
+        while (j < h->pend_final.size() && h->pend_final[j].first == h->pend_final[j - 1].first + 1 &&
+               h->pend_final[j].second == h->pend_final[i].second)
+            j++;
+        const int s0 = h->pend_final[i].first;
+        const int rc = launch_residual_steps(h->stream, (int)(j - i), h->pend_final[i].second, (size_t)h->lparts_stride,
+                                             h->d_lparts + 3 * (size_t)h->lparts_stride * s0, h->d_l2 + L2_REC * s0);
+        if (rc) {
+            h->pend_final.clear();
+            return set_err(h, rc);
+        }
+        i = j;
+    }
+    h->pend_final.clear();
     if (nl) HIPCHK(h, hipMemcpyAsync(h->h_l2, h->d_l2, sizeof(double) * L2_REC * nl, hipMemcpyDeviceToHost, h->stream));
     if (nd) HIPCHK(h, hipMemcpyAsync(h->h_drag, h->d_drag, sizeof(double) * 2 * nd, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -2534,7 +2556,20 @@ extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *a
         d.list_wgs = std::max(1, (L->nlist + 255) / 256);
         d.ntiles = L->total_tiles;
         d.rparts = L->dres;
-        d.out = h->d_l2 + L2_REC * h->pend_l2.size();
+        if (d.list_wgs > h->lparts_stride) { // room for the list partials of every record slot
+            mbl_flush_diagnostics(h);
+            if (h->last_error) break;
+            if (h->d_lparts) hipFree(h->d_lparts);
+            h->d_lparts = nullptr;
+            h->lparts_stride = 0;
+            if (hipMalloc(&h->d_lparts, sizeof(double) * 3 * (size_t)d.list_wgs * h->cap) != hipSuccess) {
+                set_err(h, MBL_E_NOMEM);
+                break;
+            }
+            h->lparts_stride = d.list_wgs;
+        }
+        const int slot = (int)h->pend_l2.size();
+        d.lout = h->d_lparts + 3 * (size_t)h->lparts_stride * slot;
         if (N_drag > 0) {
             for (int k = 0; k < 9; k++) d.drag.Fold[k] = (void *)pps_d[last].f[k];
             d.drag.links = d_pdrag;
@@ -2547,6 +2582,7 @@ extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *a
             break;
         }
         h->pend_l2.push_back(t);
+        h->pend_final.push_back({slot, d.list_wgs});
         if (N_drag > 0) h->pend_drag.push_back(PendingDrag{t, areas[last].m});
         if (cfg.sync_diag || (int)h->pend_l2.size() >= h->cap || (int)h->pend_drag.size() >= h->cap)
             mbl_flush_diagnostics(h);

@@ -134,8 +134,8 @@ struct StepDiagArgs {
     int nlist;
     int list_wgs;      // workgroups over the list (256 entries each, at least one)
     int ntiles;        // stream-collide tiles of all levels: their sums at rparts[0, 3 ntiles)
-    double *rparts;    // 3 (ntiles + list_wgs) partial sums
-    double *out;       // error_L2 record: residual, s1, s2, smallest density
+    double *rparts;    // the tiles' 3 ntiles partial sums
+    double *lout;      // the list workgroups' 3 list_wgs partial sums (launch_residual_steps sums them)
     DragArgs drag;     // drag.n == 0: no drag
 };
 
@@ -237,6 +237,9 @@ int launch_residual_cases(void *stream, const BlockDesc *d_descs, const int *d_t
                           double *partials, int ncases, const int *d_case_tiles, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_step_diag(void *stream, int mode, const StepDiagArgs &a);
+// error_L2 records of nsteps steps: step b sums partials[3 (b stride) + 3 q], q < nparts, in
+// order into out[4 b .. 4 b + 3] (residual, s1, s2, smallest density), as k_residual_final does
+int launch_residual_steps(void *stream, int nsteps, int nparts, size_t stride, const double *partials, double *out);
 int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases, double *out);
 int launch_stream_copy(void *stream, const void *src, void *dst, size_t bytes);
 // 9 planes of plane_bytes (consecutive in src and in dst) copied plane by plane

@@ -1571,9 +1571,9 @@ __global__ __launch_bounds__(256) void k_residual_tiles(const BlockDesc *__restr
 
 // The weighted tile sums in one fixed order: result sqrt(s1)/sqrt(s2), s1, s2,
 // and the smallest density of any node (a divergence check: rho <= 0).
-__global__ __launch_bounds__(1024) void k_residual_final(int total, const double *__restrict__ partials, double *out)
+__device__ __forceinline__ void residual_record(int total, const double *__restrict__ partials, double *out,
+                                                double *sh)
 {
-    __shared__ double sh[16];
     double g1 = 0.0, g2 = 0.0, mn = INFINITY;
     for (int q = threadIdx.x; q < total; q += blockDim.x) {
         g1 += partials[3 * (size_t)q];
@@ -1589,6 +1589,20 @@ __global__ __launch_bounds__(1024) void k_residual_final(int total, const double
         out[2] = all2;
         out[3] = allmn;
     }
+}
+
+__global__ __launch_bounds__(1024) void k_residual_final(int total, const double *__restrict__ partials, double *out)
+{
+    __shared__ double sh[16];
+    residual_record(total, partials, out, sh);
+}
+
+// The records of consecutive steps, one workgroup each (the same order as k_residual_final).
+__global__ __launch_bounds__(1024) void k_residual_steps(int total, size_t stride, const double *__restrict__ partials,
+                                                         double *out)
+{
+    __shared__ double sh[16];
+    residual_record(total, partials + 3 * stride * blockIdx.x, out + 4 * (size_t)blockIdx.x, sh);
 }
 
 // Per-case residual of a batch: case c's residual tiles are [case_tiles[c], case_tiles[c+1]),
@@ -1620,9 +1634,10 @@ __global__ __launch_bounds__(256) void k_residual_cases(const int *__restrict__ 
 // Workgroups [0, list_wgs) take error_L2's terms of the nodes the collides do not sum (block
 // edges, solid and ghost nodes: their macros are final once the cycle is done), u_prev <- u,
 // plus one contiguous share of the tiles' partial sums, into their slots after the tiles';
-// workgroup list_wgs (with a drag) sums the momentum exchange. k_residual_final then sums the
-// list_wgs slots in a fixed order (a second launch: within one launch, workgroups on
-// different XCDs would need a write-back of the L2 to see each other's partial sums).
+// workgroup list_wgs (with a drag) sums the momentum exchange. The list_wgs partial sums of
+// each step stay in a slot of their own until the diagnostics are read back, when one
+// launch sums every step's slots in a fixed order (k_residual_steps: within one launch,
+// workgroups on different XCDs would need a write-back of the L2 to see each other's sums).
 template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagArgs a)
 {
     __shared__ double sh[16];
@@ -1649,7 +1664,7 @@ template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagA
             acc.s2 += a.rparts[3 * (size_t)q + 1];
             acc.mn = fmin(acc.mn, a.rparts[3 * (size_t)q + 2]);
         }
-        tile_resid_store(acc, 1.0, a.rparts + 3 * ((size_t)a.ntiles + (size_t)wg));
+        tile_resid_store(acc, 1.0, a.lout + 3 * (size_t)wg);
     } else {
         drag_case<MODE>(a.drag, a.drag.out, sh);
     }
@@ -2237,13 +2252,20 @@ int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases
 
 int launch_step_diag(void *stream, int mode, const StepDiagArgs &a)
 {
-    if (a.list_wgs < 1 || a.ntiles < 0 || !a.rparts || !a.out) return MBL_E_INVALID;
+    if (a.list_wgs < 1 || a.ntiles < 0 || !a.rparts || !a.lout) return MBL_E_INVALID;
     const int grid = a.list_wgs + (a.drag.n > 0 ? 1 : 0);
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_step_diag<M_>), dim3(grid), dim3(256), 0, s, a));
     MBL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, s, a.list_wgs,
-                       (const double *)a.rparts + 3 * (size_t)a.ntiles, a.out);
+    return 0;
+}
+
+int launch_residual_steps(void *stream, int nsteps, int nparts, size_t stride, const double *partials, double *out)
+{
+    if (nsteps <= 0) return 0;
+    if (nparts < 1 || stride < (size_t)nparts || !partials || !out) return MBL_E_INVALID;
+    hipLaunchKernelGGL(k_residual_steps, dim3(nsteps), dim3(1024), 0, (hipStream_t)stream, nparts, stride, partials,
+                       out);
     MBL_LAUNCH_CHECK();
     return 0;
 }

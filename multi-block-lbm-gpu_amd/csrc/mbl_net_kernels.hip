@@ -146,7 +146,7 @@ __device__ inline int fdiv(int v, int d, float inv)
 // tap-shifted X[.][ci0..ci0+31] with one 16-byte load per thread and operand,
 // stores both transposed in LDS ([channel][pixel], so each lane's 8 K values
 // are one 16-byte LDS read), and each wave runs two MFMA k-steps (K = 32
-// pixels) on its 16 x 16 sub-tile. The workgroups of the first tap and input
+// pixels) on its 16 x 16 sub-tile while the next stage's loads are in flight. The workgroups of the first tap and input
 // channel tile also sum the dY values they load: the bias gradient
 // db[co] = sum_p dY[p][co], written after the weights' partials (the bias
 // follows the weights in the parameter vector), so one ordered reduction
@@ -157,9 +157,12 @@ constexpr int WG_LD = WG_PX + 8; // LDS row length (bf16): rows stay 16-byte ali
 
 __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sA[32 * WG_LD]; // dY^T
-    __shared__ __attribute__((aligned(16))) bf16 sB[32 * WG_LD]; // X^T (tap-shifted)
-    __shared__ float sBias[64][33];                              // per pixel lane: its 32 co sums
+    // two LDS buffers: stage s fills one while the waves may still read stage s-1's from the
+    // other, so one barrier per stage; the next stage's global loads are in flight during this
+    // stage's MFMAs
+    __shared__ __attribute__((aligned(16))) bf16 sA[2][32 * WG_LD]; // dY^T
+    __shared__ __attribute__((aligned(16))) bf16 sB[2][32 * WG_LD]; // X^T (tap-shifted)
+    __shared__ float sBias[64][33];                                 // per pixel lane: its 32 co sums
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int taps = a.ksize * a.ksize, half = a.ksize / 2;
@@ -179,9 +182,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const bool bias = (blockIdx.y == 0); // tap 0, input channels 0..31
     float bs[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int s0 = pb; s0 < pe; s0 += WG_PX) {
+    auto fetch = [&](int s0, uint4 &va, uint4 &vb) {
         const int p = s0 + lp;
-        uint4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
+        va = uint4{0u, 0u, 0u, 0u};
+        vb = uint4{0u, 0u, 0u, 0u};
         if (p < pe) {
             va = *reinterpret_cast<const uint4 *>(a.dY + (size_t)p * a.ldd + a.doff + co0 + lc);
             const int n = fdiv(p, HW, invHW), rem = p - n * HW;
@@ -191,25 +195,31 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
                 vb = *reinterpret_cast<const uint4 *>(a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff +
                                                       ci0 + lc);
         }
+    };
+    uint4 va, vb;
+    fetch(pb, va, vb);
+    int buf = 0;
+    for (int s0 = pb; s0 < pe; s0 += WG_PX) {
         const bf16 *ea = reinterpret_cast<const bf16 *>(&va);
         const bf16 *eb = reinterpret_cast<const bf16 *>(&vb);
         if (bias) {
 #pragma unroll
             for (int i = 0; i < 8; i++) bs[i] += bf2f(ea[i]);
         }
-        __syncthreads(); // the previous stage's reads are done
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            sA[(lc + i) * WG_LD + lp] = ea[i];
-            sB[(lc + i) * WG_LD + lp] = eb[i];
+            sA[buf][(lc + i) * WG_LD + lp] = ea[i];
+            sB[buf][(lc + i) * WG_LD + lp] = eb[i];
         }
         __syncthreads();
+        if (s0 + WG_PX < pe) fetch(s0 + WG_PX, va, vb);
 #pragma unroll
         for (int ks = 0; ks < WG_PX / 32; ks++) {
-            const bf16x8 A = load8(sA + (wco + r16) * WG_LD + ks * 32 + 8 * g);
-            const bf16x8 B = load8(sB + (wci + r16) * WG_LD + ks * 32 + 8 * g);
+            const bf16x8 A = load8(sA[buf] + (wco + r16) * WG_LD + ks * 32 + 8 * g);
+            const bf16x8 B = load8(sB[buf] + (wci + r16) * WG_LD + ks * 32 + 8 * g);
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
         }
+        buf ^= 1;
     }
     // acc[r] = D[co = co0 + wco + 4g + r][ci = ci0 + wci + r16]
     const size_t nw = (size_t)a.Cout * taps * a.Cin;
