@@ -1256,30 +1256,40 @@ __global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ 
 }
 
 // The whole spline step of one line in one workgroup (lines up to
-// SPL_FUSED_MAX_NC coarse points): gather the 12 columns (init_Spline3,
-// assit_multi.h:48-116), solve tridiag(l, m, u) M = rhs for all 12 by PCR at
-// once, and evaluate the fine points (calculate_Spline3_gpu, :136-185) from
+// SPL_FUSED_MAX_NC coarse points): gather the columns (init_Spline3,
+// assit_multi.h:48-116), solve tridiag(l, m, u) M = rhs for all of them by PCR
+// at once, and evaluate the fine points (calculate_Spline3_gpu, :136-185) from
 // the columns and M held in LDS. The matrix is the same at every step, so the
 // elimination factors of every PCR level are computed once per layout
-// (LineOp::pcr): a level is one LDS exchange of the 12 right-hand sides and two
+// (LineOp::pcr): a level is one LDS exchange of the right-hand sides and two
 // multiply-adds per column, with no division and no vote. The fine points'
 // write masks and node indices are loaded before the gather, off the path of
 // the dependent loads. Same operations, in the same order per column, as
 // k_spline_solve + k_spline_eval. One coarse row per thread; blockDim.x >= nc.
+// NG > 1 spreads a line over NG workgroups (grp = 0 .. NG-1): each solves the
+// rho, ux, uy columns (every fine point's equilibrium needs them) and the
+// distributions k = grp, grp + NG, ..., and writes those; grp 0 also writes the
+// macros. A column's arithmetic does not depend on the others, so every split
+// gives the same bits.
 constexpr int SPL_FUSED_MAX_NC = 303; // 24 * 8 * nc bytes of LDS within 64 KiB (and the drag's)
+constexpr int SPL_GROUPS = 9;         // workgroups per line of the spline launches
 
-template <int MODE>
+template <int MODE, int NG = 1>
 __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *__restrict__ blocks,
-                                            uint32_t parity_mask, const MarkerSet &ms, double *lds)
+                                            uint32_t parity_mask, const MarkerSet &ms, double *lds, int grp = 0)
 {
+    static_assert(NG == 1 || NG == 3 || NG == 9, "the 9 distributions split evenly");
+    constexpr int NK = 9 / NG, NCOL = 3 + NK; // distributions and columns of this workgroup
     typedef typename Codec<MODE>::T T;
     const int slot = ms.m[op.mgroup][0]; // the oldest slot of the line's ring receives t_{n+1}
     const int N = op.nc, NF = op.n, NL = op.pcr_levels;
     const BlockDesc &sb = blocks[op.src_block];
     const BlockDesc &db = blocks[op.dst_block];
     const int ps = (parity_mask >> sb.level) & 1;
-    // lds: G[12][N] | d[12][N]
-    double *G = lds, *Ld = lds + 12 * N;
+    // column c of this workgroup is global column gc(c): rho, ux, uy, then F_k for its k
+    auto gc = [&](int c) { return c < 3 ? c : 3 + grp + NG * (c - 3); };
+    // lds: G[NCOL][N] | d[NCOL][N]
+    double *G = lds, *Ld = lds + NCOL * N;
     const int ip = threadIdx.x;
     const bool row = ip < N;
     // the first fine point of this thread: its write mask and node index, loaded now
@@ -1296,55 +1306,58 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
         }
         bd = op.pcr[(size_t)(2 * NL) * N + ip];
     }
-    double d[12];
+    double d[NCOL];
     if (row) {
         const int node = op.src_idx[ip];
-        double g[12];
+        double g[NCOL];
         g[0] = (double)sb.rho[node];
         g[1] = (double)sb.ux[node];
         g[2] = (double)sb.uy[node];
 #pragma unroll
-        for (int k = 0; k < 9; k++) g[3 + k] = dec64<MODE>(((const T *)sb.P[ps][k])[node], k);
+        for (int c = 3; c < NCOL; c++) {
+            const int k = gc(c) - 3;
+            g[c] = dec64<MODE>(((const T *)sb.P[ps][k])[node], k);
+        }
 #pragma unroll
-        for (int q = 0; q < 12; q++) {
-            G[q * N + ip] = g[q];
-            op.gcol[q][ip] = (Lbd)g[q];
+        for (int c = 0; c < NCOL; c++) {
+            G[c * N + ip] = g[c];
+            if (c >= 3 || grp == 0) op.gcol[gc(c)][ip] = (Lbd)g[c];
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 12; q++)
-        d[q] = (!row || ip == 0 || ip == N - 1)
+    for (int c = 0; c < NCOL; c++)
+        d[c] = (!row || ip == 0 || ip == N - 1)
                    ? 0.0
-                   : (6.0 * G[q * N + ip] - 3.0 * G[q * N + ip - 1] - 3.0 * G[q * N + ip + 1]);
+                   : (6.0 * G[c * N + ip] - 3.0 * G[c * N + ip - 1] - 3.0 * G[c * N + ip + 1]);
 #pragma unroll
     for (int lv = 0; lv < PCR_MAX_LEVELS; lv++) {
         if (lv >= NL) continue; // NL is the same for the whole workgroup
         const int s = 1 << lv;
         if (row) {
 #pragma unroll
-            for (int q = 0; q < 12; q++) Ld[q * N + ip] = d[q];
+            for (int c = 0; c < NCOL; c++) Ld[c * N + ip] = d[c];
         }
         __syncthreads();
         if (row) {
             const int im = ip - s, iq = ip + s;
             if (im >= 0) {
 #pragma unroll
-                for (int q = 0; q < 12; q++) d[q] += al[lv] * Ld[q * N + im];
+                for (int c = 0; c < NCOL; c++) d[c] += al[lv] * Ld[c * N + im];
             }
             if (iq < N) {
 #pragma unroll
-                for (int q = 0; q < 12; q++) d[q] += ga[lv] * Ld[q * N + iq];
+                for (int c = 0; c < NCOL; c++) d[c] += ga[lv] * Ld[c * N + iq];
             }
         }
         __syncthreads(); // this level's reads before the next level's writes
     }
     if (row) {
 #pragma unroll
-        for (int q = 0; q < 12; q++) {
-            const double M = d[q] / bd;
-            Ld[q * N + ip] = M;
-            op.varibs[(size_t)q * N + ip] = (Lbd)M;
+        for (int c = 0; c < NCOL; c++) {
+            const double M = d[c] / bd;
+            Ld[c * N + ip] = M;
+            if (c >= 3 || grp == 0) op.varibs[(size_t)gc(c) * N + ip] = (Lbd)M;
         }
     }
     __syncthreads();
@@ -1356,15 +1369,15 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
         const double rho = spline3_s(xx, I - 1.0, 1.0, G[I - 1], G[I], M[I - 1], M[I]);
         const double ux = spline3_s(xx, I - 1.0, 1.0, G[N + I - 1], G[N + I], M[N + I - 1], M[N + I]);
         const double uy = spline3_s(xx, I - 1.0, 1.0, G[2 * N + I - 1], G[2 * N + I], M[2 * N + I - 1], M[2 * N + I]);
-        if (mk & 1) {
+        if ((mk & 1) && grp == 0) {
             const int idx = f == f0 ? di0 : op.dst_idx[f];
             db.rho[idx] = (Lbd)rho;
             db.ux[idx] = (Lbd)ux;
             db.uy[idx] = (Lbd)uy;
         }
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const int o = (3 + k) * N;
+        for (int c = 3; c < NCOL; c++) {
+            const int k = gc(c) - 3, o = c * N;
             const double ft = spline3_s(xx, I - 1.0, 1.0, G[o + I - 1], G[o + I], M[o + I - 1], M[o + I]);
             const double fe = feq64(k, rho, ux, uy);
             op.ring[slot][k][f] = (Lbd)(fe + op.para * (ft - fe));
@@ -1378,12 +1391,12 @@ __global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict_
                                                        MarkerSet ms)
 {
     extern __shared__ double lds[];
-    spline_line<MODE>(ops[blockIdx.y], blocks, parity_mask, ms, lds);
+    spline_line<MODE, SPL_GROUPS>(ops[blockIdx.y], blocks, parity_mask, ms, lds, (int)blockIdx.x);
 }
 
 // One-workgroup splines and interface ops of one position of the scheduled cycle in one
-// launch: workgroup row y < nspl runs spline y (its first workgroup; the line's whole spline
-// step), the rows after it the entries of interface op y (one entry per thread).
+// launch: workgroup row y < nspl runs spline y (its first SPL_GROUPS workgroups, one share of
+// the line's columns each), the rows after it the entries of interface op y (one entry per thread).
 template <int MODE>
 __global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__ ops, int nspl,
                                                       const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
@@ -1392,7 +1405,7 @@ __global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__
     extern __shared__ double lds[];
     const int y = (int)blockIdx.y;
     if (y < nspl) {
-        if (blockIdx.x == 0) spline_line<MODE>(ops[y], blocks, parity_mask, ms, lds);
+        if (blockIdx.x < SPL_GROUPS) spline_line<MODE, SPL_GROUPS>(ops[y], blocks, parity_mask, ms, lds, (int)blockIdx.x);
         return;
     }
     const LineOp &op = ops[y];
@@ -2164,9 +2177,9 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
     if (max_nc <= SPL_FUSED_MAX_NC) {
         // one thread per coarse row, and per fine point where the workgroup allows (up to 1024)
         const int threads = std::min(1024, (std::max(max_nc, max_nf) + 63) / 64 * 64);
-        const size_t lds = 24 * sizeof(double) * (size_t)max_nc;
+        const size_t lds = 2 * (3 + 9 / SPL_GROUPS) * sizeof(double) * (size_t)max_nc;
         hipStream_t s = (hipStream_t)stream;
-        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_>), dim3(1, nops), dim3(threads), lds, s, d_ops,
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_>), dim3(SPL_GROUPS, nops), dim3(threads), lds, s, d_ops,
                                                  d_blocks, parity_mask, ms));
         MBL_LAUNCH_CHECK();
         return 0;
@@ -2190,8 +2203,8 @@ int launch_iface_mixed(void *stream, int mode, const LineOp *d_ops, int nspl, in
 {
     if (nspl <= 0 || max_nc > SPL_FUSED_MAX_NC) return MBL_E_INVALID;
     const int threads = std::min(1024, (std::max(max_nc, max_nf) + 63) / 64 * 64);
-    const size_t lds = 24 * sizeof(double) * (size_t)max_nc;
-    dim3 g(std::max(1, (max_n + threads - 1) / threads), nspl + std::max(0, nops)), b(threads);
+    const size_t lds = 2 * (3 + 9 / SPL_GROUPS) * sizeof(double) * (size_t)max_nc;
+    dim3 g(std::max(SPL_GROUPS, (max_n + threads - 1) / threads), nspl + std::max(0, nops)), b(threads);
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_iface_mixed<M_>), g, b, lds, s, d_ops, nspl, d_blocks, parity_mask, ms,
                                              xm));
