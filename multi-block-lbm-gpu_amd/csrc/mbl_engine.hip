@@ -64,11 +64,13 @@ struct CycleLaunch {
     uint32_t rmask = 0;      // collide: levels whose last collide of the cycle this launch holds
     int op_off = 0, op_count = 0, max_n = 0, max_nc = 0; // interface/spline: range of the program's op table
     int nspl = 0, max_nf = 0; // kind 3: its first nspl ops are splines (of up to max_nf fine points)
+    std::vector<int> fused;   // kinds 2, 3: temporal items done by its spline workgroups
 };
 
 struct CycleProgram {
     bool built = false;
     int nitems = 0;
+    std::vector<int> fused_into; // per item: the spline item whose launch does it, or -1
     std::vector<CycleLaunch> launches;
     LineOp *dops = nullptr; // ops of every interface/spline launch, each launch contiguous
 };
@@ -1505,6 +1507,42 @@ static void cycle(Layout *L, int level, Ff_Store *t3, CycleState &cs)
     }
 }
 
+// Marker rotations (evolution.h:41-46, m -> m - 1 mod 3) that take a to b, or -1.
+static int marker_turns(const int a[3], const int b[3])
+{
+    for (int r = 0; r < 3; r++) {
+        bool same = true;
+        for (int p = 0; p < 3; p++) same = same && ((a[p] + 2 * r) % 3 == b[p]);
+        if (same) return r;
+    }
+    return -1;
+}
+
+// Whether temporal item t can be done by the workgroups of spline item w: every temporal op
+// reads the ring of one of w's splines (same fine points), on the one-workgroup spline path,
+// at most one temporal op per spline, and the item's markers are w's turned a whole number
+// of times (the op's slots are then w's markers turned as often, at any step).
+static bool fuse_temporal(const Layout *L, const std::vector<CycleItem> &items, int w, int t)
+{
+    const CycleItem &S = items[w], &T = items[t];
+    if (S.kind != 2 || T.kind != 1 || !T.temporal) return false;
+    int mx = 0;
+    for (int o = S.off; o < S.off + S.count; o++) mx = std::max(mx, L->hops[o].nc);
+    if (launch_spline_path(mx) != 0) return false;
+    std::vector<int> used(S.count, 0);
+    for (int o = T.off; o < T.off + T.count; o++) {
+        const LineOp &op = L->hops[o];
+        if (op.type != OP_TEMPORAL) return false;
+        int hit = -1;
+        for (int q = S.off; q < S.off + S.count; q++)
+            if (L->hops[q].ring[0][0] == op.ring[0][0] && L->hops[q].n == op.n && L->hops[q].mgroup == op.mgroup)
+                hit = q - S.off;
+        if (hit < 0 || used[hit]++) return false;
+        if (marker_turns(S.ms.m[op.mgroup], T.ms.m[op.mgroup]) < 0) return false;
+    }
+    return true;
+}
+
 // Dependence-driven positions of the recorded items, then the merged launches.
 static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, CycleProgram &P)
 {
@@ -1518,6 +1556,8 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
     auto epoch_key = [](int l) { return unit(0xFFFFFE, (size_t)l, 0); };
     auto ring_key = [](int b) { return unit(0xFFFFFD, (size_t)b, 0); };
     std::vector<int> pos(n, 0);
+    std::vector<int> &fused_into = P.fused_into;
+    fused_into.assign(n, -1);
     for (int j = 0; j < n; j++) {
         const CycleItem &it = items[j];
         std::vector<uint64_t> rd, wr;
@@ -1537,10 +1577,18 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         rd.erase(std::unique(rd.begin(), rd.end()), rd.end());
         std::sort(wr.begin(), wr.end());
         wr.erase(std::unique(wr.begin(), wr.end()), wr.end());
-        int p = 0;
+        // p: the earliest position; pr: the part of it due to reading ring slots, and their writer
+        int p = 0, pr = 0, rw = -2;
         for (uint64_t u : rd) {
             auto it2 = res.find(u);
-            if (it2 != res.end() && it2->second.writer >= 0) p = std::max(p, pos[it2->second.writer] + 1);
+            if (it2 == res.end() || it2->second.writer < 0) continue;
+            const int w = it2->second.writer;
+            if ((u >> 40) == 0xFFFFFD) {
+                pr = std::max(pr, pos[w] + 1);
+                rw = (rw == -2 || rw == w) ? w : -1;
+            } else {
+                p = std::max(p, pos[w] + 1);
+            }
         }
         for (uint64_t u : wr) {
             auto it2 = res.find(u);
@@ -1549,7 +1597,12 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             for (int r : it2->second.readers)
                 if (r != j) p = std::max(p, pos[r] + 1);
         }
-        pos[j] = p;
+        // a temporal interpolation right after the splines that wrote its lines' newest slots:
+        // done inside the spline launch (each spline workgroup interpolates its line's points
+        // after storing them), when nothing else keeps it from that position
+        if (it.kind == 1 && it.temporal && rw >= 0 && pr > p && p <= pos[rw] && fuse_temporal(L, items, rw, j))
+            fused_into[j] = rw;
+        pos[j] = fused_into[j] >= 0 ? pos[rw] : std::max(p, pr);
         for (uint64_t u : rd) {
             Res &r = res[u];
             if (r.readers.empty() || r.readers.back() != j) r.readers.push_back(j);
@@ -1600,7 +1653,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         // interface ops: one launch per temporal interpolation point (ops without one join the first)
         std::vector<CycleLaunch> ifl;
         for (int j = 0; j < n; j++) {
-            if (pos[j] != q || items[j].kind != 1) continue;
+            if (pos[j] != q || items[j].kind != 1 || fused_into[j] >= 0) continue;
             CycleLaunch *tgt = nullptr;
             for (auto &x : ifl) {
                 bool xt = false;
@@ -1647,6 +1700,11 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             ifl.clear();
             sp[0] = m;
         }
+        // temporal items done by the workgroups of their splines' launch
+        for (int j = 0; j < n; j++)
+            if (pos[j] == q && fused_into[j] >= 0)
+                for (auto &t : sp)
+                    if (std::find(t.items.begin(), t.items.end(), fused_into[j]) != t.items.end()) t.fused.push_back(j);
         for (auto &x : ifl) P.launches.push_back(x);
         for (auto &t : sp)
             if (!t.items.empty()) P.launches.push_back(t);
@@ -1680,6 +1738,22 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
                 cl.max_nc = std::max(cl.max_nc, op.nc);
             }
         cl.op_count = (int)tab.size() - cl.op_off;
+        // fused temporal ops after the launch's own, each named by its spline (LineOp::fuse_t)
+        for (int f : cl.fused) {
+            const CycleItem &T = items[f], &S = items[fused_into[f]];
+            for (int o = T.off; o < T.off + T.count; o++) {
+                LineOp t = L->hops[o];
+                int si = -1;
+                for (int q = cl.op_off; q < cl.op_off + cl.op_count; q++)
+                    if (tab[q].type == OP_SPLINE && tab[q].ring[0][0] == t.ring[0][0]) si = q;
+                if (si < 0 || tab[si].fuse_t) return MBL_E_INVALID;
+                t.trot = marker_turns(S.ms.m[t.mgroup], T.ms.m[t.mgroup]);
+                t.tflip = (int)(((T.pm ^ S.pm) >> L->level_of[t.dst_block]) & 1u);
+                t.txm = T.xm;
+                tab[si].fuse_t = (int)tab.size() - cl.op_off + 1;
+                tab.push_back(t);
+            }
+        }
     }
     if (P.dops) hipFree(P.dops);
     P.dops = nullptr;

@@ -107,6 +107,13 @@ struct LineOp {
     int red_e[2], red_t[2], red_ce[2];
     int red_mg[2];                    // type 1: the c_f op's marker group
     const Lbd *red_ring[2][3][MBL_Q]; // type 1: the c_f op's ring slots
+    // SPLINE: 1 + the index (in its launch's op table) of a temporal op on the same line that
+    // the spline's workgroups do after storing the line's points (0: none). That TEMPORAL op:
+    // its markers are the launch's for its group turned trot times, its block's plane set is
+    // the launch's parity bit xor tflip, and it interpolates at txm.
+    int fuse_t;
+    int trot, tflip;
+    double txm;
 };
 
 // Most PCR levels the one-workgroup spline keeps in registers (rows decouple after 6 levels for

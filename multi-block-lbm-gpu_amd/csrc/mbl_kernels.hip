@@ -980,6 +980,17 @@ template <int MODE> __global__ __launch_bounds__(256) void k_init_general(InitAr
 // ---------------------------------------------------------------------------
 // Interface line ops
 // ---------------------------------------------------------------------------
+// 3-point Lagrange interpolation in time at xm of the ring slots m0, m1, m2
+// (assit_multi.h:232-245); one rounding per operation wherever it is inlined
+__device__ __forceinline__ double temporal3(double xm, Lbd f0, Lbd f1, Lbd f2)
+{
+#pragma clang fp contract(off)
+    const double a = 0.5 * (xm - 1.0) * xm;
+    const double b = (xm - 1.0) * (xm + 1.0);
+    const double cc = 0.5 * xm * (xm + 1.0);
+    return a * (double)f0 - b * (double)f1 + cc * (double)f2;
+}
+
 // Every load an entry needs is issued before its write mask is tested (all
 // are in bounds for e < n), so an entry costs one dependent memory round trip
 // after the op table (two for CC/FC: index, then source node), not one more
@@ -999,14 +1010,9 @@ __device__ __forceinline__ void line_op_entry(const LineOp &op, int e, const Blo
 #pragma unroll
     for (int k = 0; k < 9; k++) D[k] = (T *)db.P[pd][k];
     if (op.type == OP_TEMPORAL) {
-        // 3-point Lagrange in time (assit_multi.h:232-245)
-        const double a = 0.5 * (xm - 1.0) * xm;
-        const double b = (xm - 1.0) * (xm + 1.0);
-        const double cc = 0.5 * xm * (xm + 1.0);
         double v[9];
 #pragma unroll
-        for (int k = 0; k < 9; k++)
-            v[k] = a * (double)op.ring[m0][k][e] - b * (double)op.ring[m1][k][e] + cc * (double)op.ring[m2][k][e];
+        for (int k = 0; k < 9; k++) v[k] = temporal3(xm, op.ring[m0][k][e], op.ring[m1][k][e], op.ring[m2][k][e]);
         if (!mask) return;
 #pragma unroll
         for (int k = 0; k < 9; k++) D[k][didx] = enc64<MODE>(v[k], k);
@@ -1276,7 +1282,8 @@ constexpr int SPL_GROUPS = 9;         // workgroups per line of the spline launc
 
 template <int MODE, int NG = 1>
 __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *__restrict__ blocks,
-                                            uint32_t parity_mask, const MarkerSet &ms, double *lds, int grp = 0)
+                                            uint32_t parity_mask, const MarkerSet &ms, double *lds, int grp = 0,
+                                            const LineOp *__restrict__ base = nullptr)
 {
     static_assert(NG == 1 || NG == 3 || NG == 9, "the 9 distributions split evenly");
     constexpr int NK = 9 / NG, NCOL = 3 + NK; // distributions and columns of this workgroup
@@ -1383,6 +1390,26 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
             op.ring[slot][k][f] = (Lbd)(fe + op.para * (ft - fe));
         }
     }
+    if (op.fuse_t && base) {
+        // the temporal op on this line: its fine points after this thread stored them, for this
+        // workgroup's distributions (line_op_entry's OP_TEMPORAL, same arithmetic)
+        const LineOp &t = base[op.fuse_t - 1];
+        int mt[3];
+#pragma unroll
+        for (int p = 0; p < 3; p++) mt[p] = (ms.m[t.mgroup][p] + 2 * t.trot) % 3;
+        const BlockDesc &tb = blocks[t.dst_block];
+        const int pt = ((parity_mask >> tb.level) & 1) ^ t.tflip;
+        for (int f = f0; f < t.n; f += blockDim.x) {
+            const uint8_t mk = t.mask[f];
+            const int didx = t.dst_idx[f];
+#pragma unroll
+            for (int c = 3; c < NCOL; c++) {
+                const int k = gc(c) - 3;
+                const double v = temporal3(t.txm, t.ring[mt[0]][k][f], t.ring[mt[1]][k][f], t.ring[mt[2]][k][f]);
+                if (mk) ((T *)tb.P[pt][k])[didx] = enc64<MODE>(v, k);
+            }
+        }
+    }
 }
 
 template <int MODE>
@@ -1391,7 +1418,7 @@ __global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict_
                                                        MarkerSet ms)
 {
     extern __shared__ double lds[];
-    spline_line<MODE, SPL_GROUPS>(ops[blockIdx.y], blocks, parity_mask, ms, lds, (int)blockIdx.x);
+    spline_line<MODE, SPL_GROUPS>(ops[blockIdx.y], blocks, parity_mask, ms, lds, (int)blockIdx.x, ops);
 }
 
 // One-workgroup splines and interface ops of one position of the scheduled cycle in one
@@ -1405,7 +1432,8 @@ __global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__
     extern __shared__ double lds[];
     const int y = (int)blockIdx.y;
     if (y < nspl) {
-        if (blockIdx.x < SPL_GROUPS) spline_line<MODE, SPL_GROUPS>(ops[y], blocks, parity_mask, ms, lds, (int)blockIdx.x);
+        if (blockIdx.x < SPL_GROUPS)
+            spline_line<MODE, SPL_GROUPS>(ops[y], blocks, parity_mask, ms, lds, (int)blockIdx.x, ops);
         return;
     }
     const LineOp &op = ops[y];

@@ -27,12 +27,14 @@ START = LONG // 2
 
 def test_c2b_schedule_shape(lib32):
     """The C2b coarse step as scheduled: 22 operations of the reference order (7 collides, 15
-    interface batches) in 14 positions of one launch each -- collides of adjacent levels share a
+    interface batches) in 12 positions of one launch each -- collides of adjacent levels share a
     launch, the c_f / f_c batches are one launch each, interface ops and splines of one position
-    share one. With forked positions the launches that share a position stay apart."""
+    share one, and the finest level's temporal interpolations run in the workgroups of the
+    splines that wrote their newest slots. With forked positions the launches that share a
+    position stay apart."""
     import ctypes as C
     geom = mbl.c2b_geometry(lib32)
-    for fork, want_launches in ((0, 14), (1, 16)):
+    for fork, want_launches in ((0, 12), (1, 14)):
         sim = mbl.MultiBlockSim(lib32, geom)
         mbl.check(lib32, lib32.mbl_set_cycle_fork(sim.h, fork), "fork")
         sim.step(2)
@@ -41,7 +43,7 @@ def test_c2b_schedule_shape(lib32):
         pos = lib32.mbl_cycle_positions(sim.h, 0)
         print("fork", fork, "launches", total, "positions", pos, "collide/interface/spline", col.value, ifc.value,
               spl.value)
-        assert pos == 14
+        assert pos == 12
         assert total == want_launches
         assert col.value == 4 and spl.value == 3
         sim.close()
