@@ -57,7 +57,7 @@ struct CycleItem {
 // A merged launch of the scheduled cycle: items of one kind with no dependence among them
 // (kind 3: one-workgroup splines and interface ops of one position in one launch).
 struct CycleLaunch {
-    int kind;
+    int kind; // 0 stream-collide, 1 interface ops, 2 splines, 3 splines + interface ops, 4 collide + interface ops
     int pos = 0;             // position in the schedule (launches at one position are independent)
     std::vector<int> items;  // indices into the recorded item sequence
     int lo = 0, hi = 0;      // collide: levels lo..hi (adjacent, same nodes per thread)
@@ -789,6 +789,7 @@ struct OpBuild {
     int uid = -1;  // serial number (c_f ops named by the redirected line ends of f_c ops)
     int red_uid[2] = {-1, -1};
     int line = -1; // coarse/fine line whose ring the op writes (spline) or reads (temporal, c_f)
+    std::vector<int> gred; // SPLINE: per coarse point, the (block, node) read in its place, or -1, -1
     std::vector<int> src, dst;
     std::vector<uint8_t> mask;
     std::vector<uint64_t> rd;                          // read units
@@ -1223,6 +1224,44 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
             }
         }
     };
+    // A spline gathers its coarse line right after the same-level copies of its level
+    // (evolution.h:26-33, nothing in between): a node whose two fields a copy wrote holds the
+    // copy's source node, which no copy of the batch writes, so the spline reads that source
+    // node instead (LineOp::gred) and no longer waits for the copies. One-workgroup splines
+    // only (the path that reads LineOp::gred).
+    auto redirect_spline_reads = [&](std::vector<OpBuild> &sops, size_t cc0, size_t cc1) {
+        std::unordered_map<uint64_t, std::pair<int, int>> live; // unit -> (copy op in all, entry)
+        for (size_t j = cc0; j < cc1; j++)
+            for (auto &w : all[j].wr) {
+                const int ent = w.second >> 1, fld = w.second & 1;
+                if ((all[j].mask[ent] >> fld) & 1) live[w.first] = {(int)j, ent};
+            }
+        if (live.empty()) return;
+        for (auto &o : sops) {
+            if (o.op.type != OP_SPLINE || launch_spline_path(o.op.nc) != 0) continue;
+            std::vector<int> gr(2 * o.src.size(), -1);
+            bool any = false;
+            for (size_t ip = 0; ip < o.src.size(); ip++) {
+                const uint64_t u0 = unit(o.op.src_block, (size_t)o.src[ip], 0);
+                const uint64_t u1 = unit(o.op.src_block, (size_t)o.src[ip], 1);
+                auto a = live.find(u0), b = live.find(u1);
+                if (a == live.end() || b == live.end() || a->second != b->second) continue;
+                const OpBuild &c = all[a->second.first];
+                const int sb = c.op.src_block, sn = c.src[a->second.second];
+                if (live.count(unit(sb, (size_t)sn, 0)) || live.count(unit(sb, (size_t)sn, 1))) continue;
+                gr[2 * ip] = sb;
+                gr[2 * ip + 1] = sn;
+                any = true;
+                for (auto &u : o.rd) {
+                    if (u == u0)
+                        u = unit(sb, (size_t)sn, 0);
+                    else if (u == u1)
+                        u = unit(sb, (size_t)sn, 1);
+                }
+            }
+            if (any) o.gred = std::move(gr);
+        }
+    };
     auto add_batch = [&](std::vector<OpBuild> ops, Batch &batch) -> bool {
         for (auto &o : ops)
             if (o.op.n < 0) return false;
@@ -1252,10 +1291,15 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         ops.clear();
         if (nline2 != 0)
             for (int c = L->LCC[l]; c < L->LCC[l + 1]; c++) ops.push_back(make_CC(c));
+        const size_t cc0 = all.size();
         if (!add_batch(std::move(ops), L->bCC[l])) return fail("same-level line outside its blocks");
+        const size_t cc1 = all.size();
         ops.clear();
         if (l != cfg.LEVEL - 1)
             for (int b = L->LS3[l]; b < L->LS3[l + 1]; b++) ops.push_back(make_S(b));
+        for (auto &o : ops)
+            if (o.op.n < 0) return fail("spline line outside its blocks");
+        redirect_spline_reads(ops, cc0, cc1);
         if (!add_batch(std::move(ops), L->bS[l])) return fail("spline line outside its blocks");
         ops.clear();
         if (l != 0)
@@ -1306,7 +1350,7 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
     // upload tables
     size_t nidx = 0, nmask = 0, nM = 0;
     for (auto &o : all) {
-        nidx += o.src.size() + o.dst.size();
+        nidx += o.src.size() + o.dst.size() + (o.gred.empty() ? 0 : o.gred.size() + 1);
         nmask += o.mask.size();
         if (o.op.type == OP_SPLINE) nM += 12 * (size_t)o.op.nc;
     }
@@ -1343,6 +1387,12 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         hidx.insert(hidx.end(), o.src.begin(), o.src.end());
         op.dst_idx = o.dst.empty() ? nullptr : L->didx + hidx.size();
         hidx.insert(hidx.end(), o.dst.begin(), o.dst.end());
+        op.gred = nullptr;
+        if (!o.gred.empty()) { // (block, node) pairs, 8-byte aligned
+            if (hidx.size() & 1) hidx.push_back(0);
+            op.gred = reinterpret_cast<const int2 *>(L->didx + hidx.size());
+            hidx.insert(hidx.end(), o.gred.begin(), o.gred.end());
+        }
         op.mask = L->dmask + hmask.size();
         hmask.insert(hmask.end(), o.mask.begin(), o.mask.end());
         if (op.type == OP_SPLINE) {
@@ -1507,6 +1557,9 @@ static void cycle(Layout *L, int level, Ff_Store *t3, CycleState &cs)
     }
 }
 
+static int launch_state(const Layout *L, const std::vector<CycleItem> &items, const CycleLaunch &cl, uint32_t &pm,
+                        MarkerSet &ms);
+
 // Marker rotations (evolution.h:41-46, m -> m - 1 mod 3) that take a to b, or -1.
 static int marker_turns(const int a[3], const int b[3])
 {
@@ -1579,6 +1632,17 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         wr.erase(std::unique(wr.begin(), wr.end()), wr.end());
         // p: the earliest position; pr: the part of it due to reading ring slots, and their writer
         int p = 0, pr = 0, rw = -2;
+        int why_item = -1; // the item and resource that set p (printed with the schedule)
+        uint64_t why_unit = 0;
+        char why_kind = '-';
+        auto bump = [&](int q, int item, uint64_t u, char kind) {
+            if (q > p) {
+                p = q;
+                why_item = item;
+                why_unit = u;
+                why_kind = kind;
+            }
+        };
         for (uint64_t u : rd) {
             auto it2 = res.find(u);
             if (it2 == res.end() || it2->second.writer < 0) continue;
@@ -1587,15 +1651,15 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
                 pr = std::max(pr, pos[w] + 1);
                 rw = (rw == -2 || rw == w) ? w : -1;
             } else {
-                p = std::max(p, pos[w] + 1);
+                bump(pos[w] + 1, w, u, 'R');
             }
         }
         for (uint64_t u : wr) {
             auto it2 = res.find(u);
             if (it2 == res.end()) continue;
-            if (it2->second.writer >= 0) p = std::max(p, pos[it2->second.writer] + 1);
+            if (it2->second.writer >= 0) bump(pos[it2->second.writer] + 1, it2->second.writer, u, 'W');
             for (int r : it2->second.readers)
-                if (r != j) p = std::max(p, pos[r] + 1);
+                if (r != j) bump(pos[r] + 1, r, u, 'A');
         }
         // a temporal interpolation right after the splines that wrote its lines' newest slots:
         // done inside the spline launch (each spline workgroup interpolates its line's points
@@ -1603,6 +1667,10 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         if (it.kind == 1 && it.temporal && rw >= 0 && pr > p && p <= pos[rw] && fuse_temporal(L, items, rw, j))
             fused_into[j] = rw;
         pos[j] = fused_into[j] >= 0 ? pos[rw] : std::max(p, pr);
+        if (getenv("MBL_PRINT_SCHEDULE"))
+            fprintf(stderr, "mbl schedule: item %2d after item %d (%c block %d node %d field %d)%s\n", j, why_item,
+                    why_kind, (int)(why_unit >> 40), (int)((why_unit >> 1) & 0x7FFFFFFFFFull), (int)(why_unit & 1),
+                    pr > p ? " ring" : "");
         for (uint64_t u : rd) {
             Res &r = res[u];
             if (r.readers.empty() || r.readers.back() != j) r.readers.push_back(j);
@@ -1635,6 +1703,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         for (int j = 0; j < n; j++)
             if (pos[j] == q && items[j].kind == 0) col.push_back(j);
         std::sort(col.begin(), col.end(), [&](int a, int b) { return items[a].level < items[b].level; });
+        std::vector<CycleLaunch> cls;
         for (size_t c = 0; c < col.size();) {
             CycleLaunch cl;
             cl.kind = 0;
@@ -1647,7 +1716,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
                 cl.items.push_back(col[d]);
                 d++;
             }
-            P.launches.push_back(cl);
+            cls.push_back(cl);
             c = d;
         }
         // interface ops: one launch per temporal interpolation point (ops without one join the first)
@@ -1705,6 +1774,15 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             if (pos[j] == q && fused_into[j] >= 0)
                 for (auto &t : sp)
                     if (std::find(t.items.begin(), t.items.end(), fused_into[j]) != t.items.end()) t.fused.push_back(j);
+        // one stream-collide (one node per thread) and one interface launch at one position: one
+        // launch whose first workgroups are the collide's tiles and the others the interface entries
+        if (!h->fork && cls.size() == 1 && ifl.size() == 1 && sp[0].items.empty() && sp[1].items.empty() &&
+            !L->periodic && L->vx[cls[0].lo] == 1) {
+            cls[0].kind = 4;
+            cls[0].items.insert(cls[0].items.end(), ifl[0].items.begin(), ifl[0].items.end());
+            ifl.clear();
+        }
+        for (auto &x : cls) P.launches.push_back(x);
         for (auto &x : ifl) P.launches.push_back(x);
         for (auto &t : sp)
             if (!t.items.empty()) P.launches.push_back(t);
@@ -1716,7 +1794,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
         for (int j = 0; j < n; j++)
             if (items[j].kind == 0 && items[j].level == l) lastj = j;
         for (auto &cl : P.launches)
-            if (cl.kind == 0 && std::find(cl.items.begin(), cl.items.end(), lastj) != cl.items.end())
+            if ((cl.kind == 0 || cl.kind == 4) && std::find(cl.items.begin(), cl.items.end(), lastj) != cl.items.end())
                 cl.rmask |= 1u << l;
     }
     // the program's op table: each interface/spline launch's ops contiguous
@@ -1739,16 +1817,20 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             }
         cl.op_count = (int)tab.size() - cl.op_off;
         // fused temporal ops after the launch's own, each named by its spline (LineOp::fuse_t)
+        uint32_t lpm = 0;
+        MarkerSet lms;
+        if (!cl.fused.empty() && launch_state(L, items, cl, lpm, lms)) return MBL_E_INVALID;
         for (int f : cl.fused) {
-            const CycleItem &T = items[f], &S = items[fused_into[f]];
+            const CycleItem &T = items[f];
             for (int o = T.off; o < T.off + T.count; o++) {
                 LineOp t = L->hops[o];
                 int si = -1;
                 for (int q = cl.op_off; q < cl.op_off + cl.op_count; q++)
                     if (tab[q].type == OP_SPLINE && tab[q].ring[0][0] == t.ring[0][0]) si = q;
                 if (si < 0 || tab[si].fuse_t) return MBL_E_INVALID;
-                t.trot = marker_turns(S.ms.m[t.mgroup], T.ms.m[t.mgroup]);
-                t.tflip = (int)(((T.pm ^ S.pm) >> L->level_of[t.dst_block]) & 1u);
+                t.trot = marker_turns(lms.m[t.mgroup], T.ms.m[t.mgroup]);
+                t.tflip = (int)(((T.pm ^ lpm) >> L->level_of[t.dst_block]) & 1u);
+                if (t.trot < 0) return MBL_E_INVALID;
                 t.txm = T.xm;
                 tab[si].fuse_t = (int)tab.size() - cl.op_off + 1;
                 tab.push_back(t);
@@ -1772,7 +1854,7 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
 static int launch_state(const Layout *L, const std::vector<CycleItem> &items, const CycleLaunch &cl, uint32_t &pm,
                         MarkerSet &ms)
 {
-    uint32_t known = 0;
+    uint32_t known = 0, soft = 0;
     bool msk[MBL_MAX_LEVEL] = {false};
     pm = items[cl.items[0]].pm;
     ms = items[cl.items[0]].ms;
@@ -1783,6 +1865,15 @@ static int launch_state(const Layout *L, const std::vector<CycleItem> &items, co
         known |= b;
         return 0;
     };
+    // a spline writes no plane of its fine level (only macros and its ring): that bit is taken
+    // where nothing else in the launch decides it (a temporal op done by the spline's
+    // workgroups counts from it)
+    auto soft_bit = [&](int l, uint32_t from) {
+        const uint32_t b = 1u << l;
+        if ((known | soft) & b) return;
+        pm = (pm & ~b) | (from & b);
+        soft |= b;
+    };
     for (int j : cl.items) {
         const CycleItem &it = items[j];
         if (it.kind == 0) {
@@ -1791,15 +1882,28 @@ static int launch_state(const Layout *L, const std::vector<CycleItem> &items, co
         }
         for (int o = it.off; o < it.off + it.count; o++) {
             const LineOp &op = L->hops[o];
-            if (take_bit(L->level_of[op.src_block], it.pm) || take_bit(L->level_of[op.dst_block], it.pm))
+            if (op.type == OP_SPLINE) {
+                if (take_bit(L->level_of[op.src_block], it.pm)) return MBL_E_INVALID;
+                soft_bit(L->level_of[op.dst_block], it.pm);
+            } else if (take_bit(L->level_of[op.src_block], it.pm) || take_bit(L->level_of[op.dst_block], it.pm)) {
                 return MBL_E_INVALID;
-            const int g = op.mgroup;
-            if (msk[g]) {
-                for (int p = 0; p < 3; p++)
-                    if (ms.m[g][p] != it.ms.m[g][p]) return MBL_E_INVALID;
-            } else {
-                for (int p = 0; p < 3; p++) ms.m[g][p] = it.ms.m[g][p];
-                msk[g] = true;
+            }
+            // the marker groups the op reads: its own (temporal, c_f, spline), the c_f ops'
+            // of an f_c op's redirected line ends
+            int gs[2], ng = 0;
+            if (op.type == OP_TEMPORAL || op.type == OP_CF || op.type == OP_SPLINE) gs[ng++] = op.mgroup;
+            if (op.type == OP_FC)
+                for (int r = 0; r < op.red_n; r++)
+                    if (op.red_t[r] == 1) gs[ng++] = op.red_mg[r];
+            for (int q = 0; q < ng; q++) {
+                const int g = gs[q];
+                if (msk[g]) {
+                    for (int p = 0; p < 3; p++)
+                        if (ms.m[g][p] != it.ms.m[g][p]) return MBL_E_INVALID;
+                } else {
+                    for (int p = 0; p < 3; p++) ms.m[g][p] = it.ms.m[g][p];
+                    msk[g] = true;
+                }
             }
         }
     }
@@ -1821,6 +1925,25 @@ static int launch_one(mbl_context *h, Layout *L, const std::vector<CycleItem> &i
         return launch_stream_collide(stream, mode, L->vx[cl.lo], h->tpb, L->periodic, true, true, L->db, L->dtile,
                                      L->nblocks, tiles, L->rows, pm, h->groups, L->tile_off[cl.lo], rmask,
                                      rmask ? L->dres : nullptr);
+    }
+    if (cl.kind == 4) {
+        CollideOpsArgs a;
+        memset(&a, 0, sizeof(a));
+        for (int l = cl.lo; l <= cl.hi; l++) a.tiles += L->tiles[l];
+        for (int j : cl.items)
+            if (items[j].temporal) a.xm = items[j].xm;
+        a.descs = L->db;
+        a.tile_blk = L->dtile;
+        a.ndesc = L->nblocks;
+        a.rows = L->rows;
+        a.pm = pm;
+        a.tile_base = L->tile_off[cl.lo];
+        a.rmask = fused ? cl.rmask : 0u;
+        a.rparts = a.rmask ? L->dres : nullptr;
+        a.ops = P.dops + cl.op_off;
+        a.nops = cl.op_count;
+        a.ms = ms;
+        return launch_collide_ops(stream, mode, h->tpb, a, cl.max_n);
     }
     if (cl.kind == 1 || cl.kind == 3) {
         double xm = 0.0;
@@ -2721,12 +2844,12 @@ extern "C" int mbl_cycle_launches(mbl_handle_t h, int level, int *collides, int 
     if (!h) h = g_current;
     if (!h) return MBL_E_NO_HANDLE;
     if (!h->layout || level < 0 || level >= MBL_MAX_LEVEL || !h->layout->prog[level].built) return MBL_E_INVALID;
-    int c[4] = {0, 0, 0, 0};
+    int c[5] = {0, 0, 0, 0, 0};
     for (const CycleLaunch &cl : h->layout->prog[level].launches) c[cl.kind]++;
-    if (collides) *collides = c[0];
-    if (interface_ops) *interface_ops = c[1] + c[3];
+    if (collides) *collides = c[0] + c[4];
+    if (interface_ops) *interface_ops = c[1] + c[3] + c[4];
     if (splines) *splines = c[2] + c[3];
-    return c[0] + c[1] + c[2] + c[3];
+    return c[0] + c[1] + c[2] + c[3] + c[4];
 }
 
 extern "C" int mbl_cycle_positions(mbl_handle_t h, int level)

@@ -107,6 +107,9 @@ struct LineOp {
     int red_e[2], red_t[2], red_ce[2];
     int red_mg[2];                    // type 1: the c_f op's marker group
     const Lbd *red_ring[2][3][MBL_Q]; // type 1: the c_f op's ring slots
+    // SPLINE: per coarse point, the (block, node) its column values are read from in place of
+    // src_idx's node in src_block (a same-level copy's source), block -1: none; null: none at all
+    const int2 *gred;
     // SPLINE: 1 + the index (in its launch's op table) of a temporal op on the same line that
     // the spline's workgroups do after storing the line's points (0: none). That TEMPORAL op:
     // its markers are the launch's for its group turned trot times, its block's plane set is
@@ -230,6 +233,22 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
                   const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms);
 // nspl one-workgroup spline ops (max_nc coarse, max_nf fine points) followed by nops interface ops
 // (up to max_n entries) in one launch
+// A flagged stream-collide (one node per thread) over tiles [tile_base, tile_base + tiles) and
+// the entries of nops interface ops in one launch (launch_collide_ops sets opx).
+struct CollideOpsArgs {
+    const BlockDesc *descs;
+    const int *tile_blk;
+    int ndesc, rows;
+    uint32_t pm;
+    int tiles, tile_base;
+    uint32_t rmask;
+    double *rparts;
+    const LineOp *ops;
+    int nops, opx;
+    MarkerSet ms;
+    double xm;
+};
+int launch_collide_ops(void *stream, int mode, int tpb, const CollideOpsArgs &a, int max_n);
 int launch_iface_mixed(void *stream, int mode, const LineOp *d_ops, int nspl, int nops, int max_nc, int max_nf,
                        int max_n, const BlockDesc *d_blocks, uint32_t parity_mask, const MarkerSet &ms, double xm);
 int launch_ring_fill(void *stream, int mode, const LineOp *d_ops, int nops, int max_n,

@@ -1316,6 +1316,7 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
     double d[NCOL];
     if (row) {
         const int node = op.src_idx[ip];
+        const int2 rd = op.gred ? op.gred[ip] : make_int2(-1, -1);
         double g[NCOL];
         g[0] = (double)sb.rho[node];
         g[1] = (double)sb.ux[node];
@@ -1324,6 +1325,21 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
         for (int c = 3; c < NCOL; c++) {
             const int k = gc(c) - 3;
             g[c] = dec64<MODE>(((const T *)sb.P[ps][k])[node], k);
+        }
+        // nodes a same-level copy of this step fills: read at the copy's source (same level,
+        // same plane set), in the waves that hold one
+        if (__builtin_amdgcn_ballot_w64(rd.x >= 0)) {
+            if (rd.x >= 0) {
+                const BlockDesc &rb = blocks[rd.x];
+                g[0] = (double)rb.rho[rd.y];
+                g[1] = (double)rb.ux[rd.y];
+                g[2] = (double)rb.uy[rd.y];
+#pragma unroll
+                for (int c = 3; c < NCOL; c++) {
+                    const int k = gc(c) - 3;
+                    g[c] = dec64<MODE>(((const T *)rb.P[ps][k])[rd.y], k);
+                }
+            }
         }
 #pragma unroll
         for (int c = 0; c < NCOL; c++) {
@@ -1440,6 +1456,27 @@ __global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__
     const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= op.n) return;
     line_op_entry<MODE>(op, e, blocks, parity_mask, ms, xm);
+}
+
+// A stream-collide and the interface ops of one position of the scheduled cycle in one launch
+// (no dependence between them, or the schedule would have separated them): workgroups
+// [0, tiles) are the collide's tiles (one node per thread, flagged blocks), the others run
+// the entries of the ops, opx workgroups per op.
+template <int MODE, bool RESID>
+__global__ __launch_bounds__(256) void k_collide_ops(CollideOpsArgs a)
+{
+    const int b = (int)blockIdx.x;
+    if (b < a.tiles) {
+        const int t = a.tile_base + xcd_tile(b, a.tiles);
+        sc_tile<MODE, 1, false, true, true, RESID>(a.descs, a.tile_blk, a.ndesc, a.rows, a.pm, t,
+                                                   (int)(threadIdx.x >> 6), (int)(blockDim.x >> 6), a.rmask, a.rparts);
+        return;
+    }
+    const int q = b - a.tiles, y = q / a.opx;
+    const LineOp &op = a.ops[y];
+    const int e = (q - y * a.opx) * (int)blockDim.x + (int)threadIdx.x;
+    if (e >= op.n) return;
+    line_op_entry<MODE>(op, e, a.descs, a.pm, a.ms, a.xm);
 }
 
 // init_tri_A (multi/init_multi.h:30-51), final race-free state
@@ -2134,6 +2171,24 @@ int launch_line_ops(void *stream, int mode, const LineOp *d_ops, int nops, int m
     dim3 g((max_n + 255) / 256, nops), b(256);
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_line_ops<M_>), g, b, 0, s, d_ops, d_blocks, parity_mask, ms, xm));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_collide_ops(void *stream, int mode, int tpb, const CollideOpsArgs &args, int max_n)
+{
+    if (args.tiles <= 0 || args.nops <= 0 || max_n <= 0) return MBL_E_INVALID;
+    if (tpb % 64 || tpb < 64 || tpb > 256 || args.rows < tpb / 64 || args.tile_base < 0) return MBL_E_INVALID;
+    CollideOpsArgs a = args;
+    a.opx = (max_n + tpb - 1) / tpb;
+    dim3 g(a.tiles + a.opx * a.nops), b(tpb);
+    hipStream_t s = (hipStream_t)stream;
+    if (a.rmask) {
+        if (!a.rparts) return MBL_E_INVALID;
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_collide_ops<M_, true>), g, b, 0, s, a));
+    } else {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_collide_ops<M_, false>), g, b, 0, s, a));
+    }
     MBL_LAUNCH_CHECK();
     return 0;
 }

@@ -27,14 +27,15 @@ START = LONG // 2
 
 def test_c2b_schedule_shape(lib32):
     """The C2b coarse step as scheduled: 22 operations of the reference order (7 collides, 15
-    interface batches) in 12 positions of one launch each -- collides of adjacent levels share a
+    interface batches) in 8 positions of one launch each -- collides of adjacent levels share a
     launch, the c_f / f_c batches are one launch each, interface ops and splines of one position
-    share one, and the finest level's temporal interpolations run in the workgroups of the
-    splines that wrote their newest slots. With forked positions the launches that share a
-    position stay apart."""
+    share one, the finest level's temporal interpolations run in the workgroups of the splines
+    that wrote their newest slots, the splines read the nodes a same-level copy fills at the
+    copy's source, so they share the copies' position, and a collide and the interface ops of
+    one position share one. With forked positions the launches that share a position stay apart."""
     import ctypes as C
     geom = mbl.c2b_geometry(lib32)
-    for fork, want_launches in ((0, 12), (1, 14)):
+    for fork, want_launches in ((0, 8), (1, 12)):
         sim = mbl.MultiBlockSim(lib32, geom)
         mbl.check(lib32, lib32.mbl_set_cycle_fork(sim.h, fork), "fork")
         sim.step(2)
@@ -43,9 +44,9 @@ def test_c2b_schedule_shape(lib32):
         pos = lib32.mbl_cycle_positions(sim.h, 0)
         print("fork", fork, "launches", total, "positions", pos, "collide/interface/spline", col.value, ifc.value,
               spl.value)
-        assert pos == 12
+        assert pos == 8
         assert total == want_launches
-        assert col.value == 4 and spl.value == 3
+        assert col.value == 4 and spl.value == 2
         sim.close()
 
 
