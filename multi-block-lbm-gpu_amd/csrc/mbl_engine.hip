@@ -108,6 +108,9 @@ struct Layout {
     Batch bT[MBL_MAX_LEVEL], bCC[MBL_MAX_LEVEL], bS[MBL_MAX_LEVEL], bCF[MBL_MAX_LEVEL];
     std::vector<std::vector<int>> batch_lines_T, batch_lines_S, batch_lines_CF; // line ids per level
     std::map<uint64_t, hipGraphExec_t> graphs;
+    // graphs of several consecutive fused steps: (phase key of the first, number of steps)
+    std::map<std::pair<uint64_t, int>, hipGraphExec_t> chunks;
+    uint64_t diag_sig = 0; // the diagnostics arguments the fused graphs were captured with
     std::vector<int> level_of;  // level of each block
     std::vector<OpAccess> acc;  // per op of dops: what it reads and writes (cycle scheduling)
     CycleProgram prog[MBL_MAX_LEVEL]; // scheduled launches of evolution() from each level
@@ -119,6 +122,8 @@ struct Layout {
         }
         for (auto &g : graphs) hipGraphExecDestroy(g.second);
         graphs.clear();
+        for (auto &g : chunks) hipGraphExecDestroy(g.second);
+        chunks.clear();
         if (db) hipFree(db);
         if (dops) hipFree(dops);
         if (didx) hipFree(didx);
@@ -191,6 +196,11 @@ struct mbl_context {
     std::vector<std::pair<int, int>> pend_final;
     double *d_lparts = nullptr;
     int lparts_stride = 0;
+    // the two record-slot counters of the fused step's diagnostics launch (it reads one and
+    // writes the next slot to the other, by the coarse level's parity) and their values as
+    // enqueued (-1: unknown)
+    int *d_slots = nullptr;
+    int slot_host[2] = {-1, -1};
     std::vector<int> ser_l2_t, ser_drag_t;
     std::vector<double> ser_l2, ser_cd, ser_cl;
     double last[5] = {0, 0, 0, 0, 0};
@@ -341,7 +351,8 @@ extern "C" int mbl_handle_create(mbl_handle_t *out, const mbl_config *cfg)
     const int st = storage_mode(h->cfg);
     h->vx = store_elem_bytes(st) == 2 ? 1 : 4;
     h->tpb = 128;
-    if (hipMalloc(&h->d_l2, sizeof(double) * L2_REC * h->cap) != hipSuccess ||
+    if (hipMalloc(&h->d_slots, 2 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&h->d_l2, sizeof(double) * L2_REC * h->cap) != hipSuccess ||
         hipMalloc(&h->d_drag, sizeof(double) * 2 * h->cap) != hipSuccess ||
         hipHostMalloc(&h->h_l2, sizeof(double) * L2_REC * h->cap, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&h->h_drag, sizeof(double) * 2 * h->cap, hipHostMallocDefault) != hipSuccess ||
@@ -388,6 +399,7 @@ extern "C" void mbl_handle_destroy(mbl_handle_t h)
     if (h->d_l2) hipFree(h->d_l2);
     if (h->d_drag) hipFree(h->d_drag);
     if (h->d_lparts) hipFree(h->d_lparts);
+    if (h->d_slots) hipFree(h->d_slots);
     if (h->h_l2) hipHostFree(h->h_l2);
     if (h->h_drag) hipHostFree(h->h_drag);
     if (h->ev0) hipEventDestroy(h->ev0);
@@ -2003,7 +2015,7 @@ static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &it
 }
 
 static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2,
-                           bool fused = false);
+                           bool fused = false, StepDiagArgs *diag = nullptr, int drag_block = -1);
 
 extern "C" void evolution(Lbi n, Lbi *asp, Lbi *ls3cffcp, Lbi *ltcffcp, Lbi *lccp, Lbi level, mbl_dim3 block,
                           mbl_dim3 *grid, AREA *areas, D2Q9 *pps_h, D2Q9 *pps_d, Inf_Ff_store *line_Inf_c,
@@ -2084,61 +2096,168 @@ static int phase_key(const Layout *L, const D2Q9 *pps_d, const Ff_Store *line_t3
 
 // fused (level 0 of a layout with step_ok): the collides also write the residual sums
 // (mbl_evolution_step); its graphs are kept apart from the plain cycle's.
-static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2,
-                           bool fused)
+// Every argument of a diagnostics launch except the ones the phase sets (slot counters, the
+// drag's plane set).
+static uint64_t diag_signature(const StepDiagArgs &d, int drag_block)
 {
-    const mbl_config &cfg = h->cfg;
+    uint64_t s = 0xD1A6;
+    s = mix64(s, (uintptr_t)d.descs);
+    s = mix64(s, (uintptr_t)d.list);
+    s = mix64(s, ((uint64_t)(uint32_t)d.nlist << 32) | (uint32_t)d.list_wgs);
+    s = mix64(s, ((uint64_t)(uint32_t)d.ntiles << 32) | (uint32_t)d.cap);
+    s = mix64(s, (uintptr_t)d.rparts);
+    s = mix64(s, (uintptr_t)d.lbase);
+    s = mix64(s, (uint64_t)d.lstride);
+    s = mix64(s, (uintptr_t)d.drag_base);
+    s = mix64(s, (uint64_t)(int64_t)drag_block);
+    s = mix64(s, (uintptr_t)d.drag.links);
+    s = mix64(s, ((uint64_t)(uint32_t)d.drag.n << 32) | (uint32_t)d.drag.nx);
+    return s;
+}
+
+// One call of evolution() planned on the host: its phase key, the recorded items (the ring
+// markers rotate in the caller's tables as evolution.h:41-46 does), the diagnostics launch
+// that follows it (fused step), and the caller's F/f handed over for the levels that took an
+// odd number of substeps. The launches are issued later (launch_plans), one graph for one or
+// several consecutive plans.
+struct StepPlan {
+    uint64_t key = 0;
+    std::vector<CycleItem> items;
+    bool diag = false;
+    StepDiagArgs d;
+};
+
+static void drop_fused_graphs(Layout *L)
+{
+    for (auto it = L->graphs.begin(); it != L->graphs.end();) {
+        if (it->first & (1ull << 57)) {
+            hipGraphExecDestroy(it->second);
+            it = L->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    for (auto &g : L->chunks) hipGraphExecDestroy(g.second);
+    L->chunks.clear();
+}
+
+// diag (fused only): the step's diagnostics launch, whose record slot is want_slot (its slot
+// counter is set to it here when it does not already hold it); its drag reads block
+// drag_block's older state (the caller's f after the call).
+static int plan_step(mbl_context *h, Layout *L, const EvoArgs &A, int level, int nline, bool fused,
+                     const StepDiagArgs *diag, int drag_block, int want_slot, StepPlan &sp)
+{
     D2Q9 *pps_d = A.pps;
     Ff_Store *line_t3 = A.t3;
-    Layout *L = layout_for(h, A, nblocks, nline, nline2);
-    if (!L) return;
     uint32_t pm = 0;
     uint64_t key = 0;
-    if (int rc = phase_key(L, pps_d, line_t3, level, pm, key)) {
-        set_err(h, rc);
-        return;
-    }
-    // the cycle in the reference order (markers rotate here), then its scheduled launches
+    if (int rc = phase_key(L, pps_d, line_t3, level, pm, key)) return rc;
+    // the cycle in the reference order (markers rotate here)
     std::vector<Ff_Store> saved(line_t3, line_t3 + nline);
     CycleState cs{pm, 0, {}};
     cycle(L, level, line_t3, cs);
     auto fail = [&](int code) {
         std::copy(saved.begin(), saved.end(), line_t3);
-        set_err(h, code);
+        return code;
     };
     if (cs.rc) return fail(cs.rc);
     CycleProgram &P = L->prog[level];
     if (!P.built && schedule_cycle(h, L, cs.items, P)) return fail(h->last_error ? h->last_error : MBL_E_HIP);
     if (fused) key |= 1ull << 57;
-    if (cfg.use_graph) {
-        auto it = L->graphs.find(key);
-        if (it == L->graphs.end()) {
-            hipGraph_t g = nullptr;
-            hipGraphExec_t ge = nullptr;
-            if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(MBL_E_HIP);
-            int rc = run_cycle(h, L, cs.items, P, fused);
-            hipError_t e = hipStreamEndCapture(h->stream, &g);
-            if (rc || e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-                if (g) hipGraphDestroy(g);
-                return fail(rc ? rc : MBL_E_HIP);
-            }
-            hipGraphDestroy(g);
-            L->graphs[key] = ge;
-            if (hipGraphLaunch(ge, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
-        } else {
-            if (hipGraphLaunch(it->second, h->stream) != hipSuccess) set_err(h, MBL_E_HIP);
+    sp.key = key;
+    sp.diag = fused && diag;
+    if (sp.diag) {
+        sp.d = *diag;
+        // the slot counter this step reads holds the record's index
+        const int p = (int)(pm & 1u);
+        sp.d.slot_rd = h->d_slots + p;
+        sp.d.slot_wr = h->d_slots + (1 - p);
+        if (h->slot_host[p] != want_slot) {
+            if (hipMemsetD32Async((hipDeviceptr_t)(h->d_slots + p), want_slot, 1, h->stream) != hipSuccess)
+                return fail(MBL_E_HIP);
+            h->slot_host[p] = want_slot;
         }
-    } else {
-        int rc = run_cycle(h, L, cs.items, P, fused);
-        if (rc) return fail(rc);
+        h->slot_host[1 - p] = want_slot + 1;
+        if (sp.d.drag.n > 0) {
+            if (drag_block < 0 || drag_block >= L->nblocks) return fail(MBL_E_INVALID);
+            // after the cycle the block's state is in plane set cs.pm's bit, the older one in the other
+            const int cur = (int)((cs.pm >> L->level_of[drag_block]) & 1u);
+            for (int k = 0; k < MBL_Q; k++) sp.d.drag.Fold[k] = L->hb[drag_block].P[1 - cur][k];
+        }
+        const uint64_t ds = diag_signature(sp.d, drag_block);
+        if (L->diag_sig != ds) { // captured with other diagnostics arguments: capture again
+            if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(MBL_E_HIP);
+            drop_fused_graphs(L);
+            L->diag_sig = ds;
+        }
     }
-    // hand the new state back: F <-> f for every level that took an odd
-    // number of substeps in this call
-    for (int l = level; l < cfg.LEVEL; l++) {
+    sp.items = std::move(cs.items);
+    // hand the new state back: F <-> f for every level that took an odd number of substeps
+    for (int l = level; l < h->cfg.LEVEL; l++) {
         if (!((cs.pm ^ pm) & (1u << l))) continue;
         for (int a = L->ASP[l]; a < L->ASP[l + 1]; a++)
             for (int k = 0; k < 9; k++) std::swap(pps_d[a].F[k], pps_d[a].f[k]);
     }
+    return 0;
+}
+
+// The launches of consecutive plans (each its cycle, then its diagnostics launch): one
+// captured graph per phase key of the first plan and number of plans, replayed after that.
+static int launch_plans(mbl_context *h, Layout *L, int level, const std::vector<StepPlan> &plans)
+{
+    if (plans.empty()) return 0;
+    const CycleProgram &P = L->prog[level];
+    const int mode = storage_mode(h->cfg);
+    const bool fused = (plans[0].key >> 57) & 1u;
+    auto issue = [&]() -> int {
+        for (const StepPlan &sp : plans) {
+            if (int rc = run_cycle(h, L, sp.items, P, fused)) return rc;
+            if (sp.diag)
+                if (int rc = launch_step_diag(h->stream, mode, sp.d)) return rc;
+        }
+        return 0;
+    };
+    if (!h->cfg.use_graph) return issue();
+    const int n = (int)plans.size();
+    hipGraphExec_t ge = nullptr;
+    if (n == 1) {
+        auto it = L->graphs.find(plans[0].key);
+        if (it != L->graphs.end()) ge = it->second;
+    } else {
+        auto it = L->chunks.find({plans[0].key, n});
+        if (it != L->chunks.end()) ge = it->second;
+    }
+    if (!ge) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return MBL_E_HIP;
+        int rc = issue();
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (rc || e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return rc ? rc : MBL_E_HIP;
+        }
+        hipGraphDestroy(g);
+        if (n == 1)
+            L->graphs[plans[0].key] = ge;
+        else
+            L->chunks[{plans[0].key, n}] = ge;
+    }
+    return hipGraphLaunch(ge, h->stream) == hipSuccess ? 0 : MBL_E_HIP;
+}
+
+// diag (fused only): the step's diagnostics launch after the cycle, in the same graph; its drag
+// reads block drag_block's older state (the caller's f after the call).
+static void evolution_impl(mbl_context *h, const EvoArgs &A, int level, int nblocks, int nline, int nline2,
+                           bool fused, StepDiagArgs *diag, int drag_block)
+{
+    Layout *L = layout_for(h, A, nblocks, nline, nline2);
+    if (!L) return;
+    std::vector<StepPlan> plans(1);
+    if (int rc = plan_step(h, L, A, level, nline, fused, diag, drag_block, (int)h->pend_l2.size(), plans[0])) {
+        set_err(h, rc);
+        return;
+    }
+    if (int rc = launch_plans(h, L, level, plans)) set_err(h, rc);
 }
 
 // ---------------------------------------------------------------------------
@@ -2687,6 +2806,9 @@ extern "C" void getDrag_block(Lbi t, Int3 *d_pdrag, AREA area, D2Q9 d_pp, double
     }
 }
 
+// Fused steps issued as one graph (the planning of the next steps runs ahead on the host).
+constexpr int MBL_CHUNK_STEPS = 6;
+
 // One or more iterations of the reference driver's loop (Multi-block-LBM.cu:144-158):
 // evolution -> error_L2 -> getDrag_block, fused where the layout allows (the residual sums
 // come from the last collide of each level, the rest in one diagnostics launch), otherwise
@@ -2716,7 +2838,7 @@ extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *a
     g_current = h;
     const int last = nblocks - 1;
     EvoArgs A{asp, ls3cffcp, ltcffcp, lccp, areas, pps_d, line_Inf_c, line_Inf_f, line_Inf_cc, line_t3, line_S3};
-    for (Lbi s = 0; s < nsteps && !h->last_error; s++) {
+    for (Lbi s = 0; s < nsteps && !h->last_error;) {
         const Lbi t = n + s;
         Layout *L = layout_for(h, A, nblocks, nline, nline2);
         if (!L) break;
@@ -2725,6 +2847,7 @@ extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *a
                       line_Inf_cc, line_t3, line_S3, h);
             error_L2(t, nblocks, block, grid, areas, pps_d, upsx, upsy, nullptr, nullptr, nullptr, nullptr);
             if (N_drag > 0) getDrag_block(t, d_pdrag, areas[last], pps_d[last], force, N_drag);
+            s++;
             continue;
         }
         // the caller's u_prev arrays in the descriptors the collides read
@@ -2743,8 +2866,6 @@ extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *a
                 break;
             }
         }
-        evolution_impl(h, A, 0, nblocks, nline, nline2, true);
-        if (h->last_error) break;
         StepDiagArgs d;
         memset(&d, 0, sizeof(d));
         d.descs = L->db;
@@ -2765,22 +2886,38 @@ extern "C" int mbl_evolution_step(mbl_handle_t handle, Lbi n, Lbi nsteps, Lbi *a
             }
             h->lparts_stride = d.list_wgs;
         }
+        // the drag's record slot is the residual's (both series advance once per step here)
+        if (N_drag > 0 && h->pend_drag.size() != h->pend_l2.size()) mbl_flush_diagnostics(h);
+        if (h->last_error) break;
         const int slot = (int)h->pend_l2.size();
-        d.lout = h->d_lparts + 3 * (size_t)h->lparts_stride * slot;
-        if (N_drag > 0) {
-            for (int k = 0; k < 9; k++) d.drag.Fold[k] = (void *)pps_d[last].f[k];
+        d.cap = h->cap;
+        d.lbase = h->d_lparts;
+        d.lstride = (size_t)h->lparts_stride;
+        d.drag_base = h->d_drag;
+        if (N_drag > 0) { // the planes (the caller's f after the call) follow from the phase
             d.drag.links = d_pdrag;
             d.drag.n = N_drag;
             d.drag.nx = areas[last].nx;
-            d.drag.out = h->d_drag + 2 * h->pend_drag.size();
         }
-        if (int rc = launch_step_diag(h->stream, storage_mode(cfg), d)) {
+        // up to MBL_CHUNK_STEPS steps, each its cycle and its diagnostics launch (record slots
+        // slot, slot + 1, ...), planned on the host and issued as one graph
+        int K = (int)std::min<Lbi>(nsteps - s, MBL_CHUNK_STEPS);
+        K = std::max(1, std::min(K, h->cap - slot));
+        if (cfg.sync_diag) K = 1;
+        std::vector<StepPlan> plans(K);
+        int rc = 0;
+        for (int i = 0; i < K && !rc; i++) rc = plan_step(h, L, A, 0, nline, true, &d, last, slot + i, plans[i]);
+        if (!rc) rc = launch_plans(h, L, 0, plans);
+        if (rc) {
             set_err(h, rc);
             break;
         }
-        h->pend_l2.push_back(t);
-        h->pend_final.push_back({slot, d.list_wgs});
-        if (N_drag > 0) h->pend_drag.push_back(PendingDrag{t, areas[last].m});
+        for (int i = 0; i < K; i++) {
+            h->pend_l2.push_back((int)(t + i));
+            h->pend_final.push_back({slot + i, d.list_wgs});
+            if (N_drag > 0) h->pend_drag.push_back(PendingDrag{(int)(t + i), areas[last].m});
+        }
+        s += K;
         if (cfg.sync_diag || (int)h->pend_l2.size() >= h->cap || (int)h->pend_drag.size() >= h->cap)
             mbl_flush_diagnostics(h);
         if (force && N_drag > 0) {

@@ -145,7 +145,15 @@ struct StepDiagArgs {
     int list_wgs;      // workgroups over the list (256 entries each, at least one)
     int ntiles;        // stream-collide tiles of all levels: their sums at rparts[0, 3 ntiles)
     double *rparts;    // the tiles' 3 ntiles partial sums
-    double *lout;      // the list workgroups' 3 list_wgs partial sums (launch_residual_steps sums them)
+    // the step's record slot s = *slot_rd (0 <= s < cap; *slot_wr <- s + 1 for the next step):
+    // the list workgroups' 3 list_wgs partial sums go to lbase + 3 lstride s (launch_residual_steps
+    // sums them), the drag to drag_base + 2 s
+    const int *slot_rd;
+    int *slot_wr;
+    int cap;
+    double *lbase;
+    size_t lstride;
+    double *drag_base;
     DragArgs drag;     // drag.n == 0: no drag
 };
 

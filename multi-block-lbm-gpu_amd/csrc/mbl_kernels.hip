@@ -1716,10 +1716,18 @@ __global__ __launch_bounds__(256) void k_residual_cases(const int *__restrict__ 
 // each step stay in a slot of their own until the diagnostics are read back, when one
 // launch sums every step's slots in a fixed order (k_residual_steps: within one launch,
 // workgroups on different XCDs would need a write-back of the L2 to see each other's sums).
+// The slot comes from a device counter, so the launch is the same at every step (it sits in
+// the step's captured graph): read from one of two counters, the next slot written to the
+// other, which the next step reads (no workgroup writes what another reads).
 template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagArgs a)
 {
     __shared__ double sh[16];
     const int wg = (int)blockIdx.x;
+    // this step's record slot (the previous step's diagnostics, or the host, set it); the next
+    // step reads the other counter
+    const int slot = *a.slot_rd;
+    if (slot < 0 || slot >= a.cap) return;
+    if (wg == 0 && threadIdx.x == 0) *a.slot_wr = slot + 1;
     if (wg < a.list_wgs) {
         ResAcc acc;
         const int e = wg * (int)blockDim.x + (int)threadIdx.x;
@@ -1742,9 +1750,9 @@ template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagA
             acc.s2 += a.rparts[3 * (size_t)q + 1];
             acc.mn = fmin(acc.mn, a.rparts[3 * (size_t)q + 2]);
         }
-        tile_resid_store(acc, 1.0, a.lout + 3 * (size_t)wg);
+        tile_resid_store(acc, 1.0, a.lbase + 3 * (a.lstride * (size_t)slot + (size_t)wg));
     } else {
-        drag_case<MODE>(a.drag, a.drag.out, sh);
+        drag_case<MODE>(a.drag, a.drag_base + 2 * (size_t)slot, sh);
     }
 }
 
@@ -2348,7 +2356,9 @@ int launch_drag_batch(void *stream, int mode, const DragArgs *d_args, int ncases
 
 int launch_step_diag(void *stream, int mode, const StepDiagArgs &a)
 {
-    if (a.list_wgs < 1 || a.ntiles < 0 || !a.rparts || !a.lout) return MBL_E_INVALID;
+    if (a.list_wgs < 1 || a.ntiles < 0 || !a.rparts || !a.slot_rd || !a.slot_wr || a.slot_rd == a.slot_wr ||
+        a.cap < 1 || !a.lbase || a.lstride < (size_t)a.list_wgs || (a.drag.n > 0 && !a.drag_base))
+        return MBL_E_INVALID;
     const int grid = a.list_wgs + (a.drag.n > 0 ? 1 : 0);
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_step_diag<M_>), dim3(grid), dim3(256), 0, s, a));
