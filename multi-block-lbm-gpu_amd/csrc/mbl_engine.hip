@@ -1477,11 +1477,12 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
 // markers in force. schedule_cycle() turns the recorded order into launches: it derives
 // every read-after-write, write-after-read and write-after-write dependence on the data the
 // operations touch (collide epochs per level, interface node units, ring slots), gives each
-// operation the earliest position its dependences allow, and merges operations of one kind
-// at one position into one launch (stream-collides of adjacent levels, interface ops of any
-// line, splines of any level; interface ops and one-workgroup splines of one position in one
-// launch). Every read still sees the value it sees in the reference order, so results are
-// bit-identical; the coarse step of the three-level C2b layout takes 16 launches instead of 22.
+// operation the earliest position its dependences allow, and merges the operations of one
+// position into one launch where their kinds allow (stream-collides of adjacent levels,
+// interface ops of any line, splines of any level; interface ops with one-workgroup splines or
+// with a stream-collide; temporal interpolations inside the splines that wrote their slots).
+// Every read still sees the value it sees in the reference order, so results are
+// bit-identical; the coarse step of the three-level C2b layout takes 8 launches instead of 22.
 struct CycleState {
     uint32_t pm;
     int rc;
