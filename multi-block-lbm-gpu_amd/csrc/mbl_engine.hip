@@ -73,6 +73,12 @@ struct CycleProgram {
     std::vector<int> fused_into; // per item: the spline item whose launch does it, or -1
     std::vector<CycleLaunch> launches;
     LineOp *dops = nullptr; // ops of every interface/spline launch, each launch contiguous
+    // the fused step's diagnostics in the cycle's last launch (interface ops only): its list,
+    // where nodes whose macros that launch copies are read at the copies' sources, and the
+    // blocks whose planes that launch writes (the drag's plane set is checked against them)
+    bool dmerge = false;
+    int4 *dlist_m = nullptr;
+    std::vector<int> last_fw_blocks;
 };
 
 // Device tables for one multi-block layout (areas, planes, lines) as passed
@@ -101,7 +107,11 @@ struct Layout {
     // and one diagnostics launch (the nodes no collide writes the macros of, the drag)
     bool step_ok = false;   // non-periodic, one node per thread, macros of collide nodes not rewritten
     int total_tiles = 0;    // stream-collide tiles of all levels
-    int2 *dlist = nullptr;  // (block, node) of the nodes whose macros no collide writes
+    int4 *dlist = nullptr;  // (block, node, block, node) of the nodes whose macros no collide writes
+    std::vector<int4> hlist; // the same on the host
+    // per op of hops: its entries' source and destination nodes and write masks
+    std::vector<std::vector<int>> hsrc, hdst;
+    std::vector<std::vector<uint8_t>> hmsk;
     int nlist = 0;
     double *dres = nullptr; // 3 (total_tiles + list workgroups) partial sums
     std::vector<Lbd *> up_x, up_y; // u_prev arrays installed in the descriptors
@@ -118,6 +128,7 @@ struct Layout {
     {
         for (auto &p : prog) {
             if (p.dops) hipFree(p.dops);
+            if (p.dlist_m) hipFree(p.dlist_m);
             p = CycleProgram();
         }
         for (auto &g : graphs) hipGraphExecDestroy(g.second);
@@ -1414,6 +1425,9 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
             op.pcr_levels = pcr_of[j].second;
         }
         L->hops.push_back(op);
+        L->hsrc.push_back(o.src);
+        L->hdst.push_back(o.dst);
+        L->hmsk.push_back(o.mask);
     }
     bool ok = hipMemcpy(L->db, L->hb.data(), sizeof(BlockDesc) * nblocks, hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !tile_blk.empty())
@@ -1448,16 +1462,17 @@ static int build_layout(mbl_context *h, const EvoArgs &A, int nblocks, int nline
         for (size_t e = 0; e < o.mask.size() && step_ok; e++)
             if ((o.mask[e] & 1) && summed_by_collide(o.op.dst_block, (size_t)o.dst[e])) step_ok = false;
     if (step_ok) {
-        std::vector<int2> list;
+        std::vector<int4> &list = L->hlist;
+        list.clear();
         for (int a = 0; a < nblocks; a++)
             for (size_t i = 0; i < hflags[a].size(); i++)
-                if (!summed_by_collide(a, i)) list.push_back(make_int2(a, (int)i));
+                if (!summed_by_collide(a, i)) list.push_back(make_int4(a, (int)i, a, (int)i));
         L->nlist = (int)list.size();
         const int wgs = std::max(1, (L->nlist + 255) / 256);
-        ok = hipMalloc(&L->dlist, sizeof(int2) * std::max<size_t>(1, list.size())) == hipSuccess &&
+        ok = hipMalloc(&L->dlist, sizeof(int4) * std::max<size_t>(1, list.size())) == hipSuccess &&
              hipMalloc(&L->dres, sizeof(double) * 3 * ((size_t)L->total_tiles + wgs)) == hipSuccess &&
              (list.empty() ||
-              hipMemcpy(L->dlist, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice) == hipSuccess);
+              hipMemcpy(L->dlist, list.data(), sizeof(int4) * list.size(), hipMemcpyHostToDevice) == hipSuccess);
         if (!ok) {
             L->release();
             delete L;
@@ -1850,6 +1865,49 @@ static int schedule_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem
             }
         }
     }
+    // the diagnostics in the last launch: it holds interface ops only, and every node of the
+    // list whose macros it writes is a copy (same-level or f_c) of a node whose macros it does
+    // not write
+    if (P.dlist_m) hipFree(P.dlist_m);
+    P.dlist_m = nullptr;
+    P.dmerge = false;
+    P.last_fw_blocks.clear();
+    if (L->step_ok && !h->fork && !P.launches.empty() && P.launches.back().kind == 1 && !L->hlist.empty()) {
+        const CycleLaunch &cl = P.launches.back();
+        std::unordered_map<uint64_t, std::pair<int, int>> w; // macro unit -> its copy's source (-1: no copy)
+        for (int j : cl.items)
+            for (int o = items[j].off; o < items[j].off + items[j].count; o++) {
+                const LineOp &op = L->hops[o];
+                for (size_t e = 0; e < L->hmsk[o].size(); e++) {
+                    if (L->hmsk[o][e] & 2) P.last_fw_blocks.push_back(op.dst_block);
+                    if (!(L->hmsk[o][e] & 1)) continue;
+                    const bool copy = (op.type == OP_CC || op.type == OP_FC) && e < L->hsrc[o].size();
+                    w[unit(op.dst_block, (size_t)L->hdst[o][e], 0)] =
+                        copy ? std::make_pair(op.src_block, L->hsrc[o][e]) : std::make_pair(-1, -1);
+                }
+            }
+        std::sort(P.last_fw_blocks.begin(), P.last_fw_blocks.end());
+        P.last_fw_blocks.erase(std::unique(P.last_fw_blocks.begin(), P.last_fw_blocks.end()), P.last_fw_blocks.end());
+        std::vector<int4> ml = L->hlist;
+        bool ok = true;
+        for (int4 &q : ml) {
+            auto it = w.find(unit(q.x, (size_t)q.y, 0));
+            if (it == w.end()) continue;
+            const std::pair<int, int> src = it->second;
+            if (src.first < 0 || w.count(unit(src.first, (size_t)src.second, 0))) {
+                ok = false;
+                break;
+            }
+            q.z = src.first;
+            q.w = src.second;
+        }
+        if (ok) {
+            if (hipMalloc(&P.dlist_m, sizeof(int4) * ml.size()) != hipSuccess) return set_err(h, MBL_E_NOMEM);
+            if (hipMemcpy(P.dlist_m, ml.data(), sizeof(int4) * ml.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return set_err(h, MBL_E_HIP);
+            P.dmerge = true;
+        }
+    }
     if (P.dops) hipFree(P.dops);
     P.dops = nullptr;
     if (!tab.empty()) {
@@ -1985,8 +2043,9 @@ static int side_streams(mbl_context *h)
 // dependence among them: the first runs on the handle's stream, the others on side streams
 // forked from it and joined back before the next position (in a captured graph: parallel
 // branches).
+// mdiag: the step's diagnostics, launched with the last launch (CycleProgram::dmerge).
 static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &items, const CycleProgram &P,
-                     bool fused)
+                     bool fused, const StepDiagArgs *mdiag = nullptr)
 {
     if ((int)items.size() != P.nitems) return MBL_E_INVALID;
     const size_t nl = P.launches.size();
@@ -2003,7 +2062,21 @@ static int run_cycle(mbl_context *h, Layout *L, const std::vector<CycleItem> &it
         for (size_t m = i; m < j; m++) {
             const int r = (int)(m - i);
             hipStream_t st = (k > 1 && r > 0) ? h->side[(r - 1) % 3] : h->stream;
-            if (int rc = launch_one(h, L, items, P, P.launches[m], st, fused)) return rc;
+            const CycleLaunch &cl = P.launches[m];
+            if (mdiag && m == nl - 1) {
+                if (cl.kind != 1 || k > 1) return MBL_E_INVALID;
+                uint32_t pm;
+                MarkerSet ms;
+                if (launch_state(L, items, cl, pm, ms)) return MBL_E_INVALID;
+                double xm = 0.0;
+                for (int q : cl.items)
+                    if (items[q].temporal) xm = items[q].xm;
+                if (int rc = launch_ops_diag(st, storage_mode(h->cfg), P.dops + cl.op_off, cl.op_count, cl.max_n, L->db,
+                                             pm, ms, xm, *mdiag))
+                    return rc;
+                continue;
+            }
+            if (int rc = launch_one(h, L, items, P, cl, st, fused)) return rc;
         }
         if (k > 1)
             for (int s = 0; s < std::min(k - 1, 3); s++) {
@@ -2125,6 +2198,7 @@ struct StepPlan {
     uint64_t key = 0;
     std::vector<CycleItem> items;
     bool diag = false;
+    bool merge = false; // the diagnostics go with the cycle's last launch (CycleProgram::dmerge)
     StepDiagArgs d;
 };
 
@@ -2185,7 +2259,18 @@ static int plan_step(mbl_context *h, Layout *L, const EvoArgs &A, int level, int
             const int cur = (int)((cs.pm >> L->level_of[drag_block]) & 1u);
             for (int k = 0; k < MBL_Q; k++) sp.d.drag.Fold[k] = L->hb[drag_block].P[1 - cur][k];
         }
-        const uint64_t ds = diag_signature(sp.d, drag_block);
+        // with the last launch when its ops write no plane the drag reads (the drag block's older set)
+        sp.merge = P.dmerge && P.dlist_m;
+        if (sp.merge && sp.d.drag.n > 0 &&
+            std::binary_search(P.last_fw_blocks.begin(), P.last_fw_blocks.end(), drag_block)) {
+            uint32_t lpm = 0;
+            MarkerSet lms;
+            if (launch_state(L, cs.items, P.launches.back(), lpm, lms)) return fail(MBL_E_INVALID);
+            const int cur = (int)((cs.pm >> L->level_of[drag_block]) & 1u);
+            if ((int)((lpm >> L->level_of[drag_block]) & 1u) == 1 - cur) sp.merge = false;
+        }
+        if (sp.merge) sp.d.list = P.dlist_m;
+        const uint64_t ds = diag_signature(sp.d, drag_block) ^ (sp.merge ? 0x5A5Aull : 0ull);
         if (L->diag_sig != ds) { // captured with other diagnostics arguments: capture again
             if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(MBL_E_HIP);
             drop_fused_graphs(L);
@@ -2212,8 +2297,8 @@ static int launch_plans(mbl_context *h, Layout *L, int level, const std::vector<
     const bool fused = (plans[0].key >> 57) & 1u;
     auto issue = [&]() -> int {
         for (const StepPlan &sp : plans) {
-            if (int rc = run_cycle(h, L, sp.items, P, fused)) return rc;
-            if (sp.diag)
+            if (int rc = run_cycle(h, L, sp.items, P, fused, sp.diag && sp.merge ? &sp.d : nullptr)) return rc;
+            if (sp.diag && !sp.merge)
                 if (int rc = launch_step_diag(h->stream, mode, sp.d)) return rc;
         }
         return 0;

@@ -140,7 +140,10 @@ struct DragArgs {
 // not sum and the drag, then the final sums).
 struct StepDiagArgs {
     const BlockDesc *descs;
-    const int2 *list;  // (block, node) of every node the stream-collides do not sum
+    // every node the stream-collides do not sum: (block, node, block, node whose macros it holds
+    // when the step's diagnostics run; the node itself, or the source of the copy that writes it
+    // in the same launch)
+    const int4 *list;
     int nlist;
     int list_wgs;      // workgroups over the list (256 entries each, at least one)
     int ntiles;        // stream-collide tiles of all levels: their sums at rparts[0, 3 ntiles)
@@ -271,6 +274,10 @@ int launch_residual_cases(void *stream, const BlockDesc *d_descs, const int *d_t
                           double *partials, int ncases, const int *d_case_tiles, double *d_out);
 int launch_drag(void *stream, int mode, const DragArgs &a);
 int launch_step_diag(void *stream, int mode, const StepDiagArgs &a);
+// The interface ops of the cycle's last position and the step's diagnostics in one launch
+// (the diagnostics read nothing the ops write: launch_plans checks that when it plans it).
+int launch_ops_diag(void *stream, int mode, const LineOp *d_ops, int nops, int max_n, const BlockDesc *d_blocks,
+                    uint32_t parity_mask, const MarkerSet &ms, double xm, const StepDiagArgs &a);
 // error_L2 records of nsteps steps: step b sums partials[3 (b stride) + 3 q], q < nparts, in
 // order into out[4 b .. 4 b + 3] (residual, s1, s2, smallest density), as k_residual_final does
 int launch_residual_steps(void *stream, int nsteps, int nparts, size_t stride, const double *partials, double *out);

@@ -1719,10 +1719,8 @@ __global__ __launch_bounds__(256) void k_residual_cases(const int *__restrict__ 
 // The slot comes from a device counter, so the launch is the same at every step (it sits in
 // the step's captured graph): read from one of two counters, the next slot written to the
 // other, which the next step reads (no workgroup writes what another reads).
-template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagArgs a)
+template <int MODE> __device__ __forceinline__ void step_diag_wg(const StepDiagArgs &a, int wg, double *sh)
 {
-    __shared__ double sh[16];
-    const int wg = (int)blockIdx.x;
     // this step's record slot (the previous step's diagnostics, or the host, set it); the next
     // step reads the other counter
     const int slot = *a.slot_rd;
@@ -1732,10 +1730,10 @@ template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagA
         ResAcc acc;
         const int e = wg * (int)blockDim.x + (int)threadIdx.x;
         if (e < a.nlist) {
-            const int2 q = a.list[e];
-            const BlockDesc &d = a.descs[q.x];
-            const uint32_t o = (uint32_t)q.y;
-            const Lbd ux = d.ux[o], uy = d.uy[o], r = d.rho[o], px = d.upx[o], py = d.upy[o];
+            const int4 q = a.list[e];
+            const BlockDesc &d = a.descs[q.x], &m = a.descs[q.z];
+            const uint32_t o = (uint32_t)q.y, om = (uint32_t)q.w;
+            const Lbd ux = m.ux[om], uy = m.uy[om], r = m.rho[om], px = d.upx[o], py = d.upy[o];
             const double x = (double)ux, y = (double)uy, x0 = (double)px, y0 = (double)py;
             acc.mn = (double)r;
             acc.s1 = ((x - x0) * (x - x0) + (y - y0) * (y - y0)) * d.inv_m;
@@ -1754,6 +1752,32 @@ template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagA
     } else {
         drag_case<MODE>(a.drag, a.drag_base + 2 * (size_t)slot, sh);
     }
+}
+
+template <int MODE> __global__ __launch_bounds__(256) void k_step_diag(StepDiagArgs a)
+{
+    __shared__ double sh[16];
+    step_diag_wg<MODE>(a, (int)blockIdx.x, sh);
+}
+
+// The cycle's last interface ops (workgroups [0, nops opx)) and the step's diagnostics (the
+// workgroups after them) in one launch.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ops_diag(const LineOp *__restrict__ ops, int nops, int opx,
+                                                  const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
+                                                  MarkerSet ms, double xm, StepDiagArgs a)
+{
+    __shared__ double sh[16];
+    const int b = (int)blockIdx.x, nb = nops * opx;
+    if (b >= nb) {
+        step_diag_wg<MODE>(a, b - nb, sh);
+        return;
+    }
+    const int y = b / opx;
+    const LineOp &op = ops[y];
+    const int e = (b - y * opx) * (int)blockDim.x + (int)threadIdx.x;
+    if (e >= op.n) return;
+    line_op_entry<MODE>(op, e, blocks, parity_mask, ms, xm);
 }
 
 // Zeroth and first moments of one block's 9 planes (sum of f, f e_x, f e_y over every node)
@@ -2362,6 +2386,22 @@ int launch_step_diag(void *stream, int mode, const StepDiagArgs &a)
     const int grid = a.list_wgs + (a.drag.n > 0 ? 1 : 0);
     hipStream_t s = (hipStream_t)stream;
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_step_diag<M_>), dim3(grid), dim3(256), 0, s, a));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_ops_diag(void *stream, int mode, const LineOp *d_ops, int nops, int max_n, const BlockDesc *d_blocks,
+                    uint32_t parity_mask, const MarkerSet &ms, double xm, const StepDiagArgs &a)
+{
+    if (nops <= 0 || max_n <= 0 || a.list_wgs < 1 || a.ntiles < 0 || !a.rparts || !a.slot_rd || !a.slot_wr ||
+        a.slot_rd == a.slot_wr || a.cap < 1 || !a.lbase || a.lstride < (size_t)a.list_wgs ||
+        (a.drag.n > 0 && !a.drag_base))
+        return MBL_E_INVALID;
+    const int opx = (max_n + 255) / 256;
+    dim3 g(nops * opx + a.list_wgs + (a.drag.n > 0 ? 1 : 0)), b(256);
+    hipStream_t s = (hipStream_t)stream;
+    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_ops_diag<M_>), g, b, 0, s, d_ops, nops, opx, d_blocks, parity_mask, ms,
+                                             xm, a));
     MBL_LAUNCH_CHECK();
     return 0;
 }
