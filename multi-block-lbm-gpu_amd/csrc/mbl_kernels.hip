@@ -1278,7 +1278,12 @@ __global__ __launch_bounds__(256) void k_spline_eval(const LineOp *__restrict__ 
 // macros. A column's arithmetic does not depend on the others, so every split
 // gives the same bits.
 constexpr int SPL_FUSED_MAX_NC = 303; // 24 * 8 * nc bytes of LDS within 64 KiB (and the drag's)
-constexpr int SPL_GROUPS = 9;         // workgroups per line of the spline launches
+
+// Workgroups per line of a spline launch of nlines lines: 9 while the launch is far from filling
+// the GPU (each workgroup solves 4 of the 12 columns), 3 for more lines (6 columns each: the
+// rho, ux, uy columns are solved by every workgroup). Measured on the 256-case sweep (1,024
+// lines): 27.6k MLUPS with 9, 29.0k with 3, 27.7k with 1.
+static inline int spline_groups(int nlines) { return nlines * 9 <= 1024 ? 9 : 3; }
 
 template <int MODE, int NG = 1>
 __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *__restrict__ blocks,
@@ -1428,19 +1433,19 @@ __device__ __forceinline__ void spline_line(const LineOp &op, const BlockDesc *_
     }
 }
 
-template <int MODE>
+template <int MODE, int NG>
 __global__ __launch_bounds__(1024) void k_spline_fused(const LineOp *__restrict__ ops,
                                                        const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
                                                        MarkerSet ms)
 {
     extern __shared__ double lds[];
-    spline_line<MODE, SPL_GROUPS>(ops[blockIdx.y], blocks, parity_mask, ms, lds, (int)blockIdx.x, ops);
+    spline_line<MODE, NG>(ops[blockIdx.y], blocks, parity_mask, ms, lds, (int)blockIdx.x, ops);
 }
 
 // One-workgroup splines and interface ops of one position of the scheduled cycle in one
-// launch: workgroup row y < nspl runs spline y (its first SPL_GROUPS workgroups, one share of
-// the line's columns each), the rows after it the entries of interface op y (one entry per thread).
-template <int MODE>
+// launch: workgroup row y < nspl runs spline y (its first NG workgroups, one share of the
+// line's columns each), the rows after it the entries of interface op y (one entry per thread).
+template <int MODE, int NG>
 __global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__ ops, int nspl,
                                                       const BlockDesc *__restrict__ blocks, uint32_t parity_mask,
                                                       MarkerSet ms, double xm)
@@ -1448,8 +1453,7 @@ __global__ __launch_bounds__(1024) void k_iface_mixed(const LineOp *__restrict__
     extern __shared__ double lds[];
     const int y = (int)blockIdx.y;
     if (y < nspl) {
-        if (blockIdx.x < SPL_GROUPS)
-            spline_line<MODE, SPL_GROUPS>(ops[y], blocks, parity_mask, ms, lds, (int)blockIdx.x, ops);
+        if (blockIdx.x < NG) spline_line<MODE, NG>(ops[y], blocks, parity_mask, ms, lds, (int)blockIdx.x, ops);
         return;
     }
     const LineOp &op = ops[y];
@@ -2292,10 +2296,15 @@ int launch_spline(void *stream, int mode, const LineOp *d_ops, int nops, int max
     if (max_nc <= SPL_FUSED_MAX_NC) {
         // one thread per coarse row, and per fine point where the workgroup allows (up to 1024)
         const int threads = std::min(1024, (std::max(max_nc, max_nf) + 63) / 64 * 64);
-        const size_t lds = 2 * (3 + 9 / SPL_GROUPS) * sizeof(double) * (size_t)max_nc;
+        const int ng = spline_groups(nops);
+        const size_t lds = 2 * (3 + 9 / ng) * sizeof(double) * (size_t)max_nc;
         hipStream_t s = (hipStream_t)stream;
-        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_>), dim3(SPL_GROUPS, nops), dim3(threads), lds, s, d_ops,
-                                                 d_blocks, parity_mask, ms));
+        const dim3 g(ng, nops), b(threads);
+        if (ng == 9) {
+            MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_, 9>), g, b, lds, s, d_ops, d_blocks, parity_mask, ms));
+        } else {
+            MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_spline_fused<M_, 3>), g, b, lds, s, d_ops, d_blocks, parity_mask, ms));
+        }
         MBL_LAUNCH_CHECK();
         return 0;
     }
@@ -2318,11 +2327,17 @@ int launch_iface_mixed(void *stream, int mode, const LineOp *d_ops, int nspl, in
 {
     if (nspl <= 0 || max_nc > SPL_FUSED_MAX_NC) return MBL_E_INVALID;
     const int threads = std::min(1024, (std::max(max_nc, max_nf) + 63) / 64 * 64);
-    const size_t lds = 2 * (3 + 9 / SPL_GROUPS) * sizeof(double) * (size_t)max_nc;
-    dim3 g(std::max(SPL_GROUPS, (max_n + threads - 1) / threads), nspl + std::max(0, nops)), b(threads);
+    const int ng = spline_groups(nspl);
+    const size_t lds = 2 * (3 + 9 / ng) * sizeof(double) * (size_t)max_nc;
+    dim3 g(std::max(ng, (max_n + threads - 1) / threads), nspl + std::max(0, nops)), b(threads);
     hipStream_t s = (hipStream_t)stream;
-    MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_iface_mixed<M_>), g, b, lds, s, d_ops, nspl, d_blocks, parity_mask, ms,
-                                             xm));
+    if (ng == 9) {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_iface_mixed<M_, 9>), g, b, lds, s, d_ops, nspl, d_blocks, parity_mask,
+                                                 ms, xm));
+    } else {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_iface_mixed<M_, 3>), g, b, lds, s, d_ops, nspl, d_blocks, parity_mask,
+                                                 ms, xm));
+    }
     MBL_LAUNCH_CHECK();
     return 0;
 }

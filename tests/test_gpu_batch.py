@@ -148,6 +148,23 @@ def test_persistent_equals_graph(lib64, lib32, fp64):
     p.close()
 
 
+def test_persistent_equals_graph_many_lines(lib32):
+    """32 cases of the sweep (128 spline lines: the graph's spline launches spread a line over 3
+    workgroups, the one-launch path solves it in one) give the same bits on both paths."""
+    specs = mbl.c4_specs(32, seed=7)
+    g = mbl.Batch(lib32, specs)
+    p = mbl.Batch(lib32, specs, persistent=True)
+    g.step(9)
+    p.step(9)
+    for x, y in zip(batch_state(g, len(specs)), batch_state(p, len(specs))):
+        assert np.array_equal(x, y)
+    cg, lg = g.forces(9)
+    cp, lp = p.forces(9)
+    assert np.array_equal(cg, cp) and np.array_equal(lg, lp)
+    g.close()
+    p.close()
+
+
 def test_batch_residual_matches_error_l2(lib64):
     """The per-case residual of a batch (mbl_batch_residual) equals error_L2 of the same case
     run alone, step after step (FP64)."""
