@@ -611,15 +611,18 @@ __device__ inline void flagged_node(const BlockDesc &d, const typename Codec<MOD
         }
     }
     const bool upd = flag == FLUID || nee;
+    // plain stores: the planes of a multi-block layout stay in the L2 and the last-level cache
+    // for the launches after this one (non-temporal stores measured 18 % slower on the
+    // three-level case, 11 % on the 256-case sweep)
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         const T v = upd ? C::st(Q[k]) : (solid ? raw[OPP[k]] : raw[k]);
-        __builtin_nontemporal_store(v, off32(D[k], o));
+        *off32(D[k], o) = v;
     }
     if (MACROS && upd) {
-        __builtin_nontemporal_store((Lbd)rho, off32((GLOBAL_AS Lbd *)d.rho, o));
-        __builtin_nontemporal_store((Lbd)ux, off32((GLOBAL_AS Lbd *)d.ux, o));
-        __builtin_nontemporal_store((Lbd)uy, off32((GLOBAL_AS Lbd *)d.uy, o));
+        *off32((GLOBAL_AS Lbd *)d.rho, o) = (Lbd)rho;
+        *off32((GLOBAL_AS Lbd *)d.ux, o) = (Lbd)ux;
+        *off32((GLOBAL_AS Lbd *)d.uy, o) = (Lbd)uy;
         if (rnode) resid_node(d, o, (Lbd)rho, (Lbd)ux, (Lbd)uy, px, py, *acc);
     }
 }
