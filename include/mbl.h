@@ -435,9 +435,16 @@ int mbl_set_cycle_fork(mbl_handle_t h, int on);
 int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 *pps_d, double *out);
 
 /* Advances one block by `nsteps` fused stream-collide steps on its own
- * (no interface work), swapping pp->F / pp->f after every step. This is the
- * path of the single-block periodic bandwidth benchmark. */
+ * (no interface work), swapping pp->F / pp->f after every launch. This is the
+ * path of the single-block periodic bandwidth benchmark. A periodic all-fluid
+ * block whose width is a multiple of 64 and height a multiple of 16 (8 for
+ * FP64 planes) takes two steps per launch: pp->F then holds the same state as
+ * after single steps, bit for bit, and pp->f the state two steps back (one step
+ * back after single steps); an odd remainder takes one single step. */
 int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
+/* on == 0: mbl_step_block advances one step per launch on every block (default on: two steps
+ * per launch where it can). No reference counterpart. */
+int mbl_set_step_pairs(mbl_handle_t h, int on);
 
 /* Invariants of one block's state (the F planes): out[0] = sum of f over every node and
  * direction (the mass), out[1], out[2] = sum of f e_x, f e_y (the momentum), out[3] = number

@@ -224,6 +224,7 @@ struct mbl_context {
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     bool fork = false; // launches at one position run concurrently (mbl_set_cycle_fork)
+    bool step_pairs = true; // mbl_step_block: periodic all-fluid blocks two steps per launch
 };
 
 static mbl_context *g_current = nullptr;
@@ -2397,13 +2398,41 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     int par = (pp->F[0] == (Lbd *)S.hd.P[0][0]) ? 0 : 1;
     const int mode = storage_mode(h->cfg);
     const bool macros = !S.periodic || S.flags;
-    for (int t = 0; t < nsteps; t++) {
-        int rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, nullptr, 1,
+    // periodic all-fluid blocks: two steps per launch where the shape allows (the state at
+    // step n + 2 lands in the other plane set, which then holds it as one step would have
+    // left it; the plane set handed back as f holds step n, not n + 1)
+    const bool pairs = h->step_pairs && S.periodic && !S.flags && periodic2_fits(mode, area->nx, area->ny);
+    for (int t = 0; t < nsteps;) {
+        int rc;
+        if (pairs && t + 2 <= nsteps) {
+            Periodic2Args a;
+            memset(&a, 0, sizeof(a));
+            for (int k = 0; k < 9; k++) {
+                a.S[k] = S.hd.P[par][k];
+                a.D[k] = S.hd.P[par ^ 1][k];
+            }
+            a.nx = area->nx;
+            a.ny = area->ny;
+            a.omega = S.hd.omega;
+            rc = launch_periodic2(h->stream, mode, a);
+            t += 2;
+        } else {
+            rc = launch_stream_collide(h->stream, mode, S.vx, S.tpb, S.periodic, S.flags, macros, S.dd, nullptr, 1,
                                        S.tiles, S.rows, (uint32_t)par, h->groups, 0);
+            t += 1;
+        }
         if (rc) return set_err(h, rc);
         par ^= 1;
         for (int k = 0; k < 9; k++) std::swap(pp->F[k], pp->f[k]);
     }
+    return 0;
+}
+
+extern "C" int mbl_set_step_pairs(mbl_handle_t h, int on)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    h->step_pairs = on != 0;
     return 0;
 }
 

@@ -228,6 +228,16 @@ int launch_cases(void *stream, int mode, int vx, const CaseArgs &a, size_t lds);
 // rmask != 0 (flagged, macros, one node per thread): the tiles of the levels in rmask also add
 // error_L2's terms of their fluid, inlet and outlet nodes off the block edges (u_prev <- u) and
 // write the tile's weighted sums to rparts[3 t .. 3 t + 2], t the tile's number across all levels.
+// Two time steps of a periodic all-fluid nx x ny block in one launch: S (state at step n) ->
+// D (state at step n + 2); periodic2_fits: whether the block's shape allows it.
+struct Periodic2Args {
+    const void *S[MBL_Q];
+    void *D[MBL_Q];
+    int nx, ny, tiles_x;
+    double omega;
+};
+int periodic2_fits(int mode, int nx, int ny);
+int launch_periodic2(void *stream, int mode, const Periodic2Args &a);
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
                           uint32_t parity_mask, int groups, int tile_base, uint32_t rmask = 0,

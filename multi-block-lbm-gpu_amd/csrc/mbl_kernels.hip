@@ -877,6 +877,69 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
 }
 
 // ---------------------------------------------------------------------------
+// Two time steps of a periodic all-fluid block in one pass (temporal blocking). A workgroup
+// owns an output tile of P2_TX x TY nodes: it collides step n+1 on the tile grown by one node
+// on every side (pulling step n from the planes, periodic wrap), keeps those post-collision
+// values in LDS as the planes would hold them (C::st), then collides step n+2 on the tile
+// pulling from LDS and writes it to the other plane set. Every node update is the single-step
+// kernel's (the same loads, collide_c and stores, in the same arithmetic), so two single steps
+// and one paired launch give the same bits in the plane set that holds the state; the
+// intermediate step n+1 is not written (the other plane set keeps step n). Per two node
+// updates it moves 36 B read + 36 B written (FP32) instead of twice that; the step-n+1 ring is
+// the overhead: (P2_TX + 2)(TY + 2) collisions per P2_TX TY outputs.
+// ---------------------------------------------------------------------------
+constexpr int P2_TX = 64;
+
+template <int MODE, int TY>
+__global__ __launch_bounds__(256) void k_periodic2(Periodic2Args a)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    typedef typename C::R R;
+    typedef typename C::GT GT;
+    constexpr int AX = P2_TX + 2, AY = TY + 2;
+    __shared__ T L[9][AY][AX];
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t unx = (uint32_t)nx;
+    const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int ty = b / a.tiles_x, tx = b - ty * a.tiles_x;
+    const int x0 = tx * P2_TX, y0 = ty * TY;
+    const R omega = (R)a.omega;
+    // step n+1 on the grown tile
+    for (int idx = (int)threadIdx.x; idx < AX * AY; idx += (int)blockDim.x) {
+        const int ly = idx / AX, lx = idx - ly * AX;
+        int x = x0 - 1 + lx, y = y0 - 1 + ly;
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        R q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            int xs = x - EX[k], ys = y - EY[k];
+            xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+            ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
+            q[k] = C::ld(*off32((const GT *)a.S[k], (uint32_t)ys * unx + (uint32_t)xs));
+        }
+        R Q[9], rho, ux, uy;
+        collide_c<C>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) L[k][ly][lx] = C::st(Q[k]);
+    }
+    __syncthreads();
+    // step n+2 on the tile, pulling step n+1 from LDS
+    for (int idx = (int)threadIdx.x; idx < P2_TX * TY; idx += (int)blockDim.x) {
+        const int ly = idx / P2_TX, lx = idx - ly * P2_TX;
+        R q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[k] = C::ld(L[k][ly + 1 - EY[k]][lx + 1 - EX[k]]);
+        R Q[9], rho, ux, uy;
+        collide_c<C>(q, omega, Q, rho, ux, uy);
+        const uint32_t o = (uint32_t)(y0 + ly) * unx + (uint32_t)(x0 + lx);
+#pragma unroll
+        for (int k = 0; k < 9; k++) __builtin_nontemporal_store(C::st(Q[k]), off32((GT *)a.D[k], o));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // init_General (lbm/init_block.h:2-51)
 // ---------------------------------------------------------------------------
 __device__ inline uint64_t splitmix64(uint64_t z)
@@ -2462,6 +2525,35 @@ int launch_encode(void *stream, int mode, void *plane, int k, size_t n, const do
     hipStream_t s = (hipStream_t)stream;
     dim3 g((unsigned)((n + 255) / 256)), b(256);
     MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_encode<M_>), g, b, 0, s, plane, k, n, in));
+    MBL_LAUNCH_CHECK();
+    return 0;
+}
+
+// Rows per paired tile: the step-n+1 values of one tile in LDS stay within 48 KiB.
+static int periodic2_rows(int mode)
+{
+    return store_elem_bytes(mode) > 4 ? 8 : 16;
+}
+
+int periodic2_fits(int mode, int nx, int ny)
+{
+    const int ty = periodic2_rows(mode);
+    return nx >= P2_TX && ny >= ty && nx % P2_TX == 0 && ny % ty == 0;
+}
+
+int launch_periodic2(void *stream, int mode, const Periodic2Args &args)
+{
+    if (!periodic2_fits(mode, args.nx, args.ny)) return MBL_E_INVALID;
+    Periodic2Args a = args;
+    const int ty = periodic2_rows(mode);
+    a.tiles_x = a.nx / P2_TX;
+    dim3 g(a.tiles_x * (a.ny / ty)), b(256);
+    hipStream_t s = (hipStream_t)stream;
+    if (ty == 16) {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_periodic2<M_, 16>), g, b, 0, s, a));
+    } else {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_periodic2<M_, 8>), g, b, 0, s, a));
+    }
     MBL_LAUNCH_CHECK();
     return 0;
 }

@@ -96,6 +96,7 @@ def load_library(fp64=False):
         "mbl_set_cycle_fork": (ip, [vp, ip]),
         "mbl_error_L2_noalloc": (ip, [vp, ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(vp), C.POINTER(vp), vp]),
         "mbl_step_block": (ip, [vp, C.POINTER(AREA), C.POINTER(D2Q9), ip]),
+        "mbl_set_step_pairs": (ip, [vp, ip]),
         "mbl_timer_begin": (ip, [vp]),
         "mbl_timer_end": (ip, [vp, dp]),
         "mbl_set_tuning": (ip, [vp, ip, ip]),

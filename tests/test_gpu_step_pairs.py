@@ -1,0 +1,62 @@
+"""mbl_step_block on a periodic all-fluid block: two time steps per launch (the step-n+1 values
+kept in LDS) against one step per launch (mbl_set_step_pairs(h, 0)), every storage mode and the
+FP64 build: the state in F bit for bit after even and odd step counts, and a block whose shape
+does not allow pairs stepping one step per launch either way."""
+import numpy as np
+import pytest
+
+import mbl
+from mbl import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def states(lib, n, ny, storage, steps, pairs, sigma=0.02):
+    blk = mbl.PeriodicBlock(lib, n, ny, tau=0.6, storage=storage, sigma=sigma)
+    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, 1 if pairs else 0), "pairs")
+    out = []
+    done = 0
+    for s in steps:
+        blk.step(s - done)
+        done = s
+        out.append([blk.read_F(k) for k in range(9)])
+    blk.close()
+    return out
+
+
+@pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F32S, abi.STORE_F16S, abi.STORE_F16D])
+def test_pairs_equal_single_steps(lib32, storage):
+    steps = [2, 7, 12]  # 1 pair; then 2 pairs + 1 single; then 2 pairs + 1 single
+    a = states(lib32, 128, 64, storage, steps, True)
+    b = states(lib32, 128, 64, storage, steps, False)
+    for sa, sb in zip(a, b):
+        for k in range(9):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_pairs_equal_single_steps_fp64(lib64):
+    steps = [4, 9]
+    a = states(lib64, 64, 40, abi.STORE_NATIVE, steps, True)
+    b = states(lib64, 64, 40, abi.STORE_NATIVE, steps, False)
+    for sa, sb in zip(a, b):
+        for k in range(9):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_shape_without_pairs(lib32):
+    """96 x 40: the width is not a multiple of 64, so every launch is a single step."""
+    a = states(lib32, 96, 40, abi.STORE_NATIVE, [6], True)
+    b = states(lib32, 96, 40, abi.STORE_NATIVE, [6], False)
+    for k in range(9):
+        assert np.array_equal(a[0][k], b[0][k])
+
+
+def test_pairs_conserve_mass(lib32):
+    blk = mbl.PeriodicBlock(lib32, 256, 128, tau=0.6, sigma=0.02)
+    m0 = blk.moments()
+    blk.step(40)
+    m1 = blk.moments()
+    blk.close()
+    assert m1[3] == 0
+    # FP32 planes: each step rounds every value once (4e-10 relative measured over 40 steps)
+    assert abs(m1[0] - m0[0]) / m0[0] < 1e-8
