@@ -2,11 +2,16 @@
 """Bandwidth benchmark of the fused D2Q9 stream-collide path on one MI355X.
 
 Workload: one periodic 16384 x 16384 block, rho = 1, (u, v) ~ N(0, 0.01^2)
-from a fixed seed, tau = 0.6; one step = one fused pull-stream + BGK launch
-over all nodes, F planes in HBM the whole time. MLUPS = nodes * steps / wall
-seconds. The roofline figure uses the algorithmic 72 B per node update of
-FP32 planes (9 loads + 9 stores of 4 B) against the 8 TB/s HBM peak, with the
-per-launch time taken from HIP events on the library's stream.
+from a fixed seed, tau = 0.6; one step = one fused pull-stream + BGK update of
+all nodes, F planes in HBM the whole time. The library takes two steps per
+launch on this block (k_periodic2: step n+1 of a tile and a one-node ring in
+LDS, then step n+2; the same bits as two single steps), so K steps are K/2
+launches. MLUPS = nodes * steps / wall seconds. The roofline figure uses the
+algorithmic bytes of one launch, 72 B per node for FP32 planes (9 loads of
+step n + 9 stores of step n+2, 4 B each; two node updates), against the
+8 TB/s HBM peak, with the per-launch time taken from HIP events on the
+library's stream. The single-step kernel (k_stream_collide, 72 B per node
+update) is timed beside it (`single_step`).
 
 FP32 planes are updated in FP64 arithmetic (same per-launch time as FP32
 arithmetic on this HBM-bound kernel; `dtype` names the arithmetic, `storage`
@@ -37,20 +42,23 @@ BYTES_PER_NODE = {0: 72, 1: 72, 2: 36, 3: 36, 4: 36, 5: 36}
 STORAGE_NAMES = {0: "f32", 1: "f32s", 2: "f16s", 3: "bf16s", 4: "f16c", 5: "f16d"}
 
 
-def traffic_from_profile(storage):
-    """HBM bytes per launch measured with rocprofv3 PMC counters, if recorded."""
+def traffic_from_profile(storage, kernel="k_stream_collide"):
+    """HBM bytes per launch of a kernel measured with rocprofv3 PMC counters, if recorded."""
     path = os.path.join(REPO, "profiles", f"traffic_periodic16384_{STORAGE_NAMES[storage]}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         d = json.load(fh)
-    return d.get("bytes_per_launch")
+    if kernel == "k_stream_collide":
+        return d.get("bytes_per_launch")
+    return d.get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
 
 
-def gpu_run(lib, n, steps, warmup, storage, tuning=None):
+def gpu_run(lib, n, steps, warmup, storage, tuning=None, pairs=True):
     import mbl
 
     blk = mbl.PeriodicBlock(lib, n, n, tau=0.6, storage=storage, tuning=tuning)
+    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, 1 if pairs else 0), "step pairs")
     m0 = blk.moments()  # mass, momentum and non-finite count of the initial field (device sums)
     # bring the GPU clocks up first: untimed steps for at least half a second, then the W warmup steps
     t0 = time.perf_counter()
@@ -60,8 +68,9 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
     blk.step(warmup)
     blk.sync()
     t0 = time.perf_counter()
-    ev_ms = blk.timed_steps(steps)  # events bracket the K launches; ends with a synchronize
+    ev_ms = blk.timed_steps(steps)  # events bracket the launches of K steps; ends with a synchronize
     wall = time.perf_counter() - t0
+    launches = blk.launches(steps)
     m1 = blk.moments()
     blk.close()
     nodes = n * n
@@ -70,8 +79,9 @@ def gpu_run(lib, n, steps, warmup, storage, tuning=None):
     inv = {"mass_rel_change": float((m1[0] - m0[0]) / m0[0]),
            "momentum_change_per_node": [float((m1[1] - m0[1]) / nodes), float((m1[2] - m0[2]) / nodes)],
            "nonfinite_entries": int(m1[3])}
-    return {"wall_s": wall, "event_ms": ev_ms, "per_launch_ms": ev_ms / steps,
-            "mlups": nodes * steps / wall / 1e6, "nodes": nodes, "invariants": inv}
+    return {"wall_s": wall, "event_ms": ev_ms, "launches": launches, "per_launch_ms": ev_ms / launches,
+            "steps_per_launch": steps / launches, "mlups": nodes * steps / wall / 1e6, "nodes": nodes,
+            "invariants": inv}
 
 
 def stream_copy_gbs(lib, mib=4096, reps=10):
@@ -190,8 +200,11 @@ def main():
           flush=True)
     tuning = (args.vx, args.tpb) if (args.vx or args.tpb) else None
     r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage, tuning)
-    bpn = BYTES_PER_NODE[args.storage]
+    bpn = BYTES_PER_NODE[args.storage]  # algorithmic bytes per node of one launch (either kernel)
     achieved = bpn * r["nodes"] / (r["per_launch_ms"] * 1e-3) / 1e9
+    paired = r["steps_per_launch"] > 1.5
+    single = gpu_run(lib, args.n, max(100, args.steps // 5), args.warmup, args.storage, tuning, pairs=False)
+    single_achieved = bpn * single["nodes"] / (single["per_launch_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(r["mlups"], 1),
@@ -206,13 +219,21 @@ def main():
         "dtype": "f64" if args.storage in (0, 1) else "f32",
         "data": "synthetic (seeded random velocity field)",
         "config": {"workload": f"periodic {args.n}x{args.n} D2Q9 BGK block, tau=0.6, "
-                               f"{STORAGE_NAMES[args.storage]} planes, {bpn} B/node/step",
+                               f"{STORAGE_NAMES[args.storage]} planes, "
+                               + (f"two steps per launch, {bpn} B/node/launch" if paired else f"{bpn} B/node/step"),
                    "nodes": r["nodes"], "storage": STORAGE_NAMES[args.storage],
-                   "arithmetic": "f64" if args.storage in (0, 1) else "f32"},
+                   "arithmetic": "f64" if args.storage in (0, 1) else "f32",
+                   "steps_per_launch": round(r["steps_per_launch"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.storage),
-                     "kernel": "k_stream_collide (periodic)", "per_launch_ms": round(r["per_launch_ms"], 5),
-                     "bytes_per_node": bpn},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(args.storage, "k_periodic2" if paired else "k_stream_collide"),
+                     "kernel": "k_periodic2 (two steps per launch)" if paired else "k_stream_collide (periodic)",
+                     "per_launch_ms": round(r["per_launch_ms"], 5), "bytes_per_node": bpn,
+                     "node_updates_per_launch": round(r["steps_per_launch"] * r["nodes"])},
+        "single_step": {"kernel": "k_stream_collide (periodic)", "mlups": round(single["mlups"], 1),
+                        "per_launch_ms": round(single["per_launch_ms"], 5),
+                        "achieved": round(single_achieved, 1), "frac": round(single_achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic_from_profile(args.storage, "k_stream_collide")},
         "invariants": r["invariants"],
     }
     copy, copy9 = stream_copy_gbs(lib)

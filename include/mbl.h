@@ -445,6 +445,9 @@ int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
 /* on == 0: mbl_step_block advances one step per launch on every block (default on: two steps
  * per launch where it can). No reference counterpart. */
 int mbl_set_step_pairs(mbl_handle_t h, int on);
+/* Launches mbl_step_block makes for nsteps steps of the block it stepped last (for per-launch
+ * timing). No reference counterpart. */
+int mbl_step_block_launches(mbl_handle_t h, const AREA *area, int nsteps);
 
 /* Invariants of one block's state (the F planes): out[0] = sum of f over every node and
  * direction (the mass), out[1], out[2] = sum of f e_x, f e_y (the momentum), out[3] = number

@@ -2428,6 +2428,17 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     return 0;
 }
 
+extern "C" int mbl_step_block_launches(mbl_handle_t h, const AREA *area, int nsteps)
+{
+    if (!h) h = g_current;
+    if (!h) return MBL_E_NO_HANDLE;
+    if (!area || nsteps < 0) return MBL_E_INVALID;
+    const Single &S = h->single;
+    const bool pairs = S.dd && h->step_pairs && S.periodic && !S.flags &&
+                       periodic2_fits(storage_mode(h->cfg), area->nx, area->ny);
+    return pairs ? nsteps / 2 + nsteps % 2 : nsteps;
+}
+
 extern "C" int mbl_set_step_pairs(mbl_handle_t h, int on)
 {
     if (!h) h = g_current;

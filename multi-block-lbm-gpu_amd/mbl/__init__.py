@@ -97,6 +97,7 @@ def load_library(fp64=False):
         "mbl_error_L2_noalloc": (ip, [vp, ip, C.POINTER(AREA), C.POINTER(D2Q9), C.POINTER(vp), C.POINTER(vp), vp]),
         "mbl_step_block": (ip, [vp, C.POINTER(AREA), C.POINTER(D2Q9), ip]),
         "mbl_set_step_pairs": (ip, [vp, ip]),
+        "mbl_step_block_launches": (ip, [vp, C.POINTER(AREA), ip]),
         "mbl_timer_begin": (ip, [vp]),
         "mbl_timer_end": (ip, [vp, dp]),
         "mbl_set_tuning": (ip, [vp, ip, ip]),
@@ -474,6 +475,10 @@ class PeriodicBlock:
 
     def step(self, n):
         check(self.lib, self.lib.mbl_step_block(self.h, C.byref(self.area), C.byref(self.pp), n), "step")
+
+    def launches(self, n):
+        """Kernel launches of step(n) on this block (two steps per launch where it can)."""
+        return self.lib.mbl_step_block_launches(self.h, C.byref(self.area), n)
 
     def timed_steps(self, n):
         """Runs n steps between HIP events on the handle's stream; returns milliseconds."""
