@@ -442,8 +442,10 @@ int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 *pps_d, dou
  * after single steps, bit for bit, and pp->f the state two steps back (one step
  * back after single steps); an odd remainder takes one single step. */
 int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
-/* on == 0: mbl_step_block advances one step per launch on every block (default on: two steps
- * per launch where it can). No reference counterpart. */
+/* on == 0: mbl_step_block advances one step per launch on every block; on == 1 (default): two
+ * steps per launch where the shape allows, in tiles with a one-node ring; on == 2: two steps per
+ * launch in strips of 254 columns with a 3-row window where the shape allows (width >= 256,
+ * height a multiple of 64), else tiles. Same bits in every case. No reference counterpart. */
 int mbl_set_step_pairs(mbl_handle_t h, int on);
 /* Launches mbl_step_block makes for nsteps steps of the block it stepped last (for per-launch
  * timing). No reference counterpart. */

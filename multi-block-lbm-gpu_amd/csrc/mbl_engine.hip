@@ -224,7 +224,7 @@ struct mbl_context {
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     bool fork = false; // launches at one position run concurrently (mbl_set_cycle_fork)
-    bool step_pairs = true; // mbl_step_block: periodic all-fluid blocks two steps per launch
+    int step_pairs = 1; // mbl_step_block: periodic all-fluid blocks two steps per launch (its form)
 };
 
 static mbl_context *g_current = nullptr;
@@ -2401,7 +2401,7 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     // periodic all-fluid blocks: two steps per launch where the shape allows (the state at
     // step n + 2 lands in the other plane set, which then holds it as one step would have
     // left it; the plane set handed back as f holds step n, not n + 1)
-    const bool pairs = h->step_pairs && S.periodic && !S.flags && periodic2_fits(mode, area->nx, area->ny);
+    const bool pairs = h->step_pairs && S.periodic && !S.flags && periodic2_fits(mode, area->nx, area->ny, h->step_pairs);
     for (int t = 0; t < nsteps;) {
         int rc;
         if (pairs && t + 2 <= nsteps) {
@@ -2413,6 +2413,7 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
             }
             a.nx = area->nx;
             a.ny = area->ny;
+            a.form = h->step_pairs;
             a.omega = S.hd.omega;
             rc = launch_periodic2(h->stream, mode, a);
             t += 2;
@@ -2435,7 +2436,7 @@ extern "C" int mbl_step_block_launches(mbl_handle_t h, const AREA *area, int nst
     if (!area || nsteps < 0) return MBL_E_INVALID;
     const Single &S = h->single;
     const bool pairs = S.dd && h->step_pairs && S.periodic && !S.flags &&
-                       periodic2_fits(storage_mode(h->cfg), area->nx, area->ny);
+                       periodic2_fits(storage_mode(h->cfg), area->nx, area->ny, h->step_pairs);
     return pairs ? nsteps / 2 + nsteps % 2 : nsteps;
 }
 
@@ -2443,7 +2444,8 @@ extern "C" int mbl_set_step_pairs(mbl_handle_t h, int on)
 {
     if (!h) h = g_current;
     if (!h) return MBL_E_NO_HANDLE;
-    h->step_pairs = on != 0;
+    if (on < 0 || on > 2) return MBL_E_INVALID;
+    h->step_pairs = on;
     return 0;
 }
 

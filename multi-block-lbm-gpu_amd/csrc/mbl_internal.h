@@ -234,9 +234,10 @@ struct Periodic2Args {
     const void *S[MBL_Q];
     void *D[MBL_Q];
     int nx, ny, tiles_x;
+    int form; // 1: tiles with a one-node ring; 2: strips with a 3-row window (where the shape allows)
     double omega;
 };
-int periodic2_fits(int mode, int nx, int ny);
+int periodic2_fits(int mode, int nx, int ny, int form);
 int launch_periodic2(void *stream, int mode, const Periodic2Args &a);
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,

@@ -977,6 +977,95 @@ __global__ __launch_bounds__(P2_THREADS) void k_periodic2(Periodic2Args a)
     }
 }
 
+// Two time steps per launch, strip form: a workgroup of P2S_W waves walks down a strip of
+// 64 P2S_W - 2 output columns and P2S_H rows. Lane c (64 P2S_W of them) collides step n+1 at
+// column x0 - 1 + c of one row at a time (pulling step n from the planes) into a 3-row window
+// in LDS, and step n+2 of the row above from that window; the step-n loads of the next row are
+// issued before the window is read, so they are in flight meanwhile. The ring of recomputed
+// step-n+1 nodes is two columns per strip and two rows per strip height (against a one-node
+// ring around every 64 x 16 tile), and the window is 3 rows of LDS, so more workgroups share a CU.
+// Same node updates as the single-step kernel, so the same bits. Measured on 16384^2 (same box,
+// two rounds): FP32 planes 107k MLUPS against 123k-133k for the tiles (two barriers per row
+// serialize the FP64 collisions), 1-3-12 planes 174k-175k against 170k; the tiles are the default.
+constexpr int P2S_W = 4;   // waves per workgroup
+constexpr int P2S_H = 64;  // output rows per strip
+
+template <int MODE>
+__global__ __launch_bounds__(64 * P2S_W) void k_periodic2s(Periodic2Args a)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    typedef typename C::R R;
+    typedef typename C::GT GT;
+    constexpr int NC = 64 * P2S_W, OW = NC - 2;
+    __shared__ T Wn[3][9][NC];
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t unx = (uint32_t)nx;
+    const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int ty = b / a.tiles_x, tx = b - ty * a.tiles_x;
+    const int x0 = tx * OW, y0 = ty * P2S_H;
+    const int c = (int)threadIdx.x;
+    int x = x0 - 1 + c;
+    const bool out = c >= 1 && c <= OW && x < nx; // this lane's column is an output of the strip
+    x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+    int xs[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        int v = x - EX[k];
+        xs[k] = v < 0 ? v + nx : (v >= nx ? v - nx : v);
+    }
+    const R omega = (R)a.omega;
+    // step-n values this lane pulls for step n+1 at row y (periodic rows)
+    auto pull = [&](int y, T (&v)[9]) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            int ys = y - EY[k];
+            ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
+            v[k] = *off32((const GT *)a.S[k], (uint32_t)ys * unx + (uint32_t)xs[k]);
+        }
+    };
+    auto step1 = [&](const T (&v)[9], int slot) {
+        R q[9], Q[9], rho, ux, uy;
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[k] = C::ld(v[k]);
+        collide_c<C>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) Wn[slot][k][c] = C::st(Q[k]);
+    };
+    // step n+1 at rows y0 - 1 and y0 (window slots 0, 1), the loads of row y0 + 1 in flight
+    T v[9], nv[9];
+    pull(y0 - 1 < 0 ? y0 - 1 + ny : y0 - 1, v);
+    pull(y0, nv);
+    step1(v, 0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = nv[k];
+    pull(y0 + 1 >= ny ? y0 + 1 - ny : y0 + 1, nv);
+    step1(v, 1);
+    for (int i = 0; i < P2S_H; i++) {
+        // step n+1 at row y0 + i + 1 into slot (i + 2) % 3; the loads of the row after it issued
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = nv[k];
+        if (i + 1 < P2S_H) {
+            int yn = y0 + i + 2;
+            yn = yn >= ny ? yn - ny : yn;
+            pull(yn, nv);
+        }
+        step1(v, (i + 2) % 3);
+        __syncthreads();
+        // step n+2 at row y0 + i from the window rows y0 + i - 1, y0 + i, y0 + i + 1
+        if (out) {
+            R q[9], Q[9], rho, ux, uy;
+#pragma unroll
+            for (int k = 0; k < 9; k++) q[k] = C::ld(Wn[(i + 1 - EY[k]) % 3][k][c - EX[k]]);
+            collide_c<C>(q, omega, Q, rho, ux, uy);
+            const uint32_t o = (uint32_t)(y0 + i) * unx + (uint32_t)x;
+#pragma unroll
+            for (int k = 0; k < 9; k++) __builtin_nontemporal_store(C::st(Q[k]), off32((GT *)a.D[k], o));
+        }
+        __syncthreads(); // the window's oldest row is overwritten next
+    }
+}
+
 // ---------------------------------------------------------------------------
 // init_General (lbm/init_block.h:2-51)
 // ---------------------------------------------------------------------------
@@ -2581,15 +2670,31 @@ static int periodic2_threads(int mode)
     return store_elem_bytes(mode) == 4 ? 512 : 256;
 }
 
-int periodic2_fits(int mode, int nx, int ny)
+static bool periodic2_strip_fits(int nx, int ny) { return nx >= 64 * P2S_W && ny >= P2S_H && ny % P2S_H == 0; }
+
+static bool periodic2_tile_fits(int mode, int nx, int ny)
 {
     const int ty = periodic2_rows(mode);
     return nx >= P2_TX && ny >= ty && nx % P2_TX == 0 && ny % ty == 0;
 }
 
+int periodic2_fits(int mode, int nx, int ny, int form)
+{
+    return (form == 2 && periodic2_strip_fits(nx, ny)) || periodic2_tile_fits(mode, nx, ny);
+}
+
 int launch_periodic2(void *stream, int mode, const Periodic2Args &args)
 {
-    if (!periodic2_fits(mode, args.nx, args.ny)) return MBL_E_INVALID;
+    if (!periodic2_fits(mode, args.nx, args.ny, args.form)) return MBL_E_INVALID;
+    if (args.form == 2 && periodic2_strip_fits(args.nx, args.ny)) {
+        Periodic2Args a = args;
+        a.tiles_x = (a.nx + 64 * P2S_W - 3) / (64 * P2S_W - 2);
+        dim3 g(a.tiles_x * (a.ny / P2S_H)), b(64 * P2S_W);
+        hipStream_t s = (hipStream_t)stream;
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_periodic2s<M_>), g, b, 0, s, a));
+        MBL_LAUNCH_CHECK();
+        return 0;
+    }
     Periodic2Args a = args;
     const int ty = periodic2_rows(mode), nt = periodic2_threads(mode);
     a.tiles_x = a.nx / P2_TX;

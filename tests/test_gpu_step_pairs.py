@@ -12,8 +12,9 @@ pytestmark = pytest.mark.gpu
 
 
 def states(lib, n, ny, storage, steps, pairs, sigma=0.02):
+    """pairs: 0 single steps, 1 paired tiles, 2 paired strips (where the shape allows)."""
     blk = mbl.PeriodicBlock(lib, n, ny, tau=0.6, storage=storage, sigma=sigma)
-    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, 1 if pairs else 0), "pairs")
+    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, int(pairs)), "pairs")
     out = []
     done = 0
     for s in steps:
@@ -29,6 +30,19 @@ def test_pairs_equal_single_steps(lib32, storage):
     steps = [2, 7, 12]  # 1 pair; then 2 pairs + 1 single; then 2 pairs + 1 single
     a = states(lib32, 128, 64, storage, steps, True)
     b = states(lib32, 128, 64, storage, steps, False)
+    for sa, sb in zip(a, b):
+        for k in range(9):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+@pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F16D])
+@pytest.mark.parametrize("shape", [(300, 128), (512, 64)])
+def test_strips_equal_single_steps(lib32, storage, shape):
+    """Paired strips (254 output columns and 64 rows per workgroup, a 3-row window of step n+1):
+    a width that leaves a partial last strip (300) and one that does not (512)."""
+    steps = [2, 5]
+    a = states(lib32, shape[0], shape[1], storage, steps, 2)
+    b = states(lib32, shape[0], shape[1], storage, steps, 0)
     for sa, sb in zip(a, b):
         for k in range(9):
             assert np.array_equal(sa[k], sb[k]), k
