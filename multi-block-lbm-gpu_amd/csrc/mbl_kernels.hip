@@ -890,8 +890,17 @@ __global__ __launch_bounds__(256) void k_stream_collide(const BlockDesc *__restr
 // ---------------------------------------------------------------------------
 constexpr int P2_TX = 64;
 
+// Waves per SIMD the kernel asks for: three workgroups per CU where the LDS holds three (the
+// registers are capped to fit: 512 threads, FP32 planes: 64 VGPRs, no spill; 16384^2 132k-137k
+// MLUPS against 121k-123k at the 110 VGPRs the compiler takes unbounded, two workgroups per CU).
+constexpr int p2_waves(int threads, int ty, int elem_bytes)
+{
+    return 3 * 9 * (ty + 2) * (P2_TX + 2) * elem_bytes <= 160 * 1024 ? 3 * threads / 256 : 1;
+}
+
 template <int MODE, int TY, int P2_THREADS>
-__global__ __launch_bounds__(P2_THREADS) void k_periodic2(Periodic2Args a)
+__global__ __launch_bounds__(P2_THREADS, p2_waves(P2_THREADS, TY, (int)sizeof(typename Codec<MODE>::T)))
+void k_periodic2(Periodic2Args a)
 {
     typedef Codec<MODE> C;
     typedef typename C::T T;
