@@ -37,6 +37,8 @@ struct WgradArgs {
 };
 
 void conv(hipStream_t s, const ConvArgs &a);
+// split count of a layer's weight gradient (partial sets reduced by reduce_partials)
+int wgrad_splits(int N, int H, int W, int Cin, int Cout, int ksize);
 void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit);
 void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int accumulate);
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb);

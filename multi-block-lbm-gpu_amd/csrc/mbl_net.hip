@@ -187,8 +187,8 @@ extern "C" int mbl_net_create(mbl_handle_t h, int H, int W, int in_ch, int out_c
     // split-K partial buffer: the largest layer x its split count
     for (size_t i = 0; i < n->L.size(); i++) {
         const Layer &l = n->L[i];
-        int level = i < 8 ? (int)(i / 2) : (i < 14 ? 2 - (int)((i - 8) / 2) : 0);
-        int nsplit = std::max(1, P_of(n, max_batch, level) / 4096);
+        const int level = i < 8 ? (int)(i / 2) : (i < 14 ? 2 - (int)((i - 8) / 2) : 0);
+        const int nsplit = wgrad_splits(max_batch, H >> level, W >> level, l.cin, l.cout, l.ksize);
         n->partial_n = std::max(n->partial_n, (size_t)nsplit * (l.nw() + l.cout));
     }
     n->partial = (float *)dev(n, sizeof(float) * n->partial_n);
@@ -293,8 +293,7 @@ static void conv_bwd(mbl_net *n, int li, int N, const bf16 *dZ, int ldd, const b
 {
     const Layer &l = n->L[li];
     const int lev = layer_level(li);
-    const int P = P_of(n, N, lev);
-    const int nsplit = std::max(1, P / 4096);
+    const int nsplit = wgrad_splits(N, n->H >> lev, n->W >> lev, l.cin, l.cout, l.ksize);
     WgradArgs wa;
     wa.dY = dZ;
     wa.ldd = ldd;

@@ -10,6 +10,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <type_traits>
+#include <utility>
+
 #include "mbl_net.h"
 
 namespace mbl {
@@ -32,6 +36,17 @@ __device__ inline bf16x8 load8(const bf16 *p)
     return *reinterpret_cast<bf16x8 *>(&v);
 }
 
+// f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, N - 1>{}): loop bodies whose
+// index is a compile-time constant from the start
+template <class F, int... I> __device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 __device__ inline bf16x8 zero8()
 {
     uint4 v = {0u, 0u, 0u, 0u};
@@ -47,6 +62,7 @@ __device__ inline bf16x8 zero8()
 // feeds COT/16 MFMAs. Per k-step (one tap, 32 input channels) the COT x 32
 // weight tile is loaded once per workgroup into LDS (one 16-byte load per
 // thread) and read from there by the four waves. Cin % 32 == 0, Cout % 32 == 0.
+// Launched for the 1x1 head; 3x3 layers run k_conv3_tile.
 // Epilogue: optional bias, optional multiply by (mask > 0) (ReLU derivative of
 // the layer input in the backward-data pass), optional ReLU, BF16 store into a
 // channel slice of Y (leading dimension ldy, offset yoff) and/or FP32 store.
@@ -129,6 +145,115 @@ template <int COT> __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
         }
 }
 
+// ---------------------------------------------------------------------------
+// 3x3 convolution on 2D pixel tiles: a workgroup computes an 8 x 16 pixel tile
+// of one image for COT output channels. Per block of 32 input channels it
+// stages the tile's 10 x 18 halo of X and the COT x 9 x 32 weights in LDS, and
+// each wave runs 9 taps x 2 pixel rows x COT/16 MFMAs from there, so an
+// activation is read from memory once per channel block instead of once per
+// tap, and two barriers cover 9 taps. The next block's global loads are in
+// flight during this block's MFMAs (held in registers). Same epilogue as
+// k_conv. Sums run channel block by channel block, the taps inside.
+// ---------------------------------------------------------------------------
+constexpr int CT_H = 8, CT_W = 16;                 // output tile (rows x columns)
+constexpr int CT_HH = CT_H + 2, CT_HW = CT_W + 2;  // halo tile
+constexpr int CT_LD = 40;                          // LDS row: 32 channels + 8 (rows 16-byte aligned)
+
+__device__ inline int conv_tiles_x(int W) { return (W + CT_W - 1) / CT_W; }
+__device__ inline int conv_tiles_y(int H) { return (H + CT_H - 1) / CT_H; }
+
+template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvArgs a)
+{
+    constexpr int NC = COT / 16;
+    constexpr int NXV = CT_HH * CT_HW * 4;  // 16-byte vectors of one halo block (720)
+    constexpr int NXL = (NXV + 255) / 256;  // per thread
+    constexpr int NWV = COT * 9 * 4;        // 16-byte vectors of one weight block
+    constexpr int NWL = (NWV + 255) / 256;
+    __shared__ __attribute__((aligned(16))) bf16 sX[CT_HH * CT_HW * CT_LD];
+    __shared__ __attribute__((aligned(16))) bf16 sW[COT * 9 * CT_LD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int tx_n = conv_tiles_x(a.W), ty_n = conv_tiles_y(a.H);
+    int b = blockIdx.x;
+    const int n = b / (tx_n * ty_n);
+    b -= n * tx_n * ty_n;
+    const int y0 = (b / tx_n) * CT_H, x0 = (b % tx_n) * CT_W;
+    const int c0 = blockIdx.y * COT;
+    uint4 xv[NXL], wv[NWL];
+    // register arrays indexed by compile-time constants only (static_for), so they stay in registers
+    auto fetch = [&](int ci) __attribute__((always_inline)) {
+        static_for<NXL>([&](auto m) __attribute__((always_inline)) {
+            const int i = t + 256 * m;
+            xv[m] = uint4{0u, 0u, 0u, 0u};
+            const int hp = i >> 2, part = i & 3;
+            const int yy = y0 - 1 + hp / CT_HW, xx = x0 - 1 + hp % CT_HW;
+            if (i < NXV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                xv[m] = *reinterpret_cast<const uint4 *>(a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff + ci +
+                                                         8 * part);
+        });
+        static_for<NWL>([&](auto m) __attribute__((always_inline)) {
+            const int i = t + 256 * m;
+            wv[m] = uint4{0u, 0u, 0u, 0u};
+            if (i < NWV) // row = local output channel * 9 + tap
+                wv[m] = *reinterpret_cast<const uint4 *>(a.Wt + (size_t)(c0 * 9 + (i >> 2)) * a.Cin + ci + 8 * (i & 3));
+        });
+    };
+    f32x4 acc[2][NC];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[h][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    fetch(0);
+    for (int ci = 0; ci < a.Cin; ci += 32) {
+        __syncthreads(); // the previous block's reads of sX and sW are done
+        static_for<NXL>([&](auto m) __attribute__((always_inline)) {
+            const int i = t + 256 * m;
+            if (i < NXV) *reinterpret_cast<uint4 *>(sX + (i >> 2) * CT_LD + 8 * (i & 3)) = xv[m];
+        });
+        static_for<NWL>([&](auto m) __attribute__((always_inline)) {
+            const int i = t + 256 * m;
+            if (i < NWV) *reinterpret_cast<uint4 *>(sW + (i >> 2) * CT_LD + 8 * (i & 3)) = wv[m];
+        });
+        __syncthreads();
+        if (ci + 32 < a.Cin) fetch(ci + 32);
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int ty = tap / 3, tx = tap % 3; // halo offsets of the tap
+            bf16x8 av[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) av[h] = load8(sX + ((2 * w + h + ty) * CT_HW + r16 + tx) * CT_LD + 8 * g);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const bf16x8 bv = load8(sW + ((16 * c + r16) * 9 + tap) * CT_LD + 8 * g);
+#pragma unroll
+                for (int h = 0; h < 2; h++) acc[h][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[h], bv, acc[h][c], 0, 0, 0);
+            }
+        }
+    }
+    // acc[h][c][r] = out[row y0 + 2w + h, column x0 + 4g + r][channel c0 + 16c + r16]
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int yy = y0 + 2 * w + h;
+        if (yy >= a.H) continue;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int co = c0 + 16 * c + r16;
+            const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int xx = x0 + 4 * g + r;
+                if (xx >= a.W) continue;
+                const size_t pix = (size_t)(n * a.H + yy) * a.W + xx;
+                float v = acc[h][c][r] + bias;
+                if (a.M && !(bf2f(a.M[pix * a.ldm + a.moff + co]) > 0.f)) v = 0.f;
+                if (a.relu) v = fmaxf(v, 0.f);
+                if (a.Yf) a.Yf[pix * a.ldyf + co] = v;
+                if (a.Y) a.Y[pix * a.ldy + a.yoff + co] = f2bf(v);
+            }
+        }
+    }
+}
+
 // floor(v / d) for 0 <= v < 2^24 through a float reciprocal, corrected to the exact quotient
 __device__ inline int fdiv(int v, int d, float inv)
 {
@@ -182,7 +307,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const bool bias = (blockIdx.y == 0); // tap 0, input channels 0..31
     float bs[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    auto fetch = [&](int s0, uint4 &va, uint4 &vb) {
+    auto fetch = [&](int s0, uint4 &va, uint4 &vb) __attribute__((always_inline)) {
         const int p = s0 + lp;
         va = uint4{0u, 0u, 0u, 0u};
         vb = uint4{0u, 0u, 0u, 0u};
@@ -241,14 +366,165 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a)
     }
 }
 
-// out[i] (+)= sum_s partial[s][i], in split order (deterministic)
-__global__ void k_reduce_partials(const float *partial, int nsplit, size_t n, float *out, int accumulate)
+// ---------------------------------------------------------------------------
+// 3x3 weight gradient on 2D pixel tiles: a workgroup computes the 32 co x 32 ci
+// block of all 9 taps, dW[co][tap][ci] = sum_p dY[p][co] * X[p + tap][ci], over
+// a contiguous range of 8 x 16 pixel tiles (blockIdx.z of the split), written
+// as FP32 partials for the ordered reduction. Per tile it stages dY^T (32 co x
+// 128 pixels) and the tile's 10 x 18 halo of X^T (32 ci x 10 rows, each row 32
+// columns wide with column c at x = x0 - 8 + c) in LDS. The MFMA K dimension is
+// pixels: a lane's 8 K values are 8 pixels of a tile row, one 16-byte LDS read;
+// the taps shifted by one column are assembled from the aligned reads of the
+// same row with alignbyte (no misaligned LDS access, no shifted copies). Wave w
+// owns co rows 16(w & 1).. and ci columns 16(w >> 1).. of all 9 taps; the dY
+// operand of a k-step serves the 9 taps. The next tile's loads are in flight
+// during this tile's MFMAs. Workgroups of input-channel block 0 also sum dY for
+// the bias gradient (written after the weights' partials).
+// ---------------------------------------------------------------------------
+constexpr int WT_LDA = CT_H * CT_W + 8;  // LDS row of dY^T: 128 pixels + 8
+constexpr int WT_ROW = 32;               // LDS row of X^T (one halo row)
+constexpr int WT_LDB = CT_HH * WT_ROW + 8; // per input channel: 10 halo rows + 8
+
+__global__ __launch_bounds__(256, 2) void k_wgrad3_tile(WgradArgs a)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    constexpr int NAV = CT_H * CT_W * 4 / 256;          // dY vectors per thread (2)
+    constexpr int NXV = CT_HH * CT_HW * 4;              // X halo vectors (720)
+    constexpr int NXL = (NXV + 255) / 256;              // per thread (3)
+    __shared__ __attribute__((aligned(16))) bf16 sA[32 * WT_LDA];
+    __shared__ __attribute__((aligned(16))) bf16 sB[32 * WT_LDB];
+    __shared__ float sBias[64][33];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32;
+    const int tx_n = conv_tiles_x(a.W), ty_n = conv_tiles_y(a.H);
+    const int T = a.N * tx_n * ty_n;
+    const int tb = (int)((long long)T * blockIdx.z / gridDim.z), te = (int)((long long)T * (blockIdx.z + 1) / gridDim.z);
+    const bool bias = (blockIdx.y == 0);
+    const int wco = 16 * (w & 1), wci = 16 * (w >> 1);
+    uint4 av[NAV], xv[NXL];
+    // inlined: the register arrays it fills must not go to the stack
+    auto fetch = [&](int tile) __attribute__((always_inline)) {
+        const int n = tile / (tx_n * ty_n), b = tile - n * tx_n * ty_n;
+        const int y0 = (b / tx_n) * CT_H, x0 = (b % tx_n) * CT_W;
+#pragma unroll
+        for (int m = 0; m < NAV; m++) {
+            const int i = t + 256 * m, p = i >> 2;
+            const int yy = y0 + p / CT_W, xx = x0 + p % CT_W;
+            av[m] = uint4{0u, 0u, 0u, 0u};
+            if (yy < a.H && xx < a.W)
+                av[m] = *reinterpret_cast<const uint4 *>(a.dY + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldd + a.doff + co0 +
+                                                         8 * (i & 3));
+        }
+#pragma unroll
+        for (int m = 0; m < NXL; m++) {
+            const int i = t + 256 * m;
+            xv[m] = uint4{0u, 0u, 0u, 0u};
+            if (i < NXV) {
+                const int hp = i >> 2;
+                const int yy = y0 - 1 + hp / CT_HW, xx = x0 - 1 + hp % CT_HW;
+                if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                    xv[m] = *reinterpret_cast<const uint4 *>(a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff +
+                                                             ci0 + 8 * (i & 3));
+            }
+        }
+    };
+    f32x4 acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bs[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (tb < te) fetch(tb);
+    for (int tile = tb; tile < te; tile++) {
+        __syncthreads(); // the previous tile's reads of sA and sB are done
+#pragma unroll
+        for (int m = 0; m < NAV; m++) {
+            const int i = t + 256 * m, p = i >> 2, c8 = 8 * (i & 3);
+            const bf16 *e = reinterpret_cast<const bf16 *>(&av[m]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                sA[(c8 + k) * WT_LDA + p] = e[k];
+                if (bias) bs[k] += bf2f(e[k]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NXL; m++) {
+            const int i = t + 256 * m;
+            if (i < NXV) {
+                const int hp = i >> 2, c8 = 8 * (i & 3);
+                const int col = (hp / CT_HW) * WT_ROW + hp % CT_HW + 7; // halo column hx sits at x0 - 1 + hx = x0 - 8 + col
+                const bf16 *e = reinterpret_cast<const bf16 *>(&xv[m]);
+#pragma unroll
+                for (int k = 0; k < 8; k++) sB[(c8 + k) * WT_LDB + col] = e[k];
+            }
+        }
+        __syncthreads();
+        if (tile + 1 < te) fetch(tile + 1);
+#pragma unroll
+        for (int ks = 0; ks < CT_H / 2; ks++) {
+            // K step: tile rows 2ks, 2ks + 1; lane group g holds row 2ks + (g >> 1), columns 8(g & 1) .. + 7
+            const bf16x8 A = load8(sA + (wco + r16) * WT_LDA + 32 * ks + 8 * g);
+            const int row = 2 * ks + (g >> 1), cb = 8 * (g & 1);
+#pragma unroll
+            for (int ty = 0; ty < 3; ty++) {
+                const bf16 *rp = sB + (wci + r16) * WT_LDB + (row + ty) * WT_ROW + cb;
+                const uint4 mid = *reinterpret_cast<const uint4 *>(rp + 8);   // columns cb + 8 .. cb + 15: tap column 0
+                const uint32_t lo = *reinterpret_cast<const uint32_t *>(rp + 6);  // columns cb + 6, cb + 7
+                const uint32_t hi = *reinterpret_cast<const uint32_t *>(rp + 16); // columns cb + 16, cb + 17
+                // tap column -1: columns cb + 7 .. cb + 14; +1: cb + 9 .. cb + 16
+                const uint4 left = {__builtin_amdgcn_alignbyte(mid.x, lo, 2), __builtin_amdgcn_alignbyte(mid.y, mid.x, 2),
+                                    __builtin_amdgcn_alignbyte(mid.z, mid.y, 2), __builtin_amdgcn_alignbyte(mid.w, mid.z, 2)};
+                const uint4 right = {__builtin_amdgcn_alignbyte(mid.y, mid.x, 2), __builtin_amdgcn_alignbyte(mid.z, mid.y, 2),
+                                     __builtin_amdgcn_alignbyte(mid.w, mid.z, 2), __builtin_amdgcn_alignbyte(hi, mid.w, 2)};
+                acc[3 * ty + 0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, *reinterpret_cast<const bf16x8 *>(&left),
+                                                                          acc[3 * ty + 0], 0, 0, 0);
+                acc[3 * ty + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, *reinterpret_cast<const bf16x8 *>(&mid),
+                                                                          acc[3 * ty + 1], 0, 0, 0);
+                acc[3 * ty + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, *reinterpret_cast<const bf16x8 *>(&right),
+                                                                          acc[3 * ty + 2], 0, 0, 0);
+            }
+        }
+    }
+    // acc[tap][r] = dW[co = co0 + wco + 4g + r][tap][ci = ci0 + wci + r16]
+    const size_t nw = (size_t)a.Cout * 9 * a.Cin;
+    float *out = a.partial + (size_t)blockIdx.z * (nw + a.Cout);
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int co = co0 + wco + 4 * g + r;
+            out[((size_t)co * 9 + tap) * a.Cin + ci0 + wci + r16] = acc[tap][r];
+        }
+    if (bias) { // lane q of the loaders held pixels q and 64 + q of every tile; their sums added in lane order
+        const int q = t >> 2, c8 = 8 * (t & 3);
+#pragma unroll
+        for (int k = 0; k < 8; k++) sBias[q][c8 + k] = bs[k];
+        __syncthreads();
+        if (t < 32) {
+            float sum = 0.f;
+            for (int j = 0; j < 64; j++) sum += sBias[j][t];
+            out[nw + co0 + t] = sum;
+        }
+    }
+}
+
+// out[i] (+)= sum_s partial[s][i] in a fixed order (deterministic): a workgroup takes 32 outputs;
+// thread group q = 0..7 sums splits q, q + 8, q + 16, ... of each, then the 8 group sums are added
+// in group order
+__global__ __launch_bounds__(256) void k_reduce_partials(const float *partial, int nsplit, size_t n, float *out,
+                                                         int accumulate)
+{
+    __shared__ float sh[8][32];
+    const int j = threadIdx.x & 31, q = threadIdx.x >> 5;
+    const size_t i = (size_t)blockIdx.x * 32 + j;
     float s = 0.f;
-    for (int k = 0; k < nsplit; k++) s += partial[(size_t)k * n + i];
-    out[i] = accumulate ? out[i] + s : s;
+    if (i < n)
+        for (int k = q; k < nsplit; k += 8) s += partial[(size_t)k * n + i];
+    sh[q][j] = s;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        float v = sh[0][j];
+        for (int r = 1; r < 8; r++) v += sh[r][j];
+        out[i] = accumulate ? out[i] + v : v;
+    }
 }
 
 // Backward-data weights: Wb[ci][tap][co] = W[co][taps-1-tap][ci] (BF16)
@@ -415,19 +691,39 @@ static inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 void conv(hipStream_t s, const ConvArgs &a)
 {
     const int P = a.N * a.H * a.W;
+    if (a.ksize == 3) {
+        const unsigned tiles = (unsigned)(a.N * ((a.H + CT_H - 1) / CT_H) * ((a.W + CT_W - 1) / CT_W));
+        if (a.Cout % 64 == 0)
+            hipLaunchKernelGGL((k_conv3_tile<64>), dim3(tiles, a.Cout / 64), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_conv3_tile<32>), dim3(tiles, a.Cout / 32), dim3(256), 0, s, a);
+        return;
+    }
     if (a.Cout % 64 == 0)
         hipLaunchKernelGGL((k_conv<64>), dim3((P + CONV_PX - 1) / CONV_PX, a.Cout / 64), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL((k_conv<32>), dim3((P + CONV_PX - 1) / CONV_PX, a.Cout / 32), dim3(256), 0, s, a);
 }
+int wgrad_splits(int N, int H, int W, int Cin, int Cout, int ksize)
+{
+    if (ksize != 3) return std::max(1, N * H * W / 4096);
+    // about 1024 workgroups, at least two tiles each, at most 512 partial sets
+    const int tiles = N * ((H + CT_H - 1) / CT_H) * ((W + CT_W - 1) / CT_W);
+    const int blocks = (Cout / 32) * (Cin / 32);
+    return std::max(1, std::min(std::min(tiles / 2, 1024 / blocks), 512));
+}
 void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit)
 {
+    if (a.ksize == 3) {
+        hipLaunchKernelGGL(k_wgrad3_tile, dim3(a.Cout / 32, a.Cin / 32, nsplit), dim3(256), 0, s, a);
+        return;
+    }
     const int taps = a.ksize * a.ksize;
     hipLaunchKernelGGL(k_conv_wgrad, dim3(a.Cout / 32, taps * (a.Cin / 32), nsplit), dim3(256), 0, s, a);
 }
 void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int acc)
 {
-    hipLaunchKernelGGL(k_reduce_partials, g1(n), dim3(256), 0, s, partial, nsplit, n, out, acc);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, partial, nsplit, n, out, acc);
 }
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb)
 {

@@ -38,16 +38,22 @@ def cosine(a, b):
     return float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
 
 
-def sample(n, seed):
+# 16 x 32: every level's 8 x 16 pixel tiles whole down to 8 x 16, then one partial tile;
+# 24 x 40: partial tiles at the right edge of every level and at the bottom of 12 x 20 and below
+RASTERS = [(16, 32), (24, 40)]
+
+
+def sample(n, seed, h=H, w=W):
     rng = np.random.default_rng(seed)
-    x = rng.normal(size=(n, H, W, IN)).astype(np.float32)
-    t = rng.normal(size=(n, H, W, OUT)).astype(np.float32)
+    x = rng.normal(size=(n, h, w, IN)).astype(np.float32)
+    t = rng.normal(size=(n, h, w, OUT)).astype(np.float32)
     return x, t
 
 
-def test_net_forward_matches_reference(lib32):
-    net = Net(lib32, H, W, IN, OUT, base=BASE, max_batch=4, seed=11)
-    x, _ = sample(3, 1)
+@pytest.mark.parametrize("raster", RASTERS)
+def test_net_forward_matches_reference(lib32, raster):
+    net = Net(lib32, raster[0], raster[1], IN, OUT, base=BASE, max_batch=4, seed=11)
+    x, _ = sample(3, 1, *raster)
     y = net.forward(x)
     layers = unpack(net.get_params(), IN, OUT, BASE)
     y_ref = ref.output(layers, x.astype(np.float64), bf16=True)
@@ -60,7 +66,8 @@ def test_net_forward_matches_reference(lib32):
     net.close()
 
 
-def test_net_gradients_match_reference(lib32):
+@pytest.mark.parametrize("raster", RASTERS)
+def test_net_gradients_match_reference(lib32, raster):
     """GPU gradients against the reference. The gradient of a BF16 network is
     discontinuous: 2x2 max-pool windows hold near-ties that one BF16 ulp
     flips, moving the gradient of that window to another pixel. The
@@ -71,8 +78,8 @@ def test_net_gradients_match_reference(lib32):
     quantised away, above it ReLU kinks and max-pool switches dominate. The
     reference's own gradients are checked against finite differences in
     float64 on the CPU.)"""
-    net = Net(lib32, H, W, IN, OUT, base=BASE, max_batch=4, seed=12)
-    x, t = sample(2, 2)
+    net = Net(lib32, raster[0], raster[1], IN, OUT, base=BASE, max_batch=4, seed=12)
+    x, t = sample(2, 2, *raster)
     loss, g = net.loss_grad(x, t)
     layers = unpack(net.get_params(), IN, OUT, BASE)
     gl = unpack(g, IN, OUT, BASE)
