@@ -159,8 +159,8 @@ constexpr int CT_H = 8, CT_W = 16;                 // output tile (rows x column
 constexpr int CT_HH = CT_H + 2, CT_HW = CT_W + 2;  // halo tile
 constexpr int CT_LD = 40;                          // LDS row: 32 channels + 8 (rows 16-byte aligned)
 
-__device__ inline int conv_tiles_x(int W) { return (W + CT_W - 1) / CT_W; }
-__device__ inline int conv_tiles_y(int H) { return (H + CT_H - 1) / CT_H; }
+__host__ __device__ inline int conv_tiles_x(int W) { return (W + CT_W - 1) / CT_W; }
+__host__ __device__ inline int conv_tiles_y(int H) { return (H + CT_H - 1) / CT_H; }
 
 template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvArgs a)
 {
@@ -230,27 +230,65 @@ template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvAr
             }
         }
     }
-    // acc[h][c][r] = out[row y0 + 2w + h, column x0 + 4g + r][channel c0 + 16c + r16]
+    // acc[h][c][r] = out[row y0 + 2w + h, column x0 + 4g + r][channel c0 + 16c + r16]: rounded to BF16
+    // with the bias into LDS, then written (mask and ReLU applied) as 16-byte vectors of 8 channels;
+    // the mask and ReLU commute with the rounding. The FP32 output, when asked for, is stored directly.
+    constexpr int OLD = COT + 8; // LDS row of the output tile: one pixel
+    static_assert(CT_H * CT_W * OLD <= COT * 9 * CT_LD, "output tile fits in the weight buffer");
+    bf16 *sO = sW;
+    __syncthreads(); // every wave is done with sW
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const int yy = y0 + 2 * w + h;
-        if (yy >= a.H) continue;
+    for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const int co = c0 + 16 * c + r16;
             const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int xx = x0 + 4 * g + r;
-                if (xx >= a.W) continue;
-                const size_t pix = (size_t)(n * a.H + yy) * a.W + xx;
-                float v = acc[h][c][r] + bias;
-                if (a.M && !(bf2f(a.M[pix * a.ldm + a.moff + co]) > 0.f)) v = 0.f;
-                if (a.relu) v = fmaxf(v, 0.f);
-                if (a.Yf) a.Yf[pix * a.ldyf + co] = v;
-                if (a.Y) a.Y[pix * a.ldy + a.yoff + co] = f2bf(v);
+                const int p = (2 * w + h) * CT_W + 4 * g + r;
+                const float v = acc[h][c][r] + bias;
+                sO[p * OLD + 16 * c + r16] = f2bf(v);
+                if (a.Yf) {
+                    const int yy = y0 + 2 * w + h, xx = x0 + 4 * g + r;
+                    if (yy < a.H && xx < a.W) {
+                        float u = v;
+                        const size_t pix = (size_t)(n * a.H + yy) * a.W + xx;
+                        if (a.M && !(bf2f(a.M[pix * a.ldm + a.moff + co]) > 0.f)) u = 0.f;
+                        if (a.relu) u = fmaxf(u, 0.f);
+                        a.Yf[pix * a.ldyf + co] = u;
+                    }
+                }
             }
         }
+    if (!a.Y) return;
+    __syncthreads();
+    constexpr int NV = CT_H * CT_W * COT / 8 / 256; // output vectors per thread
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+        const int i = t + 256 * m, p = i / (COT / 8), c8 = 8 * (i % (COT / 8));
+        const int yy = y0 + p / CT_W, xx = x0 + p % CT_W;
+        if (yy >= a.H || xx >= a.W) continue;
+        const size_t pix = (size_t)(n * a.H + yy) * a.W + xx;
+        uint4 v = *reinterpret_cast<const uint4 *>(sO + p * OLD + c8);
+        uint32_t u[4] = {v.x, v.y, v.z, v.w};
+        // BF16 bit tests: v > 0 <=> 1 <= bits <= 0x7F80 (+0 and NaN excluded); max(v, 0) keeps bits <= 0x7F80
+        uint32_t mw[4] = {0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u}; // 1.0: no mask
+        if (a.M) {
+            const uint4 mv = *reinterpret_cast<const uint4 *>(a.M + pix * a.ldm + a.moff + c0 + c8);
+            mw[0] = mv.x, mw[1] = mv.y, mw[2] = mv.z, mw[3] = mv.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t lo = u[k] & 0xFFFFu, hi = u[k] >> 16;
+            if (((mw[k] & 0xFFFFu) - 1u) >= 0x7F80u) lo = 0;
+            if (((mw[k] >> 16) - 1u) >= 0x7F80u) hi = 0;
+            if (a.relu) {
+                if (lo > 0x7F80u) lo = 0;
+                if (hi > 0x7F80u) hi = 0;
+            }
+            u[k] = lo | (hi << 16);
+        }
+        *reinterpret_cast<uint4 *>(a.Y + pix * a.ldy + a.yoff + c0 + c8) = uint4{u[0], u[1], u[2], u[3]};
     }
 }
 
@@ -515,10 +553,16 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const float *partial, i
     __shared__ float sh[8][32];
     const int j = threadIdx.x & 31, q = threadIdx.x >> 5;
     const size_t i = (size_t)blockIdx.x * 32 + j;
-    float s = 0.f;
-    if (i < n)
-        for (int k = q; k < nsplit; k += 8) s += partial[(size_t)k * n + i];
-    sh[q][j] = s;
+    // four running sums per thread (splits q + 8(4m + r) in sum r), so four loads are in flight
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    if (i < n) {
+        int k = q;
+        for (; k + 24 < nsplit; k += 32)
+#pragma unroll
+            for (int r = 0; r < 4; r++) s[r] += partial[(size_t)(k + 8 * r) * n + i];
+        for (int r = 0; k < nsplit; k += 8, r++) s[r] += partial[(size_t)k * n + i];
+    }
+    sh[q][j] = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();
     if (q == 0 && i < n) {
         float v = sh[0][j];
@@ -545,42 +589,78 @@ __global__ void k_to_bf16(const float *src, size_t n, bf16 *dst)
     if (i < n) dst[i] = f2bf(src[i]);
 }
 
-// 2x2 max pool (NHWC); arg holds the winning child 0..3 per output element
+// The elementwise kernels below move 8 channels (one 16-byte vector) per thread: every channel
+// count, leading dimension and channel offset they are given is a multiple of 8.
+__device__ inline void unpack8(const uint4 &v, float f[8])
+{
+    const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        f[2 * k] = __uint_as_float(u[k] << 16);
+        f[2 * k + 1] = __uint_as_float(u[k] & 0xFFFF0000u);
+    }
+}
+__device__ inline uint4 pack8(const float f[8])
+{
+    uint32_t u[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) u[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    return uint4{u[0], u[1], u[2], u[3]};
+}
+__device__ inline uint4 ld16(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ inline void st16(bf16 *p, const uint4 &v) { *reinterpret_cast<uint4 *>(p) = v; }
+
+// 2x2 max pool (NHWC); arg holds the winning child 0..3 per output element (the first of equal ones)
 __global__ void k_maxpool(const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int Ho = H / 2, Wo = W / 2;
-    const size_t n_out = (size_t)N * Ho * Wo * C;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; // one vector of 8 channels
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const size_t n_out = (size_t)N * Ho * Wo * C8;
     if (i >= n_out) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
-    float best = -3.4e38f;
-    int bi = 0;
+    float best[8], v[8];
+    uint8_t bi[8];
     for (int k = 0; k < 4; k++) {
         const int yy = 2 * yo + k / 2, xx = 2 * xo + k % 2;
-        const float v = bf2f(X[((size_t)(n * H + yy) * W + xx) * C + c]);
-        if (v > best) {
-            best = v;
-            bi = k;
-        }
+        unpack8(ld16(X + ((size_t)(n * H + yy) * W + xx) * C + c), v);
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+            if (k == 0 || v[e] > best[e]) {
+                best[e] = v[e];
+                bi[e] = (uint8_t)k;
+            }
     }
-    Y[i] = f2bf(best);
-    arg[i] = (uint8_t)bi;
+    st16(Y + q * C + c, pack8(best));
+    uint2 packed;
+    packed.x = bi[0] | (bi[1] << 8) | (bi[2] << 16) | ((uint32_t)bi[3] << 24);
+    packed.y = bi[4] | (bi[5] << 8) | (bi[6] << 16) | ((uint32_t)bi[7] << 24);
+    *reinterpret_cast<uint2 *>(arg + q * C + c) = packed;
 }
 
 __global__ void k_maxpool_bwd(const bf16 *dY, const uint8_t *arg, int N, int H, int W, int C, bf16 *dX)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int Ho = H / 2, Wo = W / 2;
-    const size_t n_out = (size_t)N * Ho * Wo * C;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const size_t n_out = (size_t)N * Ho * Wo * C8;
     if (i >= n_out) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int xo = (int)(q % Wo), yo = (int)((q / Wo) % Ho), n = (int)(q / ((size_t)Wo * Ho));
+    const uint4 d = ld16(dY + q * C + c);
+    const uint32_t dw[4] = {d.x, d.y, d.z, d.w};
+    const uint2 am = *reinterpret_cast<const uint2 *>(arg + q * C + c);
     for (int k = 0; k < 4; k++) {
         const int yy = 2 * yo + k / 2, xx = 2 * xo + k % 2;
-        dX[((size_t)(n * H + yy) * W + xx) * C + c] = (k == arg[i]) ? dY[i] : (bf16)0;
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) { // channels 2j, 2j + 1: keep each half whose winner is child k
+            const uint32_t word = j < 2 ? am.x : am.y;
+            const uint32_t lo = (word >> (16 * (j & 1))) & 0xFFu, hi = (word >> (16 * (j & 1) + 8)) & 0xFFu;
+            o[j] = (lo == (uint32_t)k ? (dw[j] & 0xFFFFu) : 0u) | (hi == (uint32_t)k ? (dw[j] & 0xFFFF0000u) : 0u);
+        }
+        st16(dX + ((size_t)(n * H + yy) * W + xx) * C + c, uint4{o[0], o[1], o[2], o[3]});
     }
 }
 
@@ -588,41 +668,54 @@ __global__ void k_maxpool_bwd(const bf16 *dY, const uint8_t *arg, int N, int H, 
 __global__ void k_upsample(const bf16 *X, int N, int H, int W, int C, bf16 *Y, int ld, int off)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n_out = (size_t)N * 2 * H * 2 * W * C;
+    const int C8 = C / 8;
+    const size_t n_out = (size_t)N * 2 * H * 2 * W * C8;
     if (i >= n_out) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int x = (int)(q % (2 * W)), y = (int)((q / (2 * W)) % (2 * H)), n = (int)(q / ((size_t)4 * W * H));
-    Y[q * ld + off + c] = X[((size_t)(n * H + y / 2) * W + x / 2) * C + c];
+    st16(Y + q * ld + off + c, ld16(X + ((size_t)(n * H + y / 2) * W + x / 2) * C + c));
 }
 
 // dX[n][y][x][c] = sum of the four children of channels [off, off+C) of dY
 __global__ void k_upsample_bwd(const bf16 *dY, int ld, int off, int N, int H, int W, int C, bf16 *dX)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n_in = (size_t)N * H * W * C;
+    const int C8 = C / 8;
+    const size_t n_in = (size_t)N * H * W * C8;
     if (i >= n_in) return;
-    const int c = (int)(i % C);
-    const size_t q = i / C;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int x = (int)(q % W), y = (int)((q / W) % H), n = (int)(q / ((size_t)W * H));
-    float s = 0.f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, v[8];
     for (int k = 0; k < 4; k++) {
         const size_t pc = (size_t)(n * 2 * H + 2 * y + k / 2) * (2 * W) + 2 * x + k % 2;
-        s += bf2f(dY[pc * ld + off + c]);
+        unpack8(ld16(dY + pc * ld + off + c), v);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += v[e];
     }
-    dX[i] = f2bf(s);
+    st16(dX + q * C + c, pack8(s));
 }
 
 // channel-slice copy / accumulate: dst[p][doff+c] (+)= src[p][soff+c]
 __global__ void k_copy_ch(const bf16 *src, int lds, int soff, bf16 *dst, int ldd, int doff, int C, int P, int add)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)P * C) return;
-    const int c = (int)(i % C);
-    const size_t p = i / C;
-    float v = bf2f(src[p * lds + soff + c]);
-    if (add) v += bf2f(dst[p * ldd + doff + c]);
-    dst[p * ldd + doff + c] = f2bf(v);
+    const int C8 = C / 8;
+    if (i >= (size_t)P * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const size_t p = i / C8;
+    const uint4 v = ld16(src + p * lds + soff + c);
+    if (!add) {
+        st16(dst + p * ldd + doff + c, v);
+        return;
+    }
+    float a[8], b[8];
+    unpack8(v, a);
+    unpack8(ld16(dst + p * ldd + doff + c), b);
+#pragma unroll
+    for (int e = 0; e < 8; e++) a[e] += b[e];
+    st16(dst + p * ldd + doff + c, pack8(a));
 }
 
 // mean squared error over P pixels x C channels of an FP32 output; dY in BF16
@@ -692,7 +785,7 @@ void conv(hipStream_t s, const ConvArgs &a)
 {
     const int P = a.N * a.H * a.W;
     if (a.ksize == 3) {
-        const unsigned tiles = (unsigned)(a.N * ((a.H + CT_H - 1) / CT_H) * ((a.W + CT_W - 1) / CT_W));
+        const unsigned tiles = (unsigned)(a.N * conv_tiles_y(a.H) * conv_tiles_x(a.W));
         if (a.Cout % 64 == 0)
             hipLaunchKernelGGL((k_conv3_tile<64>), dim3(tiles, a.Cout / 64), dim3(256), 0, s, a);
         else
@@ -735,24 +828,24 @@ void to_bf16(hipStream_t s, const float *src, size_t n, bf16 *dst)
 }
 void maxpool(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg)
 {
-    hipLaunchKernelGGL(k_maxpool, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, X, N, H, W, C, Y, arg);
+    hipLaunchKernelGGL(k_maxpool, g1((size_t)N * (H / 2) * (W / 2) * (C / 8)), dim3(256), 0, s, X, N, H, W, C, Y, arg);
 }
 void maxpool_bwd(hipStream_t s, const bf16 *dY, const uint8_t *arg, int N, int H, int W, int C, bf16 *dX)
 {
-    hipLaunchKernelGGL(k_maxpool_bwd, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, s, dY, arg, N, H, W, C,
+    hipLaunchKernelGGL(k_maxpool_bwd, g1((size_t)N * (H / 2) * (W / 2) * (C / 8)), dim3(256), 0, s, dY, arg, N, H, W, C,
                        dX);
 }
 void upsample(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, int ld, int off)
 {
-    hipLaunchKernelGGL(k_upsample, g1((size_t)N * 4 * H * W * C), dim3(256), 0, s, X, N, H, W, C, Y, ld, off);
+    hipLaunchKernelGGL(k_upsample, g1((size_t)N * 4 * H * W * (C / 8)), dim3(256), 0, s, X, N, H, W, C, Y, ld, off);
 }
 void upsample_bwd(hipStream_t s, const bf16 *dY, int ld, int off, int N, int H, int W, int C, bf16 *dX)
 {
-    hipLaunchKernelGGL(k_upsample_bwd, g1((size_t)N * H * W * C), dim3(256), 0, s, dY, ld, off, N, H, W, C, dX);
+    hipLaunchKernelGGL(k_upsample_bwd, g1((size_t)N * H * W * (C / 8)), dim3(256), 0, s, dY, ld, off, N, H, W, C, dX);
 }
 void copy_ch(hipStream_t s, const bf16 *src, int lds, int soff, bf16 *dst, int ldd, int doff, int C, int P, int add)
 {
-    hipLaunchKernelGGL(k_copy_ch, g1((size_t)P * C), dim3(256), 0, s, src, lds, soff, dst, ldd, doff, C, P, add);
+    hipLaunchKernelGGL(k_copy_ch, g1((size_t)P * (C / 8)), dim3(256), 0, s, src, lds, soff, dst, ldd, doff, C, P, add);
 }
 int mse(hipStream_t s, const float *Y, int ldy, const float *T, int C, int P, bf16 *dY, int ldg, float *partial)
 {
