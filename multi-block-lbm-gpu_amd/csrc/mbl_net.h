@@ -42,6 +42,23 @@ int wgrad_splits(int N, int H, int W, int Cin, int Cout, int ksize);
 void conv_wgrad(hipStream_t s, const WgradArgs &a, int nsplit);
 void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, float *out, int accumulate);
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb);
+
+// the flipped BF16 weights of every layer in one launch
+constexpr int FLIP_MAX = 16;
+struct FlipLayer {
+    size_t woff;  // offset of the layer's weights in W and Wb
+    size_t start; // first index of the layer in the concatenation of all layers' weights
+    int cout, taps, cin;
+};
+struct FlipArgs {
+    FlipLayer L[FLIP_MAX];
+    int n;
+    size_t total;
+};
+void flip_weights_all(hipStream_t s, const float *W, const FlipArgs &fa, bf16 *Wb);
+// dst[k][e] = src[order[b0 + k]][e], for the inputs and the targets of one batch
+void gather_samples(hipStream_t s, const float *src_in, const float *src_t, const int *order, int b0, int batch,
+                    size_t per_in, size_t per_t, float *dst_in, float *dst_t);
 void to_bf16(hipStream_t s, const float *src, size_t n, bf16 *dst);
 void maxpool(hipStream_t s, const bf16 *X, int N, int H, int W, int C, bf16 *Y, uint8_t *arg);
 void maxpool_bwd(hipStream_t s, const bf16 *dY, const uint8_t *arg, int N, int H, int W, int C, bf16 *dX);

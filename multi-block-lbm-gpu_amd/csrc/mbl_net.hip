@@ -228,7 +228,14 @@ static int layer_level(int i) { return i < 8 ? i / 2 : (i < 14 ? 2 - (i - 8) / 2
 static void refresh_weights(mbl_net *n)
 {
     to_bf16(n->s, n->w, n->nparam, n->wb);
-    for (const Layer &l : n->L) flip_weights(n->s, n->w + l.woff, l.cout, l.ksize * l.ksize, l.cin, n->wf + l.woff);
+    FlipArgs fa;
+    memset(&fa, 0, sizeof(fa));
+    for (const Layer &l : n->L) { // mbl_net_create always builds the same 15 layers (FLIP_MAX = 16)
+        fa.L[fa.n] = FlipLayer{l.woff, fa.total, l.cout, l.ksize * l.ksize, l.cin};
+        fa.total += l.nw();
+        fa.n++;
+    }
+    flip_weights_all(n->s, n->w, fa, n->wf);
 }
 
 static void conv_fwd(mbl_net *n, int li, int N, const bf16 *X, int ldx, int xoff, bf16 *Y, int ldy, int yoff, bool relu,
@@ -426,6 +433,12 @@ extern "C" int mbl_net_train(mbl_net_t n, const float *in, const float *target, 
         (void)hipFree(din);
         return MBL_E_NOMEM;
     }
+    int *dord = nullptr; // the epoch's sample order, read by the batch gather
+    if (hipMalloc(&dord, sizeof(int) * nsamples) != hipSuccess) {
+        (void)hipFree(din);
+        (void)hipFree(dt);
+        return MBL_E_NOMEM;
+    }
     int rc = 0;
     if (hipMemcpy(din, in, sizeof(float) * P1 * n->in_ch * nsamples, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dt, target, sizeof(float) * P1 * n->out_ch * nsamples, hipMemcpyHostToDevice) != hipSuccess)
@@ -443,16 +456,18 @@ extern "C" int mbl_net_train(mbl_net_t n, const float *in, const float *target, 
         }
         double sum = 0.0;
         int nb = 0;
+        // the previous epoch's gathers have read the old order before it is replaced
+        if (hipStreamSynchronize(n->s) != hipSuccess ||
+            hipMemcpy(dord, order.data(), sizeof(int) * nsamples, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = MBL_E_HIP;
+            break;
+        }
         for (int b0 = 0; b0 + batch <= nsamples && !rc; b0 += batch) {
-            for (int k = 0; k < batch && !rc; k++) {
-                const int sidx = order[b0 + k];
-                if (hipMemcpyAsync(n->din + P1 * n->in_ch * k, din + P1 * n->in_ch * sidx,
-                                   sizeof(float) * P1 * n->in_ch, hipMemcpyDeviceToDevice, n->s) != hipSuccess ||
-                    hipMemcpyAsync(n->dtarget + P1 * n->out_ch * k, dt + P1 * n->out_ch * sidx,
-                                   sizeof(float) * P1 * n->out_ch, hipMemcpyDeviceToDevice, n->s) != hipSuccess)
-                    rc = MBL_E_HIP;
+            gather_samples(n->s, din, dt, dord, b0, batch, P1 * n->in_ch, P1 * n->out_ch, n->din, n->dtarget);
+            if (hipGetLastError() != hipSuccess) { // a batch that was not staged is not trained on
+                rc = MBL_E_HIP;
+                break;
             }
-            if (rc) break; // a batch that was not staged whole is not trained on
             refresh_weights(n);
             forward(n, batch);
             double l = 0.0;
@@ -466,5 +481,6 @@ extern "C" int mbl_net_train(mbl_net_t n, const float *in, const float *target, 
     if (hipStreamSynchronize(n->s) != hipSuccess) rc = MBL_E_HIP;
     (void)hipFree(din);
     (void)hipFree(dt);
+    (void)hipFree(dord);
     return rc;
 }

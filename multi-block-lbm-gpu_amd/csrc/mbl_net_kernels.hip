@@ -583,6 +583,30 @@ __global__ void k_flip_weights(const float *W, int Cout, int taps, int Cin, bf16
     Wb[((size_t)ci * taps + (taps - 1 - tap)) * Cout + co] = f2bf(W[i]);
 }
 
+__global__ void k_flip_weights_all(const float *W, FlipArgs fa, bf16 *Wb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fa.total) return;
+    int l = 0;
+    while (l + 1 < fa.n && i >= fa.L[l + 1].start) l++;
+    const FlipLayer &L = fa.L[l];
+    const size_t j = i - L.start;
+    const int ci = (int)(j % L.cin);
+    const int tap = (int)((j / L.cin) % L.taps);
+    const int co = (int)(j / ((size_t)L.cin * L.taps));
+    Wb[L.woff + ((size_t)ci * L.taps + (L.taps - 1 - tap)) * L.cout + co] = f2bf(W[L.woff + j]);
+}
+
+__global__ void k_gather_samples(const float *src_in, const float *src_t, const int *order, int b0, size_t per_in,
+                                 size_t per_t, float *dst_in, float *dst_t)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const size_t sidx = (size_t)order[b0 + k];
+    if (e < per_in) dst_in[k * per_in + e] = src_in[sidx * per_in + e];
+    if (e < per_t) dst_t[k * per_t + e] = src_t[sidx * per_t + e];
+}
+
 __global__ void k_to_bf16(const float *src, size_t n, bf16 *dst)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -821,6 +845,17 @@ void reduce_partials(hipStream_t s, const float *partial, int nsplit, size_t n, 
 void flip_weights(hipStream_t s, const float *W, int Cout, int taps, int Cin, bf16 *Wb)
 {
     hipLaunchKernelGGL(k_flip_weights, g1((size_t)Cout * taps * Cin), dim3(256), 0, s, W, Cout, taps, Cin, Wb);
+}
+void flip_weights_all(hipStream_t s, const float *W, const FlipArgs &fa, bf16 *Wb)
+{
+    hipLaunchKernelGGL(k_flip_weights_all, g1(fa.total), dim3(256), 0, s, W, fa, Wb);
+}
+void gather_samples(hipStream_t s, const float *src_in, const float *src_t, const int *order, int b0, int batch,
+                    size_t per_in, size_t per_t, float *dst_in, float *dst_t)
+{
+    const size_t per = std::max(per_in, per_t);
+    hipLaunchKernelGGL(k_gather_samples, dim3((unsigned)((per + 255) / 256), (unsigned)batch), dim3(256), 0, s, src_in,
+                       src_t, order, b0, per_in, per_t, dst_in, dst_t);
 }
 void to_bf16(hipStream_t s, const float *src, size_t n, bf16 *dst)
 {
