@@ -4,14 +4,15 @@
 Workload: one periodic 16384 x 16384 block, rho = 1, (u, v) ~ N(0, 0.01^2)
 from a fixed seed, tau = 0.6; one step = one fused pull-stream + BGK update of
 all nodes, F planes in HBM the whole time. The library takes two steps per
-launch on this block (k_periodic2: step n+1 of a tile and a one-node ring in
-LDS, then step n+2; the same bits as two single steps), so K steps are K/2
-launches. MLUPS = nodes * steps / wall seconds. The roofline figure uses the
-algorithmic bytes of one launch, 72 B per node for FP32 planes (9 loads of
-step n + 9 stores of step n+2, 4 B each; two node updates), against the
-8 TB/s HBM peak, with the per-launch time taken from HIP events on the
-library's stream. The single-step kernel (k_stream_collide, 72 B per node
-update) is timed beside it (`single_step`).
+launch on this block with FP32 planes (k_periodic3: step n+1 of a tile and a
+two-node ring, then step n+2 of the tile and a one-node ring, in LDS, then
+step n+3; the same bits as three single steps), so K steps are K/3 launches
+(16-bit planes: two steps per launch, k_periodic2). MLUPS = nodes * steps /
+wall seconds. The roofline figure uses the algorithmic bytes of one launch,
+72 B per node for FP32 planes (9 loads of step n + 9 stores of step n+3, 4 B
+each; three node updates), against the 8 TB/s HBM peak, with the per-launch
+time taken from HIP events on the library's stream. The single-step kernel
+(k_stream_collide, 72 B per node update) is timed beside it (`single_step`).
 
 FP32 planes are updated in FP64 arithmetic (same per-launch time as FP32
 arithmetic on this HBM-bound kernel; `dtype` names the arithmetic, `storage`
@@ -54,11 +55,13 @@ def traffic_from_profile(storage, kernel="k_stream_collide"):
     return d.get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
 
 
-def gpu_run(lib, n, steps, warmup, storage, tuning=None, pairs=True):
+def gpu_run(lib, n, steps, warmup, storage, tuning=None, multi=True):
+    """multi: the library's default of several steps per launch (three for 4-byte planes, two
+    for 16-bit planes); else one step per launch."""
     import mbl
 
     blk = mbl.PeriodicBlock(lib, n, n, tau=0.6, storage=storage, tuning=tuning)
-    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, 1 if pairs else 0), "step pairs")
+    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, 3 if multi else 0), "steps per launch")
     m0 = blk.moments()  # mass, momentum and non-finite count of the initial field (device sums)
     # bring the GPU clocks up first: untimed steps for at least half a second, then the W warmup steps
     t0 = time.perf_counter()
@@ -173,7 +176,7 @@ def converged_cd():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=1200)  # a multiple of 3 and 2: whole launches
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--storage", type=int, default=0, help="0 f32, 1 f32 shifted, 2 f16 shifted, 3 bf16 shifted, 4 1-4-11 half shifted, 5 1-3-12 half shifted")
@@ -202,8 +205,11 @@ def main():
     r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage, tuning)
     bpn = BYTES_PER_NODE[args.storage]  # algorithmic bytes per node of one launch (either kernel)
     achieved = bpn * r["nodes"] / (r["per_launch_ms"] * 1e-3) / 1e9
-    paired = r["steps_per_launch"] > 1.5
-    single = gpu_run(lib, args.n, max(100, args.steps // 5), args.warmup, args.storage, tuning, pairs=False)
+    spl = r["steps_per_launch"]
+    kernel = ("k_periodic3" if spl > 2.5 else "k_periodic2") if spl > 1.5 else "k_stream_collide"
+    kname = {"k_periodic3": "k_periodic3 (three steps per launch)", "k_periodic2": "k_periodic2 (two steps per launch)",
+             "k_stream_collide": "k_stream_collide (periodic)"}[kernel]
+    single = gpu_run(lib, args.n, max(100, args.steps // 5), args.warmup, args.storage, tuning, multi=False)
     single_achieved = bpn * single["nodes"] / (single["per_launch_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -220,14 +226,15 @@ def main():
         "data": "synthetic (seeded random velocity field)",
         "config": {"workload": f"periodic {args.n}x{args.n} D2Q9 BGK block, tau=0.6, "
                                f"{STORAGE_NAMES[args.storage]} planes, "
-                               + (f"two steps per launch, {bpn} B/node/launch" if paired else f"{bpn} B/node/step"),
+                               + (f"{round(spl)} steps per launch, {bpn} B/node/launch" if spl > 1.5
+                                  else f"{bpn} B/node/step"),
                    "nodes": r["nodes"], "storage": STORAGE_NAMES[args.storage],
                    "arithmetic": "f64" if args.storage in (0, 1) else "f32",
                    "steps_per_launch": round(r["steps_per_launch"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(args.storage, "k_periodic2" if paired else "k_stream_collide"),
-                     "kernel": "k_periodic2 (two steps per launch)" if paired else "k_stream_collide (periodic)",
+                     "traffic": traffic_from_profile(args.storage, kernel),
+                     "kernel": kname,
                      "per_launch_ms": round(r["per_launch_ms"], 5), "bytes_per_node": bpn,
                      "node_updates_per_launch": round(r["steps_per_launch"] * r["nodes"])},
         "single_step": {"kernel": "k_stream_collide (periodic)", "mlups": round(single["mlups"], 1),

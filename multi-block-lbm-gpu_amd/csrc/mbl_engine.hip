@@ -224,7 +224,7 @@ struct mbl_context {
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     bool fork = false; // launches at one position run concurrently (mbl_set_cycle_fork)
-    int step_pairs = 1; // mbl_step_block: periodic all-fluid blocks two steps per launch (its form)
+    int step_pairs = 3; // mbl_step_block: periodic all-fluid blocks two or three steps per launch (its form)
 };
 
 static mbl_context *g_current = nullptr;
@@ -2401,10 +2401,13 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
     // periodic all-fluid blocks: two steps per launch where the shape allows (the state at
     // step n + 2 lands in the other plane set, which then holds it as one step would have
     // left it; the plane set handed back as f holds step n, not n + 1)
-    const bool pairs = h->step_pairs && S.periodic && !S.flags && periodic2_fits(mode, area->nx, area->ny, h->step_pairs);
+    // step_pairs 3: three steps per launch, a remainder of two as one paired launch
+    const bool pairs = h->step_pairs && S.periodic && !S.flags &&
+                       periodic2_fits(mode, area->nx, area->ny, h->step_pairs == 3 ? 1 : h->step_pairs);
+    const bool triples = h->step_pairs == 3 && S.periodic && !S.flags && periodic3_fits(mode, area->nx, area->ny);
     for (int t = 0; t < nsteps;) {
         int rc;
-        if (pairs && t + 2 <= nsteps) {
+        if (triples && t + 3 <= nsteps) {
             Periodic2Args a;
             memset(&a, 0, sizeof(a));
             for (int k = 0; k < 9; k++) {
@@ -2413,7 +2416,20 @@ extern "C" int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int ns
             }
             a.nx = area->nx;
             a.ny = area->ny;
-            a.form = h->step_pairs;
+            a.form = 1;
+            a.omega = S.hd.omega;
+            rc = launch_periodic3(h->stream, mode, a);
+            t += 3;
+        } else if (pairs && t + 2 <= nsteps) {
+            Periodic2Args a;
+            memset(&a, 0, sizeof(a));
+            for (int k = 0; k < 9; k++) {
+                a.S[k] = S.hd.P[par][k];
+                a.D[k] = S.hd.P[par ^ 1][k];
+            }
+            a.nx = area->nx;
+            a.ny = area->ny;
+            a.form = h->step_pairs == 3 ? 1 : h->step_pairs;
             a.omega = S.hd.omega;
             rc = launch_periodic2(h->stream, mode, a);
             t += 2;
@@ -2435,8 +2451,14 @@ extern "C" int mbl_step_block_launches(mbl_handle_t h, const AREA *area, int nst
     if (!h) return MBL_E_NO_HANDLE;
     if (!area || nsteps < 0) return MBL_E_INVALID;
     const Single &S = h->single;
+    const int mode = storage_mode(h->cfg);
     const bool pairs = S.dd && h->step_pairs && S.periodic && !S.flags &&
-                       periodic2_fits(storage_mode(h->cfg), area->nx, area->ny, h->step_pairs);
+                       periodic2_fits(mode, area->nx, area->ny, h->step_pairs == 3 ? 1 : h->step_pairs);
+    const bool triples = S.dd && h->step_pairs == 3 && S.periodic && !S.flags && periodic3_fits(mode, area->nx, area->ny);
+    if (triples) {
+        const int r = nsteps % 3;
+        return nsteps / 3 + (r == 2 && pairs ? 1 : r);
+    }
     return pairs ? nsteps / 2 + nsteps % 2 : nsteps;
 }
 
@@ -2444,7 +2466,7 @@ extern "C" int mbl_set_step_pairs(mbl_handle_t h, int on)
 {
     if (!h) h = g_current;
     if (!h) return MBL_E_NO_HANDLE;
-    if (on < 0 || on > 2) return MBL_E_INVALID;
+    if (on < 0 || on > 3) return MBL_E_INVALID;
     h->step_pairs = on;
     return 0;
 }

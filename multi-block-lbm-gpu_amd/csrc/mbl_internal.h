@@ -239,6 +239,9 @@ struct Periodic2Args {
 };
 int periodic2_fits(int mode, int nx, int ny, int form);
 int launch_periodic2(void *stream, int mode, const Periodic2Args &a);
+// three steps per launch (same arguments; D receives step n + 3)
+int periodic3_fits(int mode, int nx, int ny);
+int launch_periodic3(void *stream, int mode, const Periodic2Args &a);
 int launch_stream_collide(void *stream, int mode, int vx, int tpb, bool periodic, bool flags, bool macros,
                           const BlockDesc *d_descs, const int *d_tile_blk, int ndesc, int total_tiles, int tile_rows,
                           uint32_t parity_mask, int groups, int tile_base, uint32_t rmask = 0,

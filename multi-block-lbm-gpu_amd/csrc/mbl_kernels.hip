@@ -986,6 +986,112 @@ void k_periodic2(Periodic2Args a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Three time steps per launch on a periodic all-fluid block (temporal blocking
+// two deep): a workgroup owns a P2_TX x TY output tile. It collides step n+1
+// on the tile and a two-node ring ((P2_TX + 4) x (TY + 4) nodes, pulling step
+// n from the planes with the periodic wrap) into LDS, then step n+2 on the
+// tile and a one-node ring from there, then step n+3 on the tile, which it
+// writes to the other plane set. Step n+2 is computed in chunks of rows, each
+// held in registers until every thread has read the rows it replaces: it lands
+// in the rows of step n+1 that no later chunk reads, so one LDS buffer serves
+// both steps (49 KB for FP32 planes: three workgroups per CU). Every node
+// update is the single-step kernel's (same loads, collide_c, stores), so the
+// state equals three single steps bit for bit; the plane set handed back as f
+// holds step n. Per launch and output node: 9 loads of step n and 9 stores of
+// step n+3 (72 B FP32 for three updates); the rings are recomputed:
+// (P2_TX + 4)(TY + 4) + (P2_TX + 2)(TY + 2) extra collisions per P2_TX TY outputs.
+// ---------------------------------------------------------------------------
+constexpr int P3_THREADS = 512;
+
+// waves per SIMD asked for: as many workgroups per CU as the LDS holds, at most three
+constexpr int p3_waves(int ty, int elem_bytes)
+{
+    return (9 * (ty + 4) * (P2_TX + 4) * elem_bytes * 3 <= 160 * 1024   ? 3
+            : 9 * (ty + 4) * (P2_TX + 4) * elem_bytes * 2 <= 160 * 1024 ? 2
+                                                                        : 1) *
+           P3_THREADS / 256;
+}
+
+template <int MODE, int TY>
+__global__ __launch_bounds__(P3_THREADS, p3_waves(TY, (int)sizeof(typename Codec<MODE>::T)))
+void k_periodic3(Periodic2Args a)
+{
+    typedef Codec<MODE> C;
+    typedef typename C::T T;
+    typedef typename C::R R;
+    typedef typename C::GT GT;
+    constexpr int AX = P2_TX + 4, AY = TY + 4; // step n+1 region: the tile and a two-node ring
+    constexpr int BX = P2_TX + 2, BY = TY + 2; // step n+2 region: the tile and a one-node ring
+    constexpr int BR = P3_THREADS / BX;        // step n+2 rows per chunk (one node per thread)
+    static_assert(BR >= 1, "a chunk holds at least one row");
+    // L[k][r][c]: step n+1 at node (x0 - 2 + c, y0 - 2 + r); step n+2 of region row j (node row
+    // y0 - 1 + j), region column i (node column x0 - 1 + i) replaces it at L[k][j][i + 1]
+    __shared__ T L[9][AY][AX];
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t unx = (uint32_t)nx;
+    const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int ty = b / a.tiles_x, tx = b - ty * a.tiles_x;
+    const int x0 = tx * P2_TX, y0 = ty * TY;
+    const R omega = (R)a.omega;
+    // step n+1 on the tile and its two-node ring, node by node
+    for (int idx = (int)threadIdx.x; idx < AX * AY; idx += P3_THREADS) {
+        const int ly = idx / AX, lx = idx - ly * AX;
+        int x = x0 - 2 + lx, y = y0 - 2 + ly;
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        R q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            int xs = x - EX[k], ys = y - EY[k];
+            xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+            ys = ys < 0 ? ys + ny : (ys >= ny ? ys - ny : ys);
+            q[k] = C::ld(*off32((const GT *)a.S[k], (uint32_t)ys * unx + (uint32_t)xs));
+        }
+        R Q[9], rho, ux, uy;
+        collide_c<C>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+        for (int k = 0; k < 9; k++) L[k][ly][lx] = C::st(Q[k]);
+    }
+    // step n+2 in chunks of BR region rows: chunk rows j0 .. j1-1 read step n+1 rows j0 .. j1+1
+    // and replace rows j0 .. j1-1, which later chunks (rows from j1 on) do not read
+    for (int j0 = 0; j0 < BY; j0 += BR) {
+        const int j1 = j0 + BR < BY ? j0 + BR : BY;
+        __syncthreads(); // step n+1 rows complete; the previous chunk's rows written
+        const int idx = (int)threadIdx.x;
+        const bool mine = idx < (j1 - j0) * BX;
+        const int j = j0 + idx / BX, i = idx - (idx / BX) * BX;
+        T Qs[9];
+        if (mine) {
+            R q[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) q[k] = C::ld(L[k][j + 1 - EY[k]][i + 1 - EX[k]]);
+            R Q[9], rho, ux, uy;
+            collide_c<C>(q, omega, Q, rho, ux, uy);
+#pragma unroll
+            for (int k = 0; k < 9; k++) Qs[k] = C::st(Q[k]);
+        }
+        __syncthreads(); // every read of rows j0 .. j1+1 is done
+        if (mine) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) L[k][j][i + 1] = Qs[k];
+        }
+    }
+    __syncthreads();
+    // step n+3 on the tile, pulling step n+2 (region row ly + 1 - ey, column lx + 1 - ex)
+    for (int idx = (int)threadIdx.x; idx < P2_TX * TY; idx += P3_THREADS) {
+        const int ly = idx / P2_TX, lx = idx - ly * P2_TX;
+        R q[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) q[k] = C::ld(L[k][ly + 1 - EY[k]][lx + 2 - EX[k]]);
+        R Q[9], rho, ux, uy;
+        collide_c<C>(q, omega, Q, rho, ux, uy);
+        const uint32_t o = (uint32_t)(y0 + ly) * unx + (uint32_t)(x0 + lx);
+#pragma unroll
+        for (int k = 0; k < 9; k++) __builtin_nontemporal_store(C::st(Q[k]), off32((GT *)a.D[k], o));
+    }
+}
+
 // Two time steps per launch, strip form: a workgroup of P2S_W waves walks down a strip of
 // 64 P2S_W - 2 output columns and P2S_H rows. Lane c (64 P2S_W of them) collides step n+1 at
 // column x0 - 1 + c of one row at a time (pulling step n from the planes) into a 3-row window
@@ -2690,6 +2796,31 @@ static bool periodic2_tile_fits(int mode, int nx, int ny)
 int periodic2_fits(int mode, int nx, int ny, int form)
 {
     return (form == 2 && periodic2_strip_fits(nx, ny)) || periodic2_tile_fits(mode, nx, ny);
+}
+
+// three steps per launch for 4-byte planes: 16384^2 FP32 164.6k-166.0k MLUPS against 123.0k-137.4k
+// for pairs (same box); 16-bit planes stay with pairs (1-3-12: 150k against 164k-166k, their
+// update is VALU-bound), 8-byte planes too (two workgroups per CU by LDS)
+int periodic3_fits(int mode, int nx, int ny)
+{
+    return store_elem_bytes(mode) == 4 && periodic2_tile_fits(mode, nx, ny);
+}
+
+int launch_periodic3(void *stream, int mode, const Periodic2Args &args)
+{
+    if (!periodic3_fits(mode, args.nx, args.ny)) return MBL_E_INVALID;
+    Periodic2Args a = args;
+    const int ty = periodic2_rows(mode);
+    a.tiles_x = a.nx / P2_TX;
+    dim3 g(a.tiles_x * (a.ny / ty)), b(P3_THREADS);
+    hipStream_t s = (hipStream_t)stream;
+    if (ty == 8) {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_periodic3<M_, 8>), g, b, 0, s, a));
+    } else {
+        MBL_MODE_SWITCH(mode, hipLaunchKernelGGL((k_periodic3<M_, 16>), g, b, 0, s, a));
+    }
+    MBL_LAUNCH_CHECK();
+    return 0;
 }
 
 int launch_periodic2(void *stream, int mode, const Periodic2Args &args)

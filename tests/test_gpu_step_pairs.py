@@ -1,7 +1,7 @@
-"""mbl_step_block on a periodic all-fluid block: two time steps per launch (the step-n+1 values
-kept in LDS) against one step per launch (mbl_set_step_pairs(h, 0)), every storage mode and the
-FP64 build: the state in F bit for bit after even and odd step counts, and a block whose shape
-does not allow pairs stepping one step per launch either way."""
+"""mbl_step_block on a periodic all-fluid block: two or three time steps per launch (the
+intermediate steps kept in LDS) against one step per launch (mbl_set_step_pairs(h, 0)), every
+storage mode and the FP64 build: the state in F bit for bit after step counts of every remainder,
+and a block whose shape does not allow pairs stepping one step per launch either way."""
 import numpy as np
 import pytest
 
@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 
 
 def states(lib, n, ny, storage, steps, pairs, sigma=0.02):
-    """pairs: 0 single steps, 1 paired tiles, 2 paired strips (where the shape allows)."""
+    """pairs: 0 single steps, 1 paired tiles, 2 paired strips (where the shape allows), 3 three
+    steps per launch."""
     blk = mbl.PeriodicBlock(lib, n, ny, tau=0.6, storage=storage, sigma=sigma)
     mbl.check(lib, lib.mbl_set_step_pairs(blk.h, int(pairs)), "pairs")
     out = []
@@ -55,6 +56,45 @@ def test_pairs_equal_single_steps_fp64(lib64):
     for sa, sb in zip(a, b):
         for k in range(9):
             assert np.array_equal(sa[k], sb[k]), k
+
+
+@pytest.mark.parametrize("storage", [abi.STORE_NATIVE, abi.STORE_F32S, abi.STORE_F16S, abi.STORE_F16D])
+def test_triples_equal_single_steps(lib32, storage):
+    """Mode 3: three steps per launch for 4-byte planes (step n+1 with a two-node ring and step
+    n+2 with a one-node ring in LDS), pairs for 16-bit planes: 3 steps (one launch), 7 (two
+    launches + one single step), 11 (three launches + one pair), after which the F planes hold the
+    single-step state bit for bit."""
+    steps = [3, 7, 11]
+    a = states(lib32, 128, 64, storage, steps, 3)
+    b = states(lib32, 128, 64, storage, steps, 0)
+    for sa, sb in zip(a, b):
+        for k in range(9):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_triples_equal_single_steps_fp64(lib64):
+    """Mode 3 on FP64 planes (pairs, and a single step for odd remainders)."""
+    steps = [3, 8]
+    a = states(lib64, 64, 40, abi.STORE_NATIVE, steps, 3)
+    b = states(lib64, 64, 40, abi.STORE_NATIVE, steps, 0)
+    for sa, sb in zip(a, b):
+        for k in range(9):
+            assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_triples_launch_count(lib32):
+    """FP32 planes: three steps per launch by default, remainders as one pair or one single step;
+    16-bit planes: pairs."""
+    blk = mbl.PeriodicBlock(lib32, 128, 64, tau=0.6, sigma=0.02)
+    blk.step(1)
+    counts = [blk.launches(n) for n in (3, 4, 5, 6, 11)]
+    blk.close()
+    assert counts == [1, 2, 2, 2, 4]
+    blk = mbl.PeriodicBlock(lib32, 128, 64, tau=0.6, storage=abi.STORE_F16D, sigma=0.02)
+    blk.step(1)
+    counts = [blk.launches(n) for n in (3, 4, 6)]
+    blk.close()
+    assert counts == [2, 2, 3]
 
 
 def test_shape_without_pairs(lib32):
