@@ -55,13 +55,16 @@ def traffic_from_profile(storage, kernel="k_stream_collide"):
     return d.get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
 
 
-def gpu_run(lib, n, steps, warmup, storage, tuning=None, multi=True):
-    """multi: the library's default of several steps per launch (three for 4-byte planes, two
-    for 16-bit planes); else one step per launch."""
+LAUNCH_MODES = {3: 3, 2: 1, 1: 0}  # --launch-steps -> mbl_set_step_pairs mode
+
+
+def gpu_run(lib, n, steps, warmup, storage, tuning=None, launch_steps=3):
+    """launch_steps 3: the library's default (three steps per launch for 4-byte planes, two for
+    16-bit planes); 2: two steps per launch; 1: one step per launch."""
     import mbl
 
     blk = mbl.PeriodicBlock(lib, n, n, tau=0.6, storage=storage, tuning=tuning)
-    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, 3 if multi else 0), "steps per launch")
+    mbl.check(lib, lib.mbl_set_step_pairs(blk.h, LAUNCH_MODES[launch_steps]), "steps per launch")
     m0 = blk.moments()  # mass, momentum and non-finite count of the initial field (device sums)
     # bring the GPU clocks up first: untimed steps for at least half a second, then the W warmup steps
     t0 = time.perf_counter()
@@ -184,6 +187,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-converge", action="store_true", help="skip the time-to-converged-Cd runs (about 40 s)")
     ap.add_argument("--variants", action="store_true", help="also time the 16-bit storage modes")
+    ap.add_argument("--launch-steps", type=int, default=3, choices=[1, 2, 3],
+                    help="steps per launch on the periodic block: 3 = the library default (three for 4-byte "
+                         "planes, two for 16-bit planes), 2 = two, 1 = one")
     ap.add_argument("--vx", type=int, default=0, help="nodes per thread (0: library default)")
     ap.add_argument("--tpb", type=int, default=0, help="threads per workgroup (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -202,14 +208,14 @@ def main():
     print(f"# bench: periodic {args.n}^2 storage={STORAGE_NAMES[args.storage]} steps={args.steps}", file=sys.stderr,
           flush=True)
     tuning = (args.vx, args.tpb) if (args.vx or args.tpb) else None
-    r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage, tuning)
+    r = gpu_run(lib, args.n, args.steps, args.warmup, args.storage, tuning, args.launch_steps)
     bpn = BYTES_PER_NODE[args.storage]  # algorithmic bytes per node of one launch (either kernel)
     achieved = bpn * r["nodes"] / (r["per_launch_ms"] * 1e-3) / 1e9
     spl = r["steps_per_launch"]
     kernel = ("k_periodic3" if spl > 2.5 else "k_periodic2") if spl > 1.5 else "k_stream_collide"
     kname = {"k_periodic3": "k_periodic3 (three steps per launch)", "k_periodic2": "k_periodic2 (two steps per launch)",
              "k_stream_collide": "k_stream_collide (periodic)"}[kernel]
-    single = gpu_run(lib, args.n, max(100, args.steps // 5), args.warmup, args.storage, tuning, multi=False)
+    single = gpu_run(lib, args.n, max(100, args.steps // 5), args.warmup, args.storage, tuning, launch_steps=1)
     single_achieved = bpn * single["nodes"] / (single["per_launch_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
