@@ -146,10 +146,11 @@ template <int COT> __global__ __launch_bounds__(256) void k_conv(ConvArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution on 2D pixel tiles: a workgroup computes an 8 x 16 pixel tile
-// of one image for COT output channels. Per block of 32 input channels it
-// stages the tile's 10 x 18 halo of X and the COT x 9 x 32 weights in LDS, and
-// each wave runs 9 taps x 2 pixel rows x COT/16 MFMAs from there, so an
+// 3x3 convolution on 2D pixel tiles: a workgroup computes an 8 x TW pixel tile
+// of one image for COT output channels (TW = 16 on 4 waves, or 32 on 8 waves).
+// Per block of 32 input channels it stages the tile's 10 x (TW + 2) halo of X
+// and the COT x 9 x 32 weights in LDS, and each wave runs 9 taps x 2 groups of
+// 16 pixels x COT/16 MFMAs from there, so an
 // activation is read from memory once per channel block instead of once per
 // tap, and two barriers cover 9 taps. The next block's global loads are in
 // flight during this block's MFMAs (held in registers). Same epilogue as
@@ -162,37 +163,41 @@ constexpr int CT_LD = 40;                          // LDS row: 32 channels + 8 (
 __host__ __device__ inline int conv_tiles_x(int W) { return (W + CT_W - 1) / CT_W; }
 __host__ __device__ inline int conv_tiles_y(int H) { return (H + CT_H - 1) / CT_H; }
 
-template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvArgs a)
+template <int COT, int TW> __global__ __launch_bounds__(256 * TW / 16, 2) void k_conv3_tile(ConvArgs a)
 {
+    constexpr int NT = 256 * TW / 16;       // threads: 4 waves per 16 tile columns
+    constexpr int HW = TW + 2;              // halo width
     constexpr int NC = COT / 16;
-    constexpr int NXV = CT_HH * CT_HW * 4;  // 16-byte vectors of one halo block (720)
-    constexpr int NXL = (NXV + 255) / 256;  // per thread
+    constexpr int NXV = CT_HH * HW * 4;     // 16-byte vectors of one halo block
+    constexpr int NXL = (NXV + NT - 1) / NT; // per thread
     constexpr int NWV = COT * 9 * 4;        // 16-byte vectors of one weight block
-    constexpr int NWL = (NWV + 255) / 256;
-    __shared__ __attribute__((aligned(16))) bf16 sX[CT_HH * CT_HW * CT_LD];
+    constexpr int NWL = (NWV + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) bf16 sX[CT_HH * HW * CT_LD];
     __shared__ __attribute__((aligned(16))) bf16 sW[COT * 9 * CT_LD];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int g = lane >> 4, r16 = lane & 15;
-    const int tx_n = conv_tiles_x(a.W), ty_n = conv_tiles_y(a.H);
+    const int tx_n = (a.W + TW - 1) / TW, ty_n = conv_tiles_y(a.H);
     int b = blockIdx.x;
     const int n = b / (tx_n * ty_n);
     b -= n * tx_n * ty_n;
-    const int y0 = (b / tx_n) * CT_H, x0 = (b % tx_n) * CT_W;
+    const int y0 = (b / tx_n) * CT_H, x0 = (b % tx_n) * TW;
     const int c0 = blockIdx.y * COT;
+    // MFMA row group q = 2w + h: tile row q / (TW / 16), columns 16 (q % (TW / 16)) .. + 15
+    constexpr int CG = TW / 16;
     uint4 xv[NXL], wv[NWL];
     // register arrays indexed by compile-time constants only (static_for), so they stay in registers
     auto fetch = [&](int ci) __attribute__((always_inline)) {
         static_for<NXL>([&](auto m) __attribute__((always_inline)) {
-            const int i = t + 256 * m;
+            const int i = t + NT * m;
             xv[m] = uint4{0u, 0u, 0u, 0u};
             const int hp = i >> 2, part = i & 3;
-            const int yy = y0 - 1 + hp / CT_HW, xx = x0 - 1 + hp % CT_HW;
+            const int yy = y0 - 1 + hp / HW, xx = x0 - 1 + hp % HW;
             if (i < NXV && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
                 xv[m] = *reinterpret_cast<const uint4 *>(a.X + ((size_t)(n * a.H + yy) * a.W + xx) * a.ldx + a.xoff + ci +
                                                          8 * part);
         });
         static_for<NWL>([&](auto m) __attribute__((always_inline)) {
-            const int i = t + 256 * m;
+            const int i = t + NT * m;
             wv[m] = uint4{0u, 0u, 0u, 0u};
             if (i < NWV) // row = local output channel * 9 + tap
                 wv[m] = *reinterpret_cast<const uint4 *>(a.Wt + (size_t)(c0 * 9 + (i >> 2)) * a.Cin + ci + 8 * (i & 3));
@@ -207,11 +212,11 @@ template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvAr
     for (int ci = 0; ci < a.Cin; ci += 32) {
         __syncthreads(); // the previous block's reads of sX and sW are done
         static_for<NXL>([&](auto m) __attribute__((always_inline)) {
-            const int i = t + 256 * m;
+            const int i = t + NT * m;
             if (i < NXV) *reinterpret_cast<uint4 *>(sX + (i >> 2) * CT_LD + 8 * (i & 3)) = xv[m];
         });
         static_for<NWL>([&](auto m) __attribute__((always_inline)) {
-            const int i = t + 256 * m;
+            const int i = t + NT * m;
             if (i < NWV) *reinterpret_cast<uint4 *>(sW + (i >> 2) * CT_LD + 8 * (i & 3)) = wv[m];
         });
         __syncthreads();
@@ -221,7 +226,8 @@ template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvAr
             const int ty = tap / 3, tx = tap % 3; // halo offsets of the tap
             bf16x8 av[2];
 #pragma unroll
-            for (int h = 0; h < 2; h++) av[h] = load8(sX + ((2 * w + h + ty) * CT_HW + r16 + tx) * CT_LD + 8 * g);
+            for (int h = 0; h < 2; h++)
+                av[h] = load8(sX + (((2 * w + h) / CG + ty) * HW + 16 * ((2 * w + h) % CG) + r16 + tx) * CT_LD + 8 * g);
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 const bf16x8 bv = load8(sW + ((16 * c + r16) * 9 + tap) * CT_LD + 8 * g);
@@ -230,11 +236,12 @@ template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvAr
             }
         }
     }
-    // acc[h][c][r] = out[row y0 + 2w + h, column x0 + 4g + r][channel c0 + 16c + r16]: rounded to BF16
+    // acc[h][c][r] = out[row y0 + (2w + h) / CG, column x0 + 16 ((2w + h) % CG) + 4g + r][channel c0 + 16c + r16]:
+    // rounded to BF16
     // with the bias into LDS, then written (mask and ReLU applied) as 16-byte vectors of 8 channels;
     // the mask and ReLU commute with the rounding. The FP32 output, when asked for, is stored directly.
     constexpr int OLD = COT + 8; // LDS row of the output tile: one pixel
-    static_assert(CT_H * CT_W * OLD <= COT * 9 * CT_LD, "output tile fits in the weight buffer");
+    static_assert(CT_H * TW * OLD <= COT * 9 * CT_LD, "output tile fits in the weight buffer");
     bf16 *sO = sW;
     __syncthreads(); // every wave is done with sW
 #pragma unroll
@@ -245,11 +252,11 @@ template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvAr
             const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int p = (2 * w + h) * CT_W + 4 * g + r;
+                const int p = ((2 * w + h) / CG) * TW + 16 * ((2 * w + h) % CG) + 4 * g + r;
                 const float v = acc[h][c][r] + bias;
                 sO[p * OLD + 16 * c + r16] = f2bf(v);
                 if (a.Yf) {
-                    const int yy = y0 + 2 * w + h, xx = x0 + 4 * g + r;
+                    const int yy = y0 + p / TW, xx = x0 + p % TW;
                     if (yy < a.H && xx < a.W) {
                         float u = v;
                         const size_t pix = (size_t)(n * a.H + yy) * a.W + xx;
@@ -262,11 +269,11 @@ template <int COT> __global__ __launch_bounds__(256, 2) void k_conv3_tile(ConvAr
         }
     if (!a.Y) return;
     __syncthreads();
-    constexpr int NV = CT_H * CT_W * COT / 8 / 256; // output vectors per thread
+    constexpr int NV = CT_H * TW * COT / 8 / NT; // output vectors per thread
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-        const int i = t + 256 * m, p = i / (COT / 8), c8 = 8 * (i % (COT / 8));
-        const int yy = y0 + p / CT_W, xx = x0 + p % CT_W;
+        const int i = t + NT * m, p = i / (COT / 8), c8 = 8 * (i % (COT / 8));
+        const int yy = y0 + p / TW, xx = x0 + p % TW;
         if (yy >= a.H || xx >= a.W) continue;
         const size_t pix = (size_t)(n * a.H + yy) * a.W + xx;
         uint4 v = *reinterpret_cast<const uint4 *>(sO + p * OLD + c8);
@@ -809,11 +816,23 @@ void conv(hipStream_t s, const ConvArgs &a)
 {
     const int P = a.N * a.H * a.W;
     if (a.ksize == 3) {
-        const unsigned tiles = (unsigned)(a.N * conv_tiles_y(a.H) * conv_tiles_x(a.W));
-        if (a.Cout % 64 == 0)
-            hipLaunchKernelGGL((k_conv3_tile<64>), dim3(tiles, a.Cout / 64), dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_conv3_tile<32>), dim3(tiles, a.Cout / 32), dim3(256), 0, s, a);
+        // 8 x 32 tiles on 8 waves where the width is a multiple of 32 (twice the pixels per staged
+        // weight block, twice the waves per CU), else 8 x 16 tiles on 4 waves; the same sums in
+        // the same order either way
+        const bool wide = a.W % 32 == 0;
+        const unsigned tiles = (unsigned)(a.N * conv_tiles_y(a.H) * ((a.W + (wide ? 31 : 15)) / (wide ? 32 : 16)));
+        const dim3 blk(wide ? 512 : 256);
+        if (a.Cout % 64 == 0) {
+            if (wide)
+                hipLaunchKernelGGL((k_conv3_tile<64, 32>), dim3(tiles, a.Cout / 64), blk, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_conv3_tile<64, 16>), dim3(tiles, a.Cout / 64), blk, 0, s, a);
+        } else {
+            if (wide)
+                hipLaunchKernelGGL((k_conv3_tile<32, 32>), dim3(tiles, a.Cout / 32), blk, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_conv3_tile<32, 16>), dim3(tiles, a.Cout / 32), blk, 0, s, a);
+        }
         return;
     }
     if (a.Cout % 64 == 0)
