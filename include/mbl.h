@@ -438,8 +438,8 @@ int mbl_poiseuille_l2(mbl_handle_t h, Lbi N_areas, AREA *areas, D2Q9 *pps_d, dou
  * (no interface work), swapping pp->F / pp->f after every launch. This is the
  * path of the single-block periodic bandwidth benchmark. A periodic all-fluid
  * block whose width is a multiple of 64 and height a multiple of 16 (8 for
- * FP64 planes) takes three steps per launch (4-byte planes) or two (other
- * planes), a remainder of two as one paired launch and of one as a single step:
+ * FP64 planes) takes three steps per launch (4- and 8-byte planes) or two
+ * (16-bit planes), a remainder of two as one paired launch and of one as a single step:
  * pp->F then holds the same state as after single steps, bit for bit, and pp->f
  * the state one launch back (one step back after single steps). */
 int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
@@ -447,8 +447,8 @@ int mbl_step_block(mbl_handle_t h, const AREA *area, D2Q9 *pp, int nsteps);
  * launch where the shape allows, in tiles with a one-node ring; on == 2: two steps per launch in
  * strips of 254 columns with a 3-row window where the shape allows (width >= 256, height a
  * multiple of 64), else tiles; on == 3 (default): three steps per launch in tiles with a two-node
- * ring for 4-byte planes where the shape allows (as for tiled pairs), a remainder of two steps as
- * one paired launch; other planes as on == 1. Same bits in every case. No reference
+ * ring for 4- and 8-byte planes where the shape allows (as for tiled pairs), a remainder of two
+ * steps as one paired launch; 16-bit planes as on == 1. Same bits in every case. No reference
  * counterpart. */
 int mbl_set_step_pairs(mbl_handle_t h, int on);
 /* Launches mbl_step_block makes for nsteps steps of the block it stepped last (for per-launch

@@ -2798,12 +2798,12 @@ int periodic2_fits(int mode, int nx, int ny, int form)
     return (form == 2 && periodic2_strip_fits(nx, ny)) || periodic2_tile_fits(mode, nx, ny);
 }
 
-// three steps per launch for 4-byte planes: 16384^2 FP32 164.6k-166.0k MLUPS against 123.0k-137.4k
-// for pairs (same box); 16-bit planes stay with pairs (1-3-12: 150k against 164k-166k, their
-// update is VALU-bound), 8-byte planes too (two workgroups per CU by LDS)
+// three steps per launch for 4- and 8-byte planes: 16384^2 FP32 164.6k-166.0k MLUPS against
+// 123.0k-137.4k for pairs (same box), 8192^2 FP64 99.6k-100.3k against 71.8k-73.0k; 16-bit planes
+// stay with pairs (1-3-12: 150k against 164k-166k, their update is VALU-bound)
 int periodic3_fits(int mode, int nx, int ny)
 {
-    return store_elem_bytes(mode) == 4 && periodic2_tile_fits(mode, nx, ny);
+    return store_elem_bytes(mode) >= 4 && periodic2_tile_fits(mode, nx, ny);
 }
 
 int launch_periodic3(void *stream, int mode, const Periodic2Args &args)

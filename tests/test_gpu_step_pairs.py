@@ -73,7 +73,7 @@ def test_triples_equal_single_steps(lib32, storage):
 
 
 def test_triples_equal_single_steps_fp64(lib64):
-    """Mode 3 on FP64 planes (pairs, and a single step for odd remainders)."""
+    """Mode 3 on FP64 planes (three steps per launch, 64 x 8 tiles; a remainder of two as a pair)."""
     steps = [3, 8]
     a = states(lib64, 64, 40, abi.STORE_NATIVE, steps, 3)
     b = states(lib64, 64, 40, abi.STORE_NATIVE, steps, 0)
